@@ -1,0 +1,132 @@
+/*
+ * vfb.h -- C-ABI of the MI355X-native LTL synthesizer (libvfb.so).
+ *
+ * The reference has no library or FFI layer: its boundary is the process
+ * (`ltl_nocache <json_file> <maxLen>`, ltl_nocache.cu:448-487) and, inside,
+ * one monolithic host function `std::string LTLI(...)` (LTL_BS.cu:1069-1079)
+ * that owns every device buffer.  The entry points below are what a binding
+ * of that path needs: plain C types, caller-owned handles, status codes in
+ * place of the reference's sentinel strings ("see_the_error",
+ * LTL_BS.cu:1105-1110; ltl_nocache.cu:326-348, 480-482).
+ *
+ * Threading: one solve call at a time per process-wide device context is not
+ * required; independent calls on different threads are serialised inside
+ * the library.  No exception crosses this boundary.
+ */
+#ifndef VFB_H
+#define VFB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFB_MAX_SIZE 32     /* maximum formula size (ltl_nocache.cu:11) */
+#define VFB_MAX_TRACES 128  /* ltl_nocache allows 100 (ltl_nocache.cu:10), LTL_BS 63 (LTL_BS.cu:13) */
+#define VFB_MAX_ATOMS 64
+#define VFB_MAX_LEN 64      /* one 64-bit CS word per trace (ltl_nocache.cu:12) */
+
+typedef enum {
+    VFB_OK = 0,             /* formula found */
+    VFB_NOT_FOUND = 1,      /* no formula up to max_size ("Not found !", ltl_nocache.cu:439) */
+    VFB_INVALID_INPUT = 2,  /* limits exceeded or malformed input (LTL_BS.cu:1105-1110) */
+    VFB_CAPACITY = 3,       /* HBM budget, cache or hash set exhausted (never silent) */
+    VFB_DEVICE_ERROR = 4,   /* HIP runtime error, or the HIP library is unusable */
+    VFB_IO_ERROR = 5        /* file could not be read or written */
+} vfb_status;
+
+/* operator codes in the reference's segment order (LTL_BS.cu:17) */
+typedef enum {
+    VFB_OP_NOT = 0, VFB_OP_AND = 1, VFB_OP_OR = 2, VFB_OP_NEXT = 3,
+    VFB_OP_FINALLY = 4, VFB_OP_GLOBALLY = 5, VFB_OP_UNTIL = 6, VFB_OP_ATOM = 7
+} vfb_op;
+
+typedef struct vfb_instance vfb_instance;
+
+/* ---- instances --------------------------------------------------------- */
+
+/* Reads the reference's JSON trace format (replaces readJsonFile,
+   ltl_nocache.cu:35-76).  Positive traces come first. */
+vfb_status vfb_instance_load_json(const char* path, vfb_instance** out);
+
+/* Builds an instance from bit-packed traces: atoms[a * T + t] bit k is the
+   value of AP a at step k of trace t; traces 0..num_pos-1 are positive
+   (the layout the reference builds at LTL_BS.cu:1127-1163). */
+vfb_status vfb_instance_from_bits(int num_atoms, const char* const* atom_names,
+                                  int num_pos, int num_neg, const int* lens,
+                                  const uint64_t* atoms, vfb_instance** out);
+
+/* Seeded synthetic instance: config 1 plain, 2 dedup stress, 3 limits,
+   4 batched mix (5, the served stream, draws from config 4).  Instance k of
+   config c uses SplitMix64 seed (c << 32) | k.  No reference counterpart. */
+vfb_status vfb_instance_generate(int config, uint64_t index, vfb_instance** out);
+
+/* Writes the reference JSON schema (ExampleLTLF.json) including
+   generating_formula / generating_seed for generated instances. */
+vfb_status vfb_instance_write_json(const vfb_instance* inst, const char* path);
+
+/* Read-only accessors.  lens: T ints; atoms: A*T words (layout above). */
+void vfb_instance_shape(const vfb_instance* inst, int* num_atoms, int* num_traces,
+                        int* num_pos);
+void vfb_instance_copy(const vfb_instance* inst, int* lens, uint64_t* atoms);
+const char* vfb_instance_atom_name(const vfb_instance* inst, int a);
+const char* vfb_instance_planted(const vfb_instance* inst); /* "" if none */
+int vfb_instance_max_size(const vfb_instance* inst);        /* config default, 0 if none */
+void vfb_instance_free(vfb_instance* inst);
+
+/* ---- solving ----------------------------------------------------------- */
+
+typedef struct {
+    int max_size;          /* 1..32 (LTLI maxCost, LTL_BS.cu:1071) */
+    int relaxed_type;      /* key when total trace length > 126: 1, 2 or 3 (LTL_BS.cu:39-92) */
+    int verbose;           /* 1: per-level progress lines on stderr */
+    int timing;            /* 1: time each kernel class with HIP events */
+    uint64_t hbm_budget;   /* bytes of device memory per solve; 0 = 90% of free */
+} vfb_options;
+
+typedef struct {
+    vfb_status status;
+    int min_size;          /* size of the returned formula (0 if none) */
+    int top_op;            /* vfb_op of its root */
+    int levels_done;       /* highest size level processed completely */
+    uint64_t distinct[VFB_MAX_SIZE + 1];   /* CS stored per level; at the found level,
+                                              survivors ordered before the returned formula */
+    uint64_t candidates[VFB_MAX_SIZE + 1]; /* candidates per level; at the found level,
+                                              through the returned formula */
+    uint64_t level_winners[VFB_MAX_SIZE + 1]; /* distinct CS per processed level, all of it */
+    uint64_t level_candidates[VFB_MAX_SIZE + 1]; /* candidates per processed level, all of it */
+    uint64_t found_ordinal; /* canonical ordinal of the returned formula in its level */
+    char* formula;         /* ltl_nocache.cu:289-317 syntax; "Not found !" if none */
+    double ms_total;       /* wall time of the solve (host clock) */
+    double ms_expand;      /* HIP-event time of the combine+key+dedup kernels */
+    double ms_resolve;     /* winner resolution + scan kernels */
+    double ms_store;       /* survivor compaction kernels */
+    uint64_t expand_launches;
+    uint64_t table_slots;  /* final hash-set capacity */
+    uint64_t retries;      /* epochs re-run after a table or list grew */
+} vfb_result;
+
+void vfb_options_default(vfb_options* opt);
+
+/* Plain single-instance solve: the drop-in for LTLI (LTL_BS.cu:1069-1532). */
+vfb_status vfb_solve(const vfb_instance* inst, const vfb_options* opt, vfb_result* out);
+
+/* Many independent instances stepped together, one level per step, in shared
+   launches.  out[i] is identical to vfb_solve(insts[i]).  No reference
+   counterpart (the reference solves one instance per process). */
+vfb_status vfb_solve_batch(const vfb_instance* const* insts, size_t n,
+                           const vfb_options* opt, vfb_result* out);
+
+void vfb_result_free(vfb_result* res);
+const char* vfb_status_string(vfb_status s);
+
+/* Device name and compute-unit count, for reports.  Returns VFB_DEVICE_ERROR
+   when no HIP device is usable. */
+vfb_status vfb_device_info(char* name, size_t name_cap, int* num_cus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,76 @@
+/*
+ * oracle.h -- CPU restatement of the reference's LTL characteristic-set (CS)
+ * enumeration, used ONLY as test infrastructure (the checker).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library.  The product path (libvfb.so) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference repository root).  Pinning (what checks each function
+ * against something other than itself) is listed per function; see the
+ * tests in tests/test_oracle.py.
+ */
+#ifndef VFB_ORACLE_H
+#define VFB_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_MAX_TRACES 128
+#define ORA_MAX_ATOMS 64
+#define ORA_MAX_SIZE 32
+
+/* operator codes, in the reference's segment order (LTL_BS.cu:17) */
+enum { ORA_NOT = 0, ORA_AND = 1, ORA_OR = 2, ORA_NEXT = 3, ORA_FINALLY = 4,
+       ORA_GLOBALLY = 5, ORA_UNTIL = 6, ORA_ATOM = 7 };
+
+/* dedup key modes */
+enum { ORA_KEY_AUTO1 = 1, ORA_KEY_AUTO2 = 2, ORA_KEY_AUTO3 = 3, /* exact if lenSum<=126, else relaxed type n */
+       ORA_KEY_FULLROW = 100 /* semantic dedup on the full CS row (no key function) */ };
+
+typedef struct {
+    int A;                  /* number of atomic propositions */
+    int T;                  /* number of traces (positives first) */
+    int P;                  /* number of positive traces */
+    const int* len;         /* [T] trace lengths, 1..64 */
+    const uint64_t* atoms;  /* [A*T]: atoms[a*T+t] bit k = AP a holds at step k of trace t */
+} ora_instance;
+
+typedef struct {
+    int status;             /* 0 found, 1 not found, 3 capacity */
+    int min_size;           /* size of the returned formula (0 if none) */
+    int top_op;             /* operator code of the returned formula's root */
+    int levels_done;        /* highest level whose candidates were all processed */
+    uint64_t distinct[ORA_MAX_SIZE + 1];   /* survivors stored per level (found level: before the match) */
+    uint64_t candidates[ORA_MAX_SIZE + 1]; /* candidates generated per level (found level: through the match) */
+    uint64_t found_ordinal; /* level-local ordinal of the returned formula */
+    char formula[8192];     /* ltl_nocache.cu:289-317 syntax */
+} ora_result;
+
+/* ---- primitive arithmetic ---- */
+uint64_t ora_mask(int len);
+uint64_t ora_apply(int op, uint64_t l, uint64_t r, int len);
+uint64_t ora_mix64(uint64_t x);
+void ora_key(const uint64_t* cs, const int* len, int T, int relaxed_type,
+             uint64_t* hi, uint64_t* lo);
+
+/* ---- enumerators ---- */
+int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
+                  uint64_t max_rows, const char* const* names, ora_result* out);
+int ora_bruteforce(const ora_instance* in, int max_size,
+                   const char* const* names, ora_result* out);
+/* closed-form count of syntactic formulas of each size, ltl_nocache.cu:248-287 */
+void ora_num_formulas(int A, int max_size, uint64_t* out /* [(max_size+1)*8] */);
+
+/* evaluate an RPN formula (tokens: 0..A-1 atoms, 100+op operators) on trace t
+   with the reference's bit-parallel operators; returns the CS word */
+uint64_t ora_eval_rpn(const ora_instance* in, const int* tokens, int n, int t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

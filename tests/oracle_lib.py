@@ -1,0 +1,150 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: only tests, the smoke check and the CPU-baseline leg of
+bench.py load the oracle.  The JSON reader here uses Python's json module so
+that it is independent of the product's own C++ reader.
+"""
+import ctypes as C
+import json
+import os
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+NOT, AND, OR, NEXT, FINALLY, GLOBALLY, UNTIL, ATOM = range(8)
+KEY_FULLROW = 100
+MAX_SIZE = 32
+
+
+class OraInstance(C.Structure):
+    _fields_ = [("A", C.c_int), ("T", C.c_int), ("P", C.c_int),
+                ("len", C.POINTER(C.c_int)), ("atoms", C.POINTER(C.c_uint64))]
+
+
+class OraResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("min_size", C.c_int), ("top_op", C.c_int),
+                ("levels_done", C.c_int),
+                ("distinct", C.c_uint64 * (MAX_SIZE + 1)),
+                ("candidates", C.c_uint64 * (MAX_SIZE + 1)),
+                ("found_ordinal", C.c_uint64),
+                ("formula", C.c_char * 8192)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = C.CDLL(LIB)
+        _lib.ora_mask.restype = C.c_uint64
+        _lib.ora_mask.argtypes = [C.c_int]
+        _lib.ora_apply.restype = C.c_uint64
+        _lib.ora_apply.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_int]
+        _lib.ora_mix64.restype = C.c_uint64
+        _lib.ora_mix64.argtypes = [C.c_uint64]
+        _lib.ora_key.restype = None
+        _lib.ora_key.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        _lib.ora_enumerate.restype = C.c_int
+        _lib.ora_enumerate.argtypes = [C.POINTER(OraInstance), C.c_int, C.c_int, C.c_uint64,
+                                       C.POINTER(C.c_char_p), C.POINTER(OraResult)]
+        _lib.ora_bruteforce.restype = C.c_int
+        _lib.ora_bruteforce.argtypes = [C.POINTER(OraInstance), C.c_int,
+                                        C.POINTER(C.c_char_p), C.POINTER(OraResult)]
+        _lib.ora_num_formulas.restype = None
+        _lib.ora_num_formulas.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_uint64)]
+    return _lib
+
+
+class Instance:
+    """Plain-Python instance: names, lens, atoms[a][t] words, P positives first."""
+
+    def __init__(self, names, lens, atoms, P):
+        self.names = list(names)
+        self.lens = list(lens)
+        self.atoms = [list(r) for r in atoms]
+        self.P = P
+
+    @property
+    def A(self):
+        return len(self.names)
+
+    @property
+    def T(self):
+        return len(self.lens)
+
+    @classmethod
+    def from_json(cls, path):
+        with open(path) as f:
+            j = json.load(f)
+        names = j["atomic_propositions"]
+        traces = list(j["positive_traces"]) + list(j["negative_traces"])
+        lens = [len(tr[names[0]]) for tr in traces]
+        atoms = []
+        for v in names:
+            row = []
+            for tr in traces:
+                w = 0
+                for k, b in enumerate(tr[v]):
+                    if b:
+                        w |= 1 << k
+                row.append(w)
+            atoms.append(row)
+        return cls(names, lens, atoms, len(j["positive_traces"]))
+
+    def _c(self):
+        T = self.T
+        lens = (C.c_int * T)(*self.lens)
+        flat = (C.c_uint64 * (self.A * T))(*[w for row in self.atoms for w in row])
+        names = (C.c_char_p * self.A)(*[n.encode() for n in self.names])
+        inst = OraInstance(self.A, T, self.P, lens, flat)
+        return inst, (lens, flat, names), names
+
+
+def _result(r):
+    return {
+        "status": r.status, "min_size": r.min_size, "top_op": r.top_op,
+        "levels_done": r.levels_done,
+        "distinct": list(r.distinct), "candidates": list(r.candidates),
+        "found_ordinal": r.found_ordinal,
+        "formula": r.formula.decode(),
+    }
+
+
+def enumerate_cs(inst, max_size, key_mode=3, max_rows=0):
+    ci, keep, names = inst._c()
+    r = OraResult()
+    lib().ora_enumerate(C.byref(ci), max_size, key_mode, max_rows, names, C.byref(r))
+    return _result(r)
+
+
+def bruteforce(inst, max_size):
+    ci, keep, names = inst._c()
+    r = OraResult()
+    lib().ora_bruteforce(C.byref(ci), max_size, names, C.byref(r))
+    return _result(r)
+
+
+def apply(op, l, r, length):
+    return lib().ora_apply(op, l, r, length)
+
+
+def mix64(x):
+    return lib().ora_mix64(x)
+
+
+def key(words, lens, relaxed_type=3):
+    T = len(words)
+    hi, lo = C.c_uint64(), C.c_uint64()
+    lib().ora_key((C.c_uint64 * T)(*words), (C.c_int * T)(*lens), T, relaxed_type,
+                  C.byref(hi), C.byref(lo))
+    return hi.value, lo.value
+
+
+def num_formulas(A, max_size):
+    buf = (C.c_uint64 * ((max_size + 1) * 8))()
+    lib().ora_num_formulas(A, max_size, buf)
+    return [[buf[s * 8 + k] for k in range(8)] for s in range(max_size + 1)]

@@ -1,0 +1,237 @@
+"""vfb_amd -- MI355X-native LTL synthesis from positive/negative traces.
+
+Python mirror of the C-ABI in include/vfb.h (libvfb.so, built in-tree under
+vfb_amd/lib).  The hot path -- size-ordered enumeration of characteristic sets
+with 128-bit keys, GPU hash-set dedup and target check -- runs in HIP kernels
+for gfx950; nothing here computes on the CPU.  Loading fails loudly when the
+library has not been built.
+
+Reference interface mirrored: `LTLI(...)` (LTL_BS.cu:1069-1079) and the
+`ltl_nocache <json_file> <maxLen>` process contract (ltl_nocache.cu:448-487).
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libvfb.so")
+CLI_PATH = os.path.join(_HERE, "bin", "vfb")
+
+MAX_SIZE = 32
+OK, NOT_FOUND, INVALID_INPUT, CAPACITY, DEVICE_ERROR, IO_ERROR = range(6)
+OP_NAMES = ["~", "&", "|", "X", "F", "G", "U", "atom"]
+
+
+class Options(C.Structure):
+    _fields_ = [("max_size", C.c_int), ("relaxed_type", C.c_int), ("verbose", C.c_int),
+                ("timing", C.c_int), ("hbm_budget", C.c_uint64)]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("min_size", C.c_int), ("top_op", C.c_int),
+                ("levels_done", C.c_int),
+                ("distinct", C.c_uint64 * (MAX_SIZE + 1)),
+                ("candidates", C.c_uint64 * (MAX_SIZE + 1)),
+                ("level_winners", C.c_uint64 * (MAX_SIZE + 1)),
+                ("level_candidates", C.c_uint64 * (MAX_SIZE + 1)),
+                ("found_ordinal", C.c_uint64),
+                ("formula", C.c_char_p),
+                ("ms_total", C.c_double), ("ms_expand", C.c_double),
+                ("ms_resolve", C.c_double), ("ms_store", C.c_double),
+                ("expand_launches", C.c_uint64), ("table_slots", C.c_uint64),
+                ("retries", C.c_uint64)]
+
+
+# every symbol declared in include/vfb.h
+EXPORTS = [
+    "vfb_instance_load_json", "vfb_instance_from_bits", "vfb_instance_generate",
+    "vfb_instance_write_json", "vfb_instance_shape", "vfb_instance_copy",
+    "vfb_instance_atom_name", "vfb_instance_planted", "vfb_instance_max_size",
+    "vfb_instance_free", "vfb_options_default", "vfb_solve", "vfb_solve_batch",
+    "vfb_result_free", "vfb_status_string", "vfb_device_info",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libvfb.so; raises if it was not built (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libvfb.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.vfb_instance_load_json.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.vfb_instance_from_bits.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int,
+                                         C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(vp)]
+    L.vfb_instance_generate.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
+    L.vfb_instance_write_json.argtypes = [vp, C.c_char_p]
+    L.vfb_instance_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.vfb_instance_shape.restype = None
+    L.vfb_instance_copy.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
+    L.vfb_instance_copy.restype = None
+    L.vfb_instance_atom_name.argtypes = [vp, C.c_int]
+    L.vfb_instance_atom_name.restype = C.c_char_p
+    L.vfb_instance_planted.argtypes = [vp]
+    L.vfb_instance_planted.restype = C.c_char_p
+    L.vfb_instance_max_size.argtypes = [vp]
+    L.vfb_instance_free.argtypes = [vp]
+    L.vfb_instance_free.restype = None
+    L.vfb_options_default.argtypes = [C.POINTER(Options)]
+    L.vfb_options_default.restype = None
+    L.vfb_solve.argtypes = [vp, C.POINTER(Options), C.POINTER(Result)]
+    L.vfb_solve_batch.argtypes = [C.POINTER(vp), C.c_size_t, C.POINTER(Options), C.POINTER(Result)]
+    L.vfb_result_free.argtypes = [C.POINTER(Result)]
+    L.vfb_result_free.restype = None
+    L.vfb_status_string.argtypes = [C.c_int]
+    L.vfb_status_string.restype = C.c_char_p
+    L.vfb_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
+    _lib = L
+    return L
+
+
+class VfbError(RuntimeError):
+    pass
+
+
+def status_string(s):
+    return lib().vfb_status_string(s).decode()
+
+
+class Instance:
+    """Owning handle of a vfb_instance."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise VfbError("null instance")
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value and _lib is not None:
+            _lib.vfb_instance_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def from_json(cls, path):
+        h = C.c_void_p()
+        st = lib().vfb_instance_load_json(os.fsencode(path), C.byref(h))
+        if st != OK:
+            raise VfbError(f"load_json({path}): {status_string(st)}")
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, config, index):
+        h = C.c_void_p()
+        st = lib().vfb_instance_generate(int(config), int(index), C.byref(h))
+        if st != OK:
+            raise VfbError(f"generate({config},{index}): {status_string(st)}")
+        return cls(h.value)
+
+    @classmethod
+    def from_bits(cls, names, lens, atoms, num_pos):
+        """atoms[a][t]: word of AP a on trace t; traces 0..num_pos-1 positive."""
+        A, T = len(names), len(lens)
+        flat = (C.c_uint64 * (A * T))(*[int(atoms[a][t]) for a in range(A) for t in range(T)])
+        nm = (C.c_char_p * A)(*[n.encode() for n in names])
+        h = C.c_void_p()
+        st = lib().vfb_instance_from_bits(A, nm, num_pos, T - num_pos, (C.c_int * T)(*lens), flat, C.byref(h))
+        if st != OK:
+            raise VfbError(f"from_bits: {status_string(st)}")
+        return cls(h.value)
+
+    def shape(self):
+        a, t, p = C.c_int(), C.c_int(), C.c_int()
+        lib().vfb_instance_shape(self._h, C.byref(a), C.byref(t), C.byref(p))
+        return a.value, t.value, p.value
+
+    def arrays(self):
+        """(names, lens, atoms[a][t], P)"""
+        A, T, P = self.shape()
+        lens = (C.c_int * T)()
+        flat = (C.c_uint64 * (A * T))()
+        lib().vfb_instance_copy(self._h, lens, flat)
+        names = [lib().vfb_instance_atom_name(self._h, a).decode() for a in range(A)]
+        atoms = [[flat[a * T + t] for t in range(T)] for a in range(A)]
+        return names, list(lens), atoms, P
+
+    @property
+    def planted(self):
+        return lib().vfb_instance_planted(self._h).decode()
+
+    @property
+    def default_max_size(self):
+        return lib().vfb_instance_max_size(self._h)
+
+    def write_json(self, path):
+        st = lib().vfb_instance_write_json(self._h, os.fsencode(path))
+        if st != OK:
+            raise VfbError(f"write_json: {status_string(st)}")
+
+
+def _opts(max_size, relaxed_type, timing, budget, verbose):
+    o = Options()
+    lib().vfb_options_default(C.byref(o))
+    o.max_size = max_size
+    o.relaxed_type = relaxed_type
+    o.timing = 1 if timing else 0
+    o.verbose = 1 if verbose else 0
+    o.hbm_budget = int(budget)
+    return o
+
+
+def _todict(r):
+    n = MAX_SIZE + 1
+    return {
+        "status": r.status, "min_size": r.min_size, "top_op": r.top_op,
+        "levels_done": r.levels_done,
+        "distinct": [r.distinct[i] for i in range(n)],
+        "candidates": [r.candidates[i] for i in range(n)],
+        "level_winners": [r.level_winners[i] for i in range(n)],
+        "level_candidates": [r.level_candidates[i] for i in range(n)],
+        "found_ordinal": r.found_ordinal,
+        "formula": (r.formula or b"").decode(),
+        "ms_total": r.ms_total, "ms_expand": r.ms_expand,
+        "ms_resolve": r.ms_resolve, "ms_store": r.ms_store,
+        "expand_launches": r.expand_launches, "table_slots": r.table_slots,
+        "retries": r.retries,
+    }
+
+
+def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbose=False):
+    """Plain single-instance solve (the drop-in for LTLI)."""
+    o = _opts(max_size, relaxed_type, timing, budget, verbose)
+    r = Result()
+    lib().vfb_solve(inst.handle, C.byref(o), C.byref(r))
+    try:
+        return _todict(r)
+    finally:
+        lib().vfb_result_free(C.byref(r))
+
+
+def solve_batch(insts, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0):
+    """Many instances stepped together; each result equals solve() alone."""
+    n = len(insts)
+    o = _opts(max_size, relaxed_type, timing, budget, False)
+    hs = (C.c_void_p * n)(*[i.handle.value for i in insts])
+    rs = (Result * n)()
+    lib().vfb_solve_batch(hs, n, C.byref(o), rs)
+    out = []
+    for k in range(n):
+        out.append(_todict(rs[k]))
+        lib().vfb_result_free(C.byref(rs[k]))
+    return out
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    cus = C.c_int()
+    st = lib().vfb_device_info(buf, 256, C.byref(cus))
+    if st != OK:
+        return None
+    return {"name": buf.value.decode(), "cus": cus.value}

@@ -1,0 +1,108 @@
+// Device-visible data structures of the CS enumeration engine.
+//
+// HBM layout (per instance):
+//   level block s : column-major uint64 words, word(t, row) = lvl[s][t * nrows[s] + row]
+//                   so a wavefront whose 64 lanes take 64 consecutive rows reads
+//                   512 contiguous bytes per trace (coalesced), and a uniform row
+//                   (the left operand of a binary wave) is a broadcast.
+//   parents       : one uint64 per stored row = its level-local canonical ordinal
+//                   (decodes to operator, operands), replacing the reference's
+//                   left/right index arrays and segment table (LTL_BS.cu:1189-1205).
+//   hash set      : open addressing, 32-byte slots {hi, lo, ord, pad}, power-of-two
+//                   capacity, linear probing.  Replaces WarpCore's two-level
+//                   64-bit scheme (LTL_BS.cu:251-269, hash_set.cuh:192-254).
+//   epoch scratch : fresh list (ordinal|match flag, slot), winner bitmap over the
+//                   level's ordinal space, per-4096-word block counts / prefixes.
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/vfb.h"
+
+namespace vfb {
+
+constexpr int kMaxT = VFB_MAX_TRACES;
+constexpr int kMaxS = VFB_MAX_SIZE;
+constexpr int kMaxUnits = 4 + 3 * (kMaxS / 2);   // ~,X,F,G + (&,|,U) per split
+constexpr uint64_t kEmpty = ~0ull;
+constexpr uint64_t kMatchBit = 1ull << 63;
+constexpr int kScanWords = 4096;       // bitmap words per scan block (262144 ordinals)
+constexpr int kProbeLimit = 4096;
+
+enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
+
+struct alignas(32) Slot {
+    unsigned long long hi, lo, ord, pad;
+};
+
+struct Ctr {
+    unsigned long long fresh_count;  // entries appended to the fresh list this epoch
+    unsigned long long claims;       // slots claimed this epoch (= distinct new keys)
+    unsigned long long found;        // lowest matching winner, level-local ordinal
+    unsigned long long winners;      // survivors of the epoch (set by the scan)
+    unsigned long long found_rank;   // survivors ordered before `found`
+    unsigned int overflow;           // 1 probe limit, 2 fresh list full, 4 load limit
+    unsigned int pad;
+};
+
+// One segment of a level's canonical candidate order: operator `op` applied to
+// rows of level lvl_l (and lvl_r for binary operators).
+struct Unit {
+    int op;
+    int lvl_l, lvl_r;
+    uint32_t nl, nr;
+    uint64_t base;    // first level-local ordinal
+    uint64_t count;   // candidates (Until: 2 per pair)
+};
+
+struct InstDesc {
+    int T, P, key_kind, half;
+    int stride;       // KEY_STRIDE sampling stride
+    int level;        // level being built
+    int nunits, pad0;
+    uint64_t lenmask[kMaxT];
+    uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
+    int16_t koff[kMaxT];        // first key bit of the trace's field / samples
+    int16_t kbits[kMaxT];       // field width (KEY_FIELD) or number of samples (KEY_STRIDE)
+    const unsigned long long* lvl[kMaxS + 2];
+    uint32_t nrows[kMaxS + 2];
+    Slot* table;
+    uint64_t table_mask;
+    uint64_t claim_limit;       // claims allowed this epoch before the set must grow
+    uint64_t epoch_base;        // global ordinal of the level's first candidate
+    uint64_t ord_lo, ord_hi;    // level-local ordinal range of this epoch
+    unsigned long long* fresh_ord;
+    uint32_t* fresh_slot;
+    uint64_t fresh_cap;
+    unsigned long long* winbits;
+    uint64_t nwords;
+    uint32_t* blkcount;
+    unsigned long long* blkprefix;
+    uint64_t nblk;
+    unsigned long long* out_rows;
+    unsigned long long* out_par;
+    uint64_t out_n;
+    Ctr* ctr;
+    Unit units[kMaxUnits];
+};
+
+// Fused work item: all operators that read the same operand rows.
+//   kind 0: rows of level lvl_l -> ~, X, F, G     (base[0..3])
+//   kind 1: pairs (level lvl_l x level lvl_r) -> &, |, lUr, rUl  (base[0..2])
+struct Task {
+    uint32_t inst;
+    uint8_t kind, lvl_l, lvl_r, pad;
+    uint32_t nl, nr;
+    uint32_t nchunk;            // ceil(nr / 64)
+    uint32_t pad1;
+    uint64_t base[4];
+    uint64_t wave_begin;        // prefix over tasks, in wavefronts
+};
+
+// Segment of a flat index space belonging to one instance.
+struct Seg {
+    uint32_t inst, pad;
+    uint64_t begin;             // prefix in the flat space
+};
+
+}  // namespace vfb

@@ -1,0 +1,572 @@
+// HIP kernels (gfx950, wave64) of the CS enumeration hot path.
+//
+// One epoch = one size level of one or more instances:
+//   k_expand  : every candidate of the level.  One lane per candidate; the
+//               four operators that read the same operand rows are fused
+//               (~,X,F,G per row; &,|,lUr,rUl per pair).  Computes the CS words
+//               trace by trace, folds them into the 128-bit key and the target
+//               mismatch bits, then probes the hash set.  Candidates whose key
+//               is new to the set are appended to the fresh list.
+//               Reference: processOperator + applyOperator + makeRlxUnqChkCSs +
+//               processUniqueCS + insertInCache (LTL_BS.cu:30-110, 183-238,
+//               251-269, 299-329, 383-446).
+//   k_resolve : a fresh candidate survives iff it holds the lowest ordinal
+//               of its key (slot.ord == own ordinal); survivors set their bit in
+//               the level's winner bitmap; matching survivors atomicMin `found`.
+//   k_count / k_scan : popcount blocks of the bitmap, exclusive scan.
+//   k_store   : stable compaction: survivors in ordinal order, CS recomputed
+//               and written column-major with their ordinal as parent record
+//               (replaces the Thrust remove/copy passes, LTL_BS.cu:489-530).
+#include <hip/hip_runtime.h>
+
+#include "engine_dev.h"
+
+namespace vfb {
+
+__device__ __forceinline__ uint64_t dmix64(uint64_t x) {
+    // SplitMix64 finaliser (LTL_BS.cu:80-83)
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ uint64_t suffix_or(uint64_t x) {
+    x |= x >> 1; x |= x >> 2; x |= x >> 4;
+    x |= x >> 8; x |= x >> 16; x |= x >> 32;
+    return x;
+}
+
+__device__ __forceinline__ uint64_t until_op(uint64_t l, uint64_t r) {
+    // bit-parallel carry doubling (LTL_BS.cu:222-233)
+    r |= l & (r >> 1);  l &= l >> 1;
+    r |= l & (r >> 2);  l &= l >> 2;
+    r |= l & (r >> 4);  l &= l >> 4;
+    r |= l & (r >> 8);  l &= l >> 8;
+    r |= l & (r >> 16); l &= l >> 16;
+    r |= l & (r >> 32);
+    return r;
+}
+
+__device__ __forceinline__ uint64_t apply_op(int op, uint64_t a, uint64_t b, uint64_t m) {
+    switch (op) {
+    case VFB_OP_NOT: return ~a & m;
+    case VFB_OP_AND: return a & b;
+    case VFB_OP_OR: return a | b;
+    case VFB_OP_NEXT: return a >> 1;
+    case VFB_OP_FINALLY: return suffix_or(a);
+    case VFB_OP_GLOBALLY: return a & ~suffix_or(~a & m);
+    default: return until_op(a, b);
+    }
+}
+
+struct Key {
+    uint64_t hi, lo;
+};
+
+// Folds trace t's CS word into the key (LTL_BS.cu:30-110).  Trace-uniform
+// parameters (offsets, widths, masks) were precomputed on the host.
+template <int KEY>
+__device__ __forceinline__ void key_acc(Key& k, uint64_t c, const InstDesc& d, int t) {
+    if constexpr (KEY == KEY_MIX) {
+        const uint64_t x = dmix64(c);
+        if (t < d.half) k.hi ^= x; else k.lo ^= x;
+    } else if constexpr (KEY == KEY_FIELD) {
+        const int m = d.kbits[t];
+        if (m > 0) {
+            const uint64_t f = c & d.kmask[t];
+            const int j = d.koff[t];
+            if (j < 63) {
+                k.lo |= (f << j) & 0x7fffffffffffffffull;
+                if (j + m > 63) k.hi |= f >> (63 - j);
+            } else {
+                k.hi |= f << (j - 63);
+            }
+        }
+    } else {
+        const int n = d.kbits[t];
+        const int s = d.stride;
+        int j = d.koff[t];
+        for (int q = 0; q < n; ++q, ++j) {
+            const uint64_t b = (c >> (q * s)) & 1ull;
+            if (j < 63) k.lo |= b << j; else k.hi |= b << (j - 63);
+        }
+    }
+}
+
+__device__ __forceinline__ void key_fin(Key& k) {
+    // -1 and -2 are one half-key in the reference (hash_set.cuh:198-204)
+    if (k.hi == kEmpty) k.hi = kEmpty - 1;
+    if (k.lo == kEmpty) k.lo = kEmpty - 1;
+}
+
+__device__ __forceinline__ uint64_t slot_hash(uint64_t hi, uint64_t lo) {
+    return dmix64(hi ^ dmix64(lo + 0x9e3779b97f4a7c15ull));
+}
+
+__device__ __forceinline__ uint64_t ld_any(const unsigned long long* p) {
+    // may return a stale value; only EMPTY can be stale because every field
+    // of a slot is written once (hi, lo) or only decreases (ord)
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+enum { PR_DUP = 0, PR_FRESH = 1, PR_FAIL = 2 };
+
+// Insert-or-find of a 128-bit key with lowest-ordinal tracking, lock-free:
+// hi and lo are each claimed by a 64-bit CAS from EMPTY; a prober decides on
+// the values the CAS returns, so every prober of a key stops at the same slot.
+// Keys from earlier epochs carry ord < epoch_base and are never modified.
+__device__ __forceinline__ int probe_insert(Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo,
+                                            uint64_t gord, uint64_t epoch_base,
+                                            uint32_t* slot_out, int* claimed) {
+    uint64_t i = slot_hash(hi, lo) & mask;
+    for (int n = 0; n < kProbeLimit; ++n, i = (i + 1) & mask) {
+        Slot* s = tab + i;
+        uint64_t th = ld_any(&s->hi);
+        uint64_t tl = ld_any(&s->lo);
+        const uint64_t to = ld_any(&s->ord);
+        if (th == kEmpty) {
+            th = atomicCAS(&s->hi, kEmpty, hi);
+            if (th == kEmpty) { th = hi; *claimed += 1; }
+        }
+        if (th != hi) continue;
+        if (tl == kEmpty) {
+            tl = atomicCAS(&s->lo, kEmpty, lo);
+            if (tl == kEmpty) tl = lo;
+        }
+        if (tl != lo) continue;
+        *slot_out = (uint32_t)i;
+        if (to < epoch_base) return PR_DUP;
+        const uint64_t old = atomicMin(&s->ord, (unsigned long long)gord);
+        return old < epoch_base ? PR_DUP : PR_FRESH;
+    }
+    return PR_FAIL;
+}
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Probes the candidates of this lane (up to 4) and appends the fresh ones.
+__device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* key, const uint64_t* ord,
+                                                  uint32_t mismatch, uint32_t enable, int lane) {
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    int claimed = 0;
+    int fail = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const bool en = (enable >> q) & 1u;
+        uint32_t slot = 0xffffffffu;
+        int r = PR_DUP;
+        if (en) {
+            const uint64_t gord = d.epoch_base + ord[q];
+            r = probe_insert(d.table, d.table_mask, key[q].hi, key[q].lo, gord, d.epoch_base, &slot, &claimed);
+            if (r == PR_FAIL) fail = 1;
+        }
+        const bool fresh = en && r == PR_FRESH;
+        const uint64_t bal = __ballot(fresh);
+        if (bal) {
+            const int leader = __ffsll((unsigned long long)bal) - 1;
+            uint64_t base = 0;
+            if (lane == leader) base = atomicAdd(&d.ctr->fresh_count, (unsigned long long)__popcll(bal));
+            base = bcast64(base, leader);
+            if (fresh) {
+                const uint64_t idx = base + __popcll(bal & lt_mask);
+                if (idx < d.fresh_cap) {
+                    const uint64_t flag = ((mismatch >> q) & 1u) ? 0ull : kMatchBit;
+                    d.fresh_ord[idx] = (d.epoch_base + ord[q]) | flag;
+                    d.fresh_slot[idx] = slot;
+                } else {
+                    atomicOr(&d.ctr->overflow, 2u);
+                }
+            }
+        }
+    }
+    // claims, aggregated per wavefront, bound the table's load factor
+    uint32_t wclaims = 0;
+#pragma unroll
+    for (int q = 1; q <= 4; ++q) wclaims += __popcll(__ballot(claimed >= q));
+    if (wclaims) {
+        const int leader = __ffsll((unsigned long long)__ballot(true)) - 1;
+        if (lane == leader) {
+            const unsigned long long before = atomicAdd(&d.ctr->claims, (unsigned long long)wclaims);
+            if (before + wclaims > d.claim_limit) atomicOr(&d.ctr->overflow, 4u);
+        }
+    }
+    if (__ballot(fail)) {
+        if (fail) atomicOr(&d.ctr->overflow, 1u);
+    }
+}
+
+__device__ __forceinline__ uint32_t epoch_enable(const InstDesc& d, const uint64_t* ord, bool active) {
+    uint32_t en = 0;
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (ord[q] >= d.ord_lo && ord[q] < d.ord_hi) en |= 1u << q;
+    }
+    return en;
+}
+
+template <int KEY>
+__device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    const uint64_t x = wl * 64 + (uint64_t)lane;
+    const bool active = x < tk.nl;
+    const uint64_t xr = active ? x : 0;
+    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
+    Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    uint32_t mm = 0;
+    const int T = d.T;
+    for (int t = 0; t < T; ++t) {
+        const uint64_t a = L[(uint64_t)t * tk.nl + xr];
+        const uint64_t m = d.lenmask[t];
+        const uint64_t c0 = ~a & m;                      // ~
+        const uint64_t c1 = a >> 1;                      // X
+        const uint64_t c2 = suffix_or(a);                // F
+        const uint64_t c3 = a & ~suffix_or(c0);          // G
+        const uint64_t want = t < d.P ? 1ull : 0ull;
+        mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
+              ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
+        key_acc<KEY>(k[0], c0, d, t);
+        key_acc<KEY>(k[1], c1, d, t);
+        key_acc<KEY>(k[2], c2, d, t);
+        key_acc<KEY>(k[3], c3, d, t);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) key_fin(k[q]);
+    const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
+    finish_candidates(d, k, ord, mm, epoch_enable(d, ord, active), lane);
+}
+
+template <int KEY>
+__device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    const uint64_t l = wl / tk.nchunk;
+    const uint64_t rc = wl - l * tk.nchunk;
+    const uint64_t r = rc * 64 + (uint64_t)lane;
+    const bool active = r < tk.nr;
+    const uint64_t rr = active ? r : 0;
+    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
+    const unsigned long long* __restrict__ R = d.lvl[tk.lvl_r];
+    Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    uint32_t mm = 0;
+    const int T = d.T;
+    for (int t = 0; t < T; ++t) {
+        const uint64_t a = L[(uint64_t)t * tk.nl + l];
+        const uint64_t b = R[(uint64_t)t * tk.nr + rr];
+        const uint64_t c0 = a & b;
+        const uint64_t c1 = a | b;
+        const uint64_t c2 = until_op(a, b);
+        const uint64_t c3 = until_op(b, a);
+        const uint64_t want = t < d.P ? 1ull : 0ull;
+        mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
+              ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
+        key_acc<KEY>(k[0], c0, d, t);
+        key_acc<KEY>(k[1], c1, d, t);
+        key_acc<KEY>(k[2], c2, d, t);
+        key_acc<KEY>(k[3], c3, d, t);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) key_fin(k[q]);
+    const uint64_t pair = l * tk.nr + r;
+    const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
+                             tk.base[2] + 2 * pair + 1};
+    finish_candidates(d, k, ord, mm, epoch_enable(d, ord, active), lane);
+}
+
+__global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, const Task* __restrict__ tasks,
+                                                int ntasks, uint64_t total_waves) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t wpb = blockDim.x >> 6;
+    const uint64_t stride = (uint64_t)gridDim.x * wpb;
+    for (uint64_t w = (uint64_t)blockIdx.x * wpb + wib; w < total_waves; w += stride) {
+        int lo = 0, hi = ntasks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tasks[mid].wave_begin <= w) lo = mid; else hi = mid - 1;
+        }
+        const Task& tk = tasks[lo];
+        const InstDesc& d = D[tk.inst];
+        const uint64_t wl = w - tk.wave_begin;
+        if (tk.kind == 0) {
+            if (d.key_kind == KEY_MIX) expand_unary<KEY_MIX>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_unary<KEY_FIELD>(d, tk, wl, lane);
+            else expand_unary<KEY_STRIDE>(d, tk, wl, lane);
+        } else {
+            if (d.key_kind == KEY_MIX) expand_binary<KEY_MIX>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_binary<KEY_FIELD>(d, tk, wl, lane);
+            else expand_binary<KEY_STRIDE>(d, tk, wl, lane);
+        }
+    }
+}
+
+// Inserts the rows of level 1 (the atoms) with ordinals 0..A-1 so that later
+// levels deduplicate against them (hashSetsInitialisation, LTL_BS.cu:136-160).
+template <int KEY>
+__device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n) {
+    const unsigned long long* L = d.lvl[1];
+    Key k{0, 0};
+    for (int t = 0; t < d.T; ++t) key_acc<KEY>(k, L[(uint64_t)t * n + x], d, t);
+    key_fin(k);
+    uint32_t slot;
+    int claimed = 0;
+    const int r = probe_insert(d.table, d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed);
+    if (r == PR_FAIL) atomicOr(&d.ctr->overflow, 1u);
+    if (claimed) atomicAdd(&d.ctr->claims, 1ull);
+}
+
+__global__ void k_insert_atoms(const InstDesc* __restrict__ D, uint32_t inst, uint32_t n) {
+    const uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const InstDesc& d = D[inst];
+    if (d.key_kind == KEY_MIX) insert_row<KEY_MIX>(d, x, n);
+    else if (d.key_kind == KEY_FIELD) insert_row<KEY_FIELD>(d, x, n);
+    else insert_row<KEY_STRIDE>(d, x, n);
+}
+
+// Moves the keys of completed epochs (ord < keep_below) into a fresh table.
+__global__ void k_rehash(const Slot* __restrict__ old, uint64_t old_cap, Slot* tab, uint64_t mask,
+                         uint64_t keep_below, unsigned int* fail) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < old_cap; i += stride) {
+        const Slot s = old[i];
+        if (s.hi == kEmpty || s.lo == kEmpty || s.ord >= keep_below) continue;
+        uint32_t slot;
+        int claimed = 0;
+        if (probe_insert(tab, mask, s.hi, s.lo, s.ord, 0, &slot, &claimed) == PR_FAIL) atomicOr(fail, 1u);
+    }
+}
+
+__global__ void k_fill(unsigned long long* p, uint64_t n, unsigned long long v) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__device__ __forceinline__ int seg_find(const Seg* segs, int nseg, uint64_t g) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].begin <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
+                                                 int nseg, uint64_t total) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
+        const Seg& sg = segs[seg_find(segs, nseg, g)];
+        const InstDesc& d = D[sg.inst];
+        const uint64_t e = g - sg.begin;
+        const uint64_t of = d.fresh_ord[e];
+        const uint32_t slot = d.fresh_slot[e];
+        const uint64_t o = of & ~kMatchBit;
+        if (d.table[slot].ord == o) {
+            const uint64_t local = o - d.epoch_base;
+            atomicOr(&d.winbits[local >> 6], 1ull << (local & 63));
+            if (of & kMatchBit) atomicMin(&d.ctr->found, (unsigned long long)local);
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
+    // 256 threads = 4 wavefronts
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    return sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// popcount of each 4096-word block of every instance's winner bitmap
+__global__ __launch_bounds__(256) void k_count(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
+                                               int nseg, uint64_t total_blocks) {
+    __shared__ uint32_t sh[4];
+    for (uint64_t g = blockIdx.x; g < total_blocks; g += gridDim.x) {
+        const Seg& sg = segs[seg_find(segs, nseg, g)];
+        const InstDesc& d = D[sg.inst];
+        const uint64_t b = g - sg.begin;
+        const uint64_t w0 = b * kScanWords;
+        uint32_t c = 0;
+        for (int i = threadIdx.x; i < kScanWords; i += 256) {
+            const uint64_t w = w0 + i;
+            if (w < d.nwords) c += __popcll(d.winbits[w]);
+        }
+        c = block_sum_u32(c, sh);
+        if (threadIdx.x == 0) d.blkcount[b] = c;
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// one workgroup per instance: exclusive scan of the block counts, the level's
+// survivor total, and the rank of the returned formula among the survivors
+__global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, const uint32_t* __restrict__ insts) {
+    __shared__ uint64_t wsum[4];
+    __shared__ uint64_t carry;
+    __shared__ uint32_t sh[4];
+    const InstDesc& d = D[insts[blockIdx.x]];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint64_t base = 0; base < d.nblk; base += 256) {
+        const uint64_t i = base + threadIdx.x;
+        const uint64_t v = i < d.nblk ? d.blkcount[i] : 0;
+        const uint64_t inc = wave_incl_scan(v);
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint64_t pre = carry;
+        for (int k = 0; k < w; ++k) pre += wsum[k];
+        if (i < d.nblk) d.blkprefix[i] = pre + inc - v;
+        __syncthreads();
+        if (threadIdx.x == 255) carry = pre + inc;
+        __syncthreads();
+    }
+    const uint64_t total = carry;
+    const uint64_t f = d.ctr->found;
+    uint64_t rank = 0;
+    if (f != kEmpty) {
+        const uint64_t fw = f >> 6, fb = fw / kScanWords;
+        uint32_t c = 0;
+        for (uint64_t x = fb * kScanWords + threadIdx.x; x < fw; x += 256) c += __popcll(d.winbits[x]);
+        c = block_sum_u32(c, sh);
+        rank = d.blkprefix[fb] + c + __popcll(d.winbits[fw] & ((1ull << (f & 63)) - 1ull));
+    }
+    if (threadIdx.x == 0) {
+        d.ctr->winners = total;
+        d.ctr->found_rank = rank;
+    }
+}
+
+// recomputes the CS word of trace t for the candidate with level-local ordinal o
+__device__ __forceinline__ uint64_t cs_of(const InstDesc& d, const Unit& u, uint64_t o, int t) {
+    const uint64_t k = o - u.base;
+    const uint64_t m = d.lenmask[t];
+    const unsigned long long* L = d.lvl[u.lvl_l];
+    if (u.op == VFB_OP_NOT || u.op == VFB_OP_NEXT || u.op == VFB_OP_FINALLY || u.op == VFB_OP_GLOBALLY)
+        return apply_op(u.op, L[(uint64_t)t * u.nl + k], 0, m);
+    const unsigned long long* R = d.lvl[u.lvl_r];
+    if (u.op == VFB_OP_UNTIL) {
+        const uint64_t p = k >> 1, l = p / u.nr, r = p - l * u.nr;
+        const uint64_t a = L[(uint64_t)t * u.nl + l], b = R[(uint64_t)t * u.nr + r];
+        return (k & 1) ? until_op(b, a) : until_op(a, b);
+    }
+    const uint64_t l = k / u.nr, r = k - l * u.nr;
+    return apply_op(u.op, L[(uint64_t)t * u.nl + l], R[(uint64_t)t * u.nr + r], m);
+}
+
+__device__ __forceinline__ int unit_of(const InstDesc& d, uint64_t o) {
+    int lo = 0, hi = d.nunits - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d.units[mid].base <= o) lo = mid; else hi = mid - 1;
+    }
+    // skip empty units that share a base with the next one
+    while (lo + 1 < d.nunits && d.units[lo].count == 0) ++lo;
+    return lo;
+}
+
+// stable compaction of the survivors: 64 bitmap words per round are expanded
+// into an LDS list of ordinals, then 256 lanes write consecutive rows
+__global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
+                                               int nseg, uint64_t total_blocks) {
+    __shared__ uint64_t list[64 * 64];
+    __shared__ uint32_t cnt;
+    const int lane = threadIdx.x & 63;
+    for (uint64_t g = blockIdx.x; g < total_blocks; g += gridDim.x) {
+        const Seg& sg = segs[seg_find(segs, nseg, g)];
+        const InstDesc& d = D[sg.inst];
+        const uint64_t b = g - sg.begin;
+        if (d.blkcount[b] == 0) continue;
+        uint64_t rank = d.blkprefix[b];
+        const uint64_t w0 = b * kScanWords;
+        const uint64_t w1 = w0 + kScanWords < d.nwords ? w0 + kScanWords : d.nwords;
+        for (uint64_t wr = w0; wr < w1; wr += 64) {
+            if (threadIdx.x < 64) {
+                const uint64_t w = wr + lane;
+                uint64_t bits = w < w1 ? d.winbits[w] : 0ull;
+                const uint64_t pc = __popcll(bits);
+                const uint64_t inc = wave_incl_scan(pc);
+                uint64_t pos = inc - pc;
+                while (bits) {
+                    const int q = __ffsll((unsigned long long)bits) - 1;
+                    bits &= bits - 1;
+                    list[pos++] = w * 64 + (uint64_t)q;
+                }
+                if (lane == 63) cnt = (uint32_t)inc;
+            }
+            __syncthreads();
+            const uint32_t n = cnt;
+            for (uint32_t i = threadIdx.x; i < n; i += 256) {
+                const uint64_t o = list[i];
+                const Unit& u = d.units[unit_of(d, o)];
+                const uint64_t row = rank + i;
+                for (int t = 0; t < d.T; ++t) d.out_rows[(uint64_t)t * d.out_n + row] = cs_of(d, u, o, t);
+                d.out_par[row] = o;
+            }
+            rank += n;
+            __syncthreads();
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host)
+// ---------------------------------------------------------------------------
+
+hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int grid,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_insert_atoms, dim3((n + 63) / 64), dim3(64), 0, st, D, inst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rehash(const Slot* old, uint64_t old_cap, Slot* tab, uint64_t mask, uint64_t keep_below,
+                         unsigned int* fail, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_rehash, dim3(grid), dim3(256), 0, st, old, old_cap, tab, mask, keep_below, fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, int grid, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, uint64_t total, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, st, D, segs, nseg, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_blocks, int grid,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_count, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_blocks);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_scan, dim3(n), dim3(256), 0, st, D, insts);
+    return hipGetLastError();
+}
+
+hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_blocks, int grid,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_store, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_blocks);
+    return hipGetLastError();
+}
+
+}  // namespace vfb

@@ -1,0 +1,45 @@
+// Internal host-side types shared by the library's translation units.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vfb.h"
+
+struct vfb_instance {
+    std::vector<std::string> names;   // AP names, in input order
+    std::vector<int> lens;            // per trace, positives first
+    std::vector<uint64_t> atoms;      // [a * T + t]
+    int P = 0;                        // positive traces
+    std::string planted;              // generating formula (generated instances)
+    uint64_t seed = 0;                // generating seed
+    int default_max_size = 0;         // config default (generated instances)
+    int A() const { return (int)names.size(); }
+    int T() const { return (int)lens.size(); }
+};
+
+namespace vfb {
+
+// Validates limits; returns VFB_OK or VFB_INVALID_INPUT with a message.
+vfb_status validate(const vfb_instance& in, std::string* why);
+
+inline uint64_t len_mask(int len) {
+    return len >= 64 ? ~uint64_t(0) : ((uint64_t(1) << len) - 1);
+}
+
+// Bit-parallel operator semantics (LTL_BS.cu:183-238), host side: used for
+// the atom check, planted-formula labels and formula re-verification.
+uint64_t host_apply(int op, uint64_t l, uint64_t r, int len);
+
+// Evaluates RPN tokens (atoms 0..A-1, operators 100+op) on trace t.
+uint64_t eval_rpn(const vfb_instance& in, const std::vector<int>& rpn, int t);
+
+// Renders RPN in ltl_nocache.cu:289-317 syntax.
+std::string rpn_to_string(const std::vector<int>& rpn, const std::vector<std::string>& names);
+
+// Parses the library's own formula syntax back to RPN; false on error.
+bool parse_formula(const std::string& s, const std::vector<std::string>& names,
+                   std::vector<int>* rpn);
+
+}  // namespace vfb
