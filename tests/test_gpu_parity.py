@@ -1,0 +1,98 @@
+"""GPU parity: the HIP path (libvfb.so, through its C-ABI) against the oracle.
+
+Bar: bit-exact -- same status, minimal size, returned formula text, and
+per-level distinct / candidate counts -- on seeded instances at sizes the
+oracle finishes in seconds.  Every returned formula is also re-verified on
+every trace with the textbook per-position semantics.
+"""
+import os
+import random
+
+import pytest
+
+import oracle_lib as O
+import vfb_amd as V
+from ltl_semantics import separates, parse, size
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _oracle_for(inst, max_size, relaxed_type=3):
+    names, lens, atoms, P = inst.arrays()
+    return O.enumerate_cs(O.Instance(names, lens, atoms, P), max_size, relaxed_type), (names, lens, atoms, P)
+
+
+def _check(inst, max_size, relaxed_type=3, label=""):
+    print(f"[parity] {label} max_size={max_size} relaxed={relaxed_type} shape={inst.shape()}", flush=True)
+    want, (names, lens, atoms, P) = _oracle_for(inst, max_size, relaxed_type)
+    got = V.solve(inst, max_size=max_size, relaxed_type=relaxed_type)
+    assert got["status"] == want["status"], (label, got["status"], want["status"])
+    assert got["min_size"] == want["min_size"], (label, got["min_size"], want["min_size"])
+    assert got["formula"] == want["formula"], (label, got["formula"], want["formula"])
+    assert got["distinct"] == want["distinct"], (label, got["distinct"], want["distinct"])
+    assert got["candidates"] == want["candidates"], (label, got["candidates"], want["candidates"])
+    if got["status"] == V.OK:
+        assert got["found_ordinal"] == want["found_ordinal"]
+        assert separates(got["formula"], names, lens, atoms, P), got["formula"]
+        assert size(parse(got["formula"], names)) == got["min_size"]
+    return got
+
+
+def test_example_json():
+    inst = V.Instance.from_json(os.path.join(GOLDEN, "ExampleLTLF.json"))
+    got = _check(inst, 10, label="ExampleLTLF")
+    assert got["min_size"] == 10
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_config1(k):
+    _check(V.Instance.generate(1, k), 10, label=f"cfg1 #{k}")
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_config2_relaxed3(k):
+    _check(V.Instance.generate(2, k), 7, 3, label=f"cfg2 #{k}")
+
+
+@pytest.mark.parametrize("rt", [1, 2, 3])
+def test_relaxed_types(rt):
+    _check(V.Instance.generate(2, 2), 7, rt, label=f"cfg2 #2 type {rt}")
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_config3(k):
+    _check(V.Instance.generate(3, k), 6, 3, label=f"cfg3 #{k}")
+
+
+def _random_instance(rng, A, T, lmin, lmax, p=0.5):
+    lens = [rng.randint(lmin, lmax) for _ in range(T)]
+    atoms = [[sum((rng.random() < p) << k for k in range(lens[t])) for t in range(T)] for _ in range(A)]
+    P = rng.randint(0, T)
+    return V.Instance.from_bits([f"p{a}" for a in range(A)], lens, atoms, P)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_small(seed):
+    rng = random.Random(1000 + seed)
+    A = rng.randint(1, 4)
+    T = rng.randint(1, 16)
+    inst = _random_instance(rng, A, T, 1, 12)
+    _check(inst, 7, 3, label=f"random seed {seed}")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_length_64_and_many_traces(seed):
+    rng = random.Random(77 + seed)
+    inst = _random_instance(rng, 3, 128 if seed % 2 else 64, 60, 64, 0.4)
+    _check(inst, 5, 3, label=f"long traces {seed}")
+
+
+def test_batch_equals_plain():
+    insts = [V.Instance.generate(4, k) for k in range(24)]
+    plain = [V.solve(i, max_size=8) for i in insts]
+    batch = V.solve_batch(insts, max_size=8)
+    for k, (a, b) in enumerate(zip(plain, batch)):
+        for f in ("status", "min_size", "formula", "distinct", "candidates", "level_winners"):
+            assert a[f] == b[f], (k, f, a[f], b[f])

@@ -1,0 +1,136 @@
+"""Host-side checks of libvfb.so that need no GPU: the C-ABI exports every
+symbol include/vfb.h declares, the JSON reader agrees with Python's json
+module, the generator is deterministic and labels traces by its planted
+formula, limits are enforced, and without a GPU a solve fails loudly
+(VFB_DEVICE_ERROR) instead of falling back to the CPU."""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+
+import pytest
+
+import vfb_amd as V
+from ltl_semantics import separates, parse
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "vfb.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vfb_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol():
+    L = C.CDLL(V.LIB_PATH)
+    declared = _header_functions()
+    assert len(declared) >= 15
+    for name in declared:
+        assert hasattr(L, name), name
+    assert sorted(V.EXPORTS) == declared
+
+
+def _py_arrays(path):
+    with open(path) as f:
+        j = json.load(f)
+    names = j["atomic_propositions"]
+    traces = list(j["positive_traces"]) + list(j["negative_traces"])
+    lens = [len(tr[names[0]]) for tr in traces]
+    atoms = [[sum(1 << k for k, b in enumerate(tr[v]) if b) for tr in traces] for v in names]
+    return names, lens, atoms, len(j["positive_traces"])
+
+
+def test_json_reader_matches_python_json():
+    p = os.path.join(GOLDEN, "ExampleLTLF.json")
+    assert V.Instance.from_json(p).arrays() == _py_arrays(p)
+
+
+@pytest.mark.parametrize("cfg,k", [(1, 0), (2, 3), (3, 1), (4, 7)])
+def test_generated_roundtrip_and_labels(tmp_path, cfg, k):
+    inst = V.Instance.generate(cfg, k)
+    again = V.Instance.generate(cfg, k)
+    assert inst.arrays() == again.arrays()          # deterministic
+    path = str(tmp_path / "g.json")
+    inst.write_json(path)
+    assert V.Instance.from_json(path).arrays() == inst.arrays() == _py_arrays(path)
+    names, lens, atoms, P = inst.arrays()
+    # the planted formula labels every trace (checked per position)
+    assert separates(inst.planted, names, lens, atoms, P)
+    with open(path) as f:
+        j = json.load(f)
+    assert j["generating_seed"] == str((cfg << 32) | k)
+
+
+def test_generator_shapes():
+    A, T, P = V.Instance.generate(1, 0).shape()
+    assert (A, T, P) == (3, 10, 5)
+    A, T, P = V.Instance.generate(2, 0).shape()
+    assert (A, T, P) == (5, 40, 20)
+    A, T, P = V.Instance.generate(3, 0).shape()
+    assert (A, T, P) == (10, 63, 32)
+    for k in range(20):
+        inst = V.Instance.generate(4, k)
+        A, T, P = inst.shape()
+        names, lens, atoms, P = inst.arrays()
+        assert 2 <= A <= 8 and 8 <= T <= 63 and 1 <= P <= T - 1
+        assert all(8 <= n <= 64 for n in lens)
+        assert inst.default_max_size == 16
+    names, lens, _, _ = V.Instance.generate(2, 5).arrays()
+    assert all(16 <= n <= 32 for n in lens)
+    assert "U" in V.Instance.generate(2, 5).planted
+    assert V.Instance.generate(1, 3).planted == "F((a0) & (X(a1)))"
+
+
+def test_limits_rejected():
+    with pytest.raises(V.VfbError):
+        V.Instance.from_bits(["a"], [65], [[1]], 1)          # trace longer than 64
+    with pytest.raises(V.VfbError):
+        V.Instance.from_bits(["a"], [3] * 129, [[1] * 129], 1)  # more than 128 traces
+    with pytest.raises(V.VfbError):
+        V.Instance.from_bits(["a"], [0], [[0]], 1)           # empty trace
+
+
+def test_bad_json(tmp_path):
+    p = tmp_path / "bad.json"
+    p.write_text('{"atomic_propositions": ["a"], "positive_traces": [{"a": [1, 0]}], '
+                 '"negative_traces": [{"a": [1]}], "number_traces": 5}')
+    with pytest.raises(V.VfbError):
+        V.Instance.from_json(str(p))
+    with pytest.raises(V.VfbError):
+        V.Instance.from_json(str(tmp_path / "missing.json"))
+
+
+def test_from_bits_masks_and_accessors():
+    inst = V.Instance.from_bits(["x", "y"], [2, 3], [[0xFF, 0x1], [0x0, 0x7]], 1)
+    names, lens, atoms, P = inst.arrays()
+    assert names == ["x", "y"] and lens == [2, 3] and P == 1
+    assert atoms == [[0x3, 0x1], [0x0, 0x7]]
+
+
+def _has_gpu():
+    return V.device_info() is not None
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the no-GPU behaviour")
+def test_no_gpu_fails_loudly():
+    r = V.solve(V.Instance.generate(1, 0), max_size=5)
+    assert r["status"] == V.DEVICE_ERROR
+
+
+def test_cli_usage_and_missing_file():
+    out = subprocess.run([V.CLI_PATH], capture_output=True, text=True)
+    assert out.returncode == 0 and "input_file_address" in out.stderr
+    out = subprocess.run([V.CLI_PATH, "/nonexistent.json", "5"], capture_output=True, text=True)
+    assert out.returncode == 0 and "Could not open" in out.stderr
+    out = subprocess.run([V.CLI_PATH, os.path.join(GOLDEN, "ExampleLTLF.json"), "99"], capture_output=True, text=True)
+    assert "between 1 and 32" in out.stderr
+
+
+def test_parser_roundtrip():
+    names = ["a0", "a1", "X1"]
+    for f in ["a0", "~(a1)", "(a0) & (X(X1))", "((a0) U (G(a1))) | (F(~(X1)))"]:
+        tree = parse(f, names)
+        assert tree

@@ -1,0 +1,227 @@
+"""Pins the CPU oracle against things other than itself.
+
+- operators: textbook per-position finite-trace semantics, all lengths 1..64;
+- LTL identities (~~p = p, F p = true U p, G p = ~F~p, expansion law of U,
+  De Morgan, idempotence of F and G);
+- SplitMix64 finaliser: the published first outputs of SplitMix64 seeded 0;
+- exact key: injective (decoded back bit by bit);
+- relaxed keys 1 and 2: known answers worked out by hand below;
+- per-level distinct counts with exact keys == semantic dedup on full rows;
+- minimal size and formula == independent brute-force syntactic enumeration
+  (ltl_nocache numbering), the formula re-verified per position;
+- the brute-force formula counts == direct enumeration of syntax trees.
+"""
+import itertools
+import os
+import random
+
+import pytest
+
+import oracle_lib as O
+from ltl_semantics import pos_apply, separates, parse, size
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+M64 = (1 << 64) - 1
+
+
+def mask(n):
+    return (1 << n) - 1
+
+
+@pytest.mark.parametrize("op", range(7))
+def test_operators_match_position_semantics(op):
+    rng = random.Random(op)
+    for n in range(1, 65):
+        for _ in range(12):
+            a = rng.getrandbits(n)
+            b = rng.getrandbits(n)
+            # sparse and dense words too
+            if _ % 3 == 1:
+                a &= rng.getrandbits(n)
+            if _ % 3 == 2:
+                b |= rng.getrandbits(n)
+            got = O.apply(op, a, b, n) & mask(n)
+            assert got == pos_apply(op, a, b, n), (op, n, hex(a), hex(b))
+            # results never carry bits at or above the trace length
+            assert O.apply(op, a, b, n) >> n == 0
+
+
+def test_identities():
+    rng = random.Random(5)
+    for n in list(range(1, 65)):
+        full = mask(n)
+        for _ in range(8):
+            p, q = rng.getrandbits(n), rng.getrandbits(n)
+            ap = lambda op, x, y=0: O.apply(op, x, y, n)
+            assert ap(O.NOT, ap(O.NOT, p)) == p
+            assert ap(O.FINALLY, p) == ap(O.UNTIL, full, p)
+            assert ap(O.GLOBALLY, p) == ap(O.NOT, ap(O.FINALLY, ap(O.NOT, p)))
+            u = ap(O.UNTIL, p, q)
+            assert u == ap(O.OR, q, ap(O.AND, p, ap(O.NEXT, u)))
+            assert ap(O.NOT, ap(O.AND, p, q)) == ap(O.OR, ap(O.NOT, p), ap(O.NOT, q))
+            assert ap(O.FINALLY, ap(O.FINALLY, p)) == ap(O.FINALLY, p)
+            assert ap(O.GLOBALLY, ap(O.GLOBALLY, p)) == ap(O.GLOBALLY, p)
+
+
+def test_splitmix64_published_outputs():
+    # SplitMix64 seeded with 0 yields mix(k * 0x9e3779b97f4a7c15) for k = 1, 2, 3
+    g = 0x9E3779B97F4A7C15
+    assert O.mix64(g) == 0xE220A8397B1DCDAF
+    assert O.mix64((2 * g) & M64) == 0x6E789E6AA1B965F4
+    assert O.mix64((3 * g) & M64) == 0x06C45D188009454F
+
+
+def test_key_type3_known_answer():
+    g = 0x9E3779B97F4A7C15
+    # two traces of length 64 (lenSum 128 > 126): trace 0 feeds hi, trace 1 lo
+    hi, lo = O.key([g, (2 * g) & M64], [64, 64], 3)
+    assert (hi, lo) == (0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4)
+    # equal words in one half cancel (XOR), a real collision of the hashed key
+    w = [5, 5, 9, 1]
+    assert O.key(w, [40, 40, 40, 40], 3) == O.key([7, 7, 9, 1], [40, 40, 40, 40], 3)
+
+
+def test_key_exact_is_injective_and_decodes():
+    rng = random.Random(11)
+    for _ in range(300):
+        T = rng.randint(1, 12)
+        budget = rng.randint(T, 126)
+        lens = []
+        for t in range(T):
+            room = budget - sum(lens) - (T - t - 1)
+            lens.append(rng.randint(1, max(1, min(64, room))))
+        words = [rng.getrandbits(n) for n in lens]
+        hi, lo = O.key(words, lens, 3)
+        assert hi >> 63 == 0 and lo >> 63 == 0
+        bits = lo | (hi << 63)
+        j = 0
+        for t, n in enumerate(lens):
+            assert (bits >> j) & mask(n) == words[t]
+            j += n
+
+
+def test_key_type1_known_answer():
+    # lenSum = 2 * 64 + 1 = 129 -> stride 129 // 126 = 1: the first 126
+    # trace bits in order; trace 2 (length 1) is beyond bit 126 and dropped
+    lens = [64, 64, 1]
+    w0, w1 = (1 << 63) | 1, (1 << 61) | (1 << 62) | 4
+    hi, lo = O.key([w0, w1, 1], lens, 1)
+    # global bits: trace0 bits 0..63 -> j 0..63, trace1 bits 0..61 -> j 64..125
+    # j=0 -> lo bit 0; j=63 -> hi bit 0; trace1 bit2 -> j=66 -> hi bit 3;
+    # trace1 bit 61 -> j=125 -> hi bit 62; trace1 bit 62 -> j=126 dropped
+    assert lo == 1
+    assert hi == (1 | (1 << 3) | (1 << 62))
+    # stride 2: lenSum = 4 * 63 = 252 -> stride 2, samples k = 0, 2, ..., 62
+    lens = [63, 63, 63, 63]
+    w = [0b101, 0b10, 1 << 62, (1 << 60) | 1]
+    hi, lo = O.key(w, lens, 1)
+    # trace0: k=0 -> j0, k=2 -> j1; trace1: 32 samples, k=1 not sampled;
+    # trace2: k=62 is sample 31 -> j = 64 + 31 = 95 -> hi bit 32;
+    # trace3: k=0 -> j=96 -> hi bit 33; k=60 -> sample 30 -> j=126 dropped
+    assert lo == 0b11
+    assert hi == (1 << 32) | (1 << 33)
+
+
+def test_key_type2_known_answer():
+    lens = [64, 63]
+    # lens 64, 63 -> lenSum 127: maxbits 64 and 63; j: trace0 0..63, trace1 64..126 (126 dropped)
+    w0 = (1 << 63) | (1 << 62)
+    w1 = (1 << 62) | (1 << 61) | 1
+    hi, lo = O.key([w0, w1], lens, 2)
+    # trace0 bit 62 -> j 62 -> lo bit 62; bit 63 -> j 63 -> hi bit 0
+    # trace1 bit 0 -> j 64 -> hi bit 1; bit 61 -> j 125 -> hi bit 62; bit 62 -> j 126 dropped
+    assert lo == 1 << 62
+    assert hi == 1 | 2 | (1 << 62)
+
+
+def _rand_inst(rng, A, T, lmin, lmax, p=0.5):
+    lens = [rng.randint(lmin, lmax) for _ in range(T)]
+    atoms = [[sum((rng.random() < p) << k for k in range(lens[t])) for t in range(T)] for _ in range(A)]
+    return O.Instance([f"p{a}" for a in range(A)], lens, atoms, rng.randint(0, T))
+
+
+def test_example_bruteforce_vs_cs():
+    inst = O.Instance.from_json(os.path.join(GOLDEN, "ExampleLTLF.json"))
+    bf = O.bruteforce(inst, 10)
+    cs = O.enumerate_cs(inst, 10, 3)
+    assert bf["status"] == cs["status"] == 0
+    assert bf["min_size"] == cs["min_size"] == 10
+    for r in (bf, cs):
+        assert separates(r["formula"], inst.names, inst.lens, inst.atoms, inst.P)
+        assert size(parse(r["formula"], inst.names)) == 10
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_min_size_matches_bruteforce(seed):
+    rng = random.Random(seed)
+    inst = _rand_inst(rng, rng.randint(1, 3), rng.randint(1, 6), 1, 6)
+    ms = 6
+    bf = O.bruteforce(inst, ms)
+    for mode in (3, O.KEY_FULLROW):
+        cs = O.enumerate_cs(inst, ms, mode)
+        assert cs["status"] == bf["status"], (seed, mode)
+        assert cs["min_size"] == bf["min_size"], (seed, mode)
+        if cs["status"] == 0:
+            assert separates(cs["formula"], inst.names, inst.lens, inst.atoms, inst.P)
+            assert size(parse(cs["formula"], inst.names)) == cs["min_size"]
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_exact_counts_equal_semantic_dedup(seed):
+    rng = random.Random(100 + seed)
+    T = rng.randint(2, 12)
+    inst = _rand_inst(rng, rng.randint(1, 4), T, 1, max(1, 126 // T))
+    assert sum(inst.lens) <= 126
+    a = O.enumerate_cs(inst, 7, 3)
+    b = O.enumerate_cs(inst, 7, O.KEY_FULLROW)
+    for f in ("status", "min_size", "distinct", "candidates", "formula", "found_ordinal"):
+        assert a[f] == b[f], (seed, f)
+
+
+def test_candidate_counts_closed_form():
+    # candidates of level s: 4 * n(s-1) + 4 * sum_{i <= s-1-i} n(i) n(s-1-i)
+    # trace 0 is repeated as the last (negative) trace: nothing separates them,
+    # so every level is enumerated completely
+    rng = random.Random(3)
+    inst = _rand_inst(rng, 3, 5, 3, 8)
+    inst = O.Instance(inst.names, inst.lens + [inst.lens[0]], [row + [row[0]] for row in inst.atoms], 1)
+    r = O.enumerate_cs(inst, 6, O.KEY_FULLROW)
+    assert r["status"] == 1
+    n = r["distinct"]
+    for s in range(2, 7):
+        want = 4 * n[s - 1] + 4 * sum(n[i] * n[s - 1 - i] for i in range(1, s) if 2 * i <= s - 1)
+        assert r["candidates"][s] == want
+
+
+def _count_trees(A, s):
+    """Direct enumeration of syntax trees in the ltl_nocache numbering's
+    grammar: unary ~ X F G, & and | with left size <= right size... as the
+    numbering counts them (left size j <= (s-1)//2), U with any split."""
+    memo = {}
+
+    def trees(k):
+        if k in memo:
+            return memo[k]
+        out = []
+        if k == 1:
+            out = [("a", i) for i in range(A)]
+        else:
+            for op in "~XFG":
+                out += [(op, t) for t in trees(k - 1)]
+            for j in range(1, k - 1):
+                for op in "&|U":
+                    if op != "U" and j > (k - 1) // 2:
+                        continue
+                    for l, r in itertools.product(trees(j), trees(k - 1 - j)):
+                        out.append((op, l, r))
+        memo[k] = out
+        return out
+
+    return len(trees(s))
+
+
+def test_num_formulas_closed_form():
+    for A in (1, 2, 3):
+        nf = O.num_formulas(A, 6)
+        for s in range(1, 7):
+            assert nf[s][7] == _count_trees(A, s), (A, s)

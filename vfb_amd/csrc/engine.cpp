@@ -167,6 +167,12 @@ class Engine {
     void* d_aux_ = nullptr;
     size_t aux_cap_ = 0;
     unsigned int* d_flag_ = nullptr;
+    char* pin_ = nullptr;        // pinned staging for host->device uploads,
+    size_t pin_cap_ = 0;         // recycled only at stream sync points
+    size_t pin_used_ = 0;
+
+    void sync();
+    void h2d(void* dst, const void* src, size_t bytes);
 
     void dalloc(Inst& I, Buf& b, size_t bytes);
     void dfree(Inst& I, Buf& b);
@@ -201,6 +207,7 @@ Engine::~Engine() {
     if (d_ctr_) (void)hipFree(d_ctr_);
     if (d_aux_) (void)hipFree(d_aux_);
     if (d_flag_) (void)hipFree(d_flag_);
+    if (pin_) (void)hipHostFree(pin_);
     arena_.reset();
     if (ev_[0]) (void)hipEventDestroy(ev_[0]);
     if (ev_[1]) (void)hipEventDestroy(ev_[1]);
@@ -231,8 +238,31 @@ void Engine::ensure(Inst& I, Buf& b, size_t bytes) {
     dalloc(I, b, bytes);
 }
 
+void Engine::sync() {
+    CK(hipStreamSynchronize(st_));
+    pin_used_ = 0;
+}
+
+void Engine::h2d(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (pin_used_ + need > pin_cap_) {
+        sync();
+        if (need > pin_cap_) {
+            if (pin_) CK(hipHostFree(pin_));
+            pin_ = nullptr;
+            pin_cap_ = std::max(need, std::max<size_t>(pin_cap_ * 2, 1 << 22));
+            CK(hipHostMalloc((void**)&pin_, pin_cap_, hipHostMallocDefault));
+        }
+    }
+    std::memcpy(pin_ + pin_used_, src, bytes);
+    CK(hipMemcpyAsync(dst, pin_ + pin_used_, bytes, hipMemcpyHostToDevice, st_));
+    pin_used_ += need;
+}
+
 void* Engine::aux(size_t bytes) {
     if (bytes > aux_cap_) {
+        sync();
         if (d_aux_) CK(hipFree(d_aux_));
         aux_cap_ = std::max(bytes, aux_cap_ * 2);
         CK(hipMalloc(&d_aux_, aux_cap_));
@@ -347,7 +377,7 @@ void Engine::setup(Inst& I, uint32_t idx) {
     for (int a = 0; a < A; ++a)
         for (int t = 0; t < T; ++t) col[(size_t)t * A + a] = in.atoms[(size_t)a * T + t] & len_mask(in.lens[t]);
     dalloc(I, I.rows[1], col.size() * 8);
-    CK(hipMemcpyAsync(I.rows[1].p, col.data(), col.size() * 8, hipMemcpyHostToDevice, st_));
+    h2d(I.rows[1].p, col.data(), col.size() * 8);
     I.nrows[1] = (uint32_t)A;
     I.level = 1;
     I.gbase = (uint64_t)A;
@@ -479,7 +509,7 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
                          d_flag_, grid, st_));
         unsigned int fail = 0;
         CK(hipMemcpyAsync(&fail, d_flag_, sizeof fail, hipMemcpyDeviceToHost, st_));
-        CK(hipStreamSynchronize(st_));
+        sync();
         if (fail) throw CapacityError("hash set rebuild failed");
         dfree(I, I.table);
     }
@@ -494,11 +524,13 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
 void Engine::upload_descs(std::vector<Inst*>& v) {
     const size_t n = v.size();
     if (n > desc_cap_) {
+        sync();
         if (d_desc_) CK(hipFree(d_desc_));
         CK(hipMalloc(&d_desc_, n * sizeof(InstDesc)));
         desc_cap_ = n;
     }
     if (n > ctr_cap_) {
+        sync();
         if (d_ctr_) CK(hipFree(d_ctr_));
         CK(hipMalloc(&d_ctr_, n * sizeof(Ctr)));
         ctr_cap_ = n;
@@ -508,7 +540,7 @@ void Engine::upload_descs(std::vector<Inst*>& v) {
         v[k]->h.ctr = d_ctr_ + k;
         hd[k] = v[k]->h;
     }
-    CK(hipMemcpyAsync(d_desc_, hd.data(), n * sizeof(InstDesc), hipMemcpyHostToDevice, st_));
+    h2d(d_desc_, hd.data(), n * sizeof(InstDesc));
 }
 
 std::string Engine::node_string(Inst& I, int level, uint64_t row) {
@@ -578,7 +610,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             CK(launch_insert_atoms(d_desc_, (uint32_t)k, act[k]->nrows[1], st_));
         std::vector<Ctr> hc(act.size());
         CK(hipMemcpyAsync(hc.data(), d_ctr_, hc.size() * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
-        CK(hipStreamSynchronize(st_));
+        sync();
         for (size_t k = 0; k < act.size(); ++k) {
             if (hc[k].overflow) throw DeviceError("atom insertion overflow");
             act[k]->committed = hc[k].claims;
@@ -606,7 +638,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             upload_descs(pend);
             std::vector<Ctr> init(pend.size());
             for (auto& c : init) { c = Ctr{}; c.found = kEmpty; }
-            CK(hipMemcpyAsync(d_ctr_, init.data(), init.size() * sizeof(Ctr), hipMemcpyHostToDevice, st_));
+            h2d(d_ctr_, init.data(), init.size() * sizeof(Ctr));
             std::vector<Task> tasks;
             uint64_t waves = 0;
             for (size_t k = 0; k < pend.size(); ++k)
@@ -618,7 +650,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
                 }
             if (waves) {
                 Task* d_tasks = (Task*)aux(tasks.size() * sizeof(Task));
-                CK(hipMemcpyAsync(d_tasks, tasks.data(), tasks.size() * sizeof(Task), hipMemcpyHostToDevice, st_));
+                h2d(d_tasks, tasks.data(), tasks.size() * sizeof(Task));
                 int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * 8);
                 timed_begin();
                 CK(launch_expand(d_desc_, d_tasks, (int)tasks.size(), waves, grid, st_));
@@ -631,7 +663,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             }
             std::vector<Ctr> hc(pend.size());
             CK(hipMemcpyAsync(hc.data(), d_ctr_, hc.size() * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
-            CK(hipStreamSynchronize(st_));
+            sync();
             std::vector<Inst*> again;
             for (size_t k = 0; k < pend.size(); ++k) {
                 Inst& I = *pend[k];
@@ -667,7 +699,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             // but instances re-run individually had their own slot index)
             std::vector<Ctr> hc(live.size());
             for (size_t k = 0; k < live.size(); ++k) hc[k] = live[k]->ctr;
-            CK(hipMemcpyAsync(d_ctr_, hc.data(), hc.size() * sizeof(Ctr), hipMemcpyHostToDevice, st_));
+            h2d(d_ctr_, hc.data(), hc.size() * sizeof(Ctr));
         }
 
         // ---- 4. resolve, count, scan ----
@@ -684,9 +716,9 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             size_t off1 = fsegs.size() * sizeof(Seg);
             size_t off2 = off1 + bsegs.size() * sizeof(Seg);
             char* a = (char*)aux(off2 + ids.size() * 4 + 64);
-            if (!fsegs.empty()) CK(hipMemcpyAsync(a, fsegs.data(), off1, hipMemcpyHostToDevice, st_));
-            if (!bsegs.empty()) CK(hipMemcpyAsync(a + off1, bsegs.data(), off2 - off1, hipMemcpyHostToDevice, st_));
-            CK(hipMemcpyAsync(a + off2, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st_));
+            if (!fsegs.empty()) h2d(a, fsegs.data(), off1);
+            if (!bsegs.empty()) h2d(a + off1, bsegs.data(), off2 - off1);
+            h2d(a + off2, ids.data(), ids.size() * 4);
             timed_begin();
             if (ftot) {
                 int grid = (int)std::min<uint64_t>((ftot + 255) / 256, (uint64_t)cus_ * 16);
@@ -704,7 +736,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             }
             std::vector<Ctr> hc(live.size());
             CK(hipMemcpyAsync(hc.data(), d_ctr_, hc.size() * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
-            CK(hipStreamSynchronize(st_));
+            sync();
             for (size_t k = 0; k < live.size(); ++k) live[k]->ctr = hc[k];
         }
 
@@ -762,7 +794,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
                 }
             if (tot) {
                 Seg* d = (Seg*)aux(segs.size() * sizeof(Seg));
-                CK(hipMemcpyAsync(d, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice, st_));
+                h2d(d, segs.data(), segs.size() * sizeof(Seg));
                 int grid = (int)std::min<uint64_t>(tot, (uint64_t)cus_ * 8);
                 timed_begin();
                 CK(launch_store(d_desc_, d, (int)segs.size(), tot, grid, st_));
@@ -786,7 +818,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             if (!I->done) nxt.push_back(I);
         act.swap(nxt);
     }
-    CK(hipStreamSynchronize(st_));
+    sync();
 }
 
 static std::mutex g_mu;
