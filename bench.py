@@ -1,0 +1,160 @@
+"""Benchmark of the CS enumeration hot path on one MI355X.
+
+Workload (the dedup-stress configuration): instances 0..M-1 of synthetic
+config 2 -- 5 atomic propositions, 40 traces (20 positive / 20 negative),
+lengths uniform in [16, 32], bits Bernoulli(0.3), labels from a planted
+size-9 formula containing Until, max_formula_size 14, hashed 128-bit keys
+(total trace length > 126 bits).  One step = solving all M instances, each
+as a plain single-instance solve (the drop-in path).  Inputs are resident
+(generated on the host before the timed region; each solve uploads its atom
+rows, a few KB).
+
+Prints ONE JSON line: distinct CS/s over the timed steps, plus batched vs
+plain instances/s (checked bit-exact), p50/p99 solve latency, the roofline
+of the combine+key+dedup kernel (k_expand) from HIP events, and the CPU
+baseline (the single-threaded oracle on a bounded sample).
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * q
+    f = int(k)
+    c = min(f + 1, len(xs) - 1)
+    return xs[f] + (xs[c] - xs[f]) * (k - f)
+
+
+def cpu_baseline(insts, max_size, budget_s):
+    """Oracle (single thread) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    done, distinct, t_total = 0, 0, 0.0
+    for inst in insts:
+        names, lens, atoms, P = inst.arrays()
+        t0 = time.perf_counter()
+        r = O.enumerate_cs(O.Instance(names, lens, atoms, P), max_size, 3)
+        t_total += time.perf_counter() - t0
+        distinct += sum(r["distinct"])
+        done += 1
+        if t_total > budget_s:
+            break
+    return {"value": distinct / t_total if t_total else None, "unit": "distinct CS/s", "cores": 1,
+            "kind": "port",
+            "sample": f"config-2 instances 0..{done - 1} at max_formula_size {max_size} "
+                      f"({distinct} distinct CS in {t_total:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=32)
+    ap.add_argument("--max-size", type=int, default=14)
+    ap.add_argument("--cpu-max-size", type=int, default=14)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(instances shard across GPUs with no exchange; see DESIGN.md)"}))
+        sys.exit(2)
+
+    import vfb_amd as V
+    dev = V.device_info()
+    if dev is None:
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    print(f"[bench] device {dev}", file=sys.stderr, flush=True)
+    insts = [V.Instance.generate(2, k) for k in range(args.instances)]
+
+    def step():
+        res = []
+        for inst in insts:
+            res.append(V.solve(inst, max_size=args.max_size, timing=True))
+        return res
+
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        step()
+        print(f"[bench] warmup {w} {time.perf_counter() - t0:.3f}s", file=sys.stderr, flush=True)
+
+    step_times, all_res = [], []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        res = step()
+        dt = time.perf_counter() - t0
+        step_times.append(dt)
+        all_res.append(res)
+        print(f"[bench] step {s} {dt:.3f}s", file=sys.stderr, flush=True)
+    T = sum(step_times)
+    distinct = sum(sum(r["level_winners"]) for res in all_res for r in res)
+    cands = sum(sum(r["level_candidates"]) for res in all_res for r in res)
+    lat = [r["ms_total"] for res in all_res for r in res]
+    plain = all_res[-1]
+    statuses = {}
+    for r in plain:
+        statuses[r["status"]] = statuses.get(r["status"], 0) + 1
+
+    # batched: the same instances stepped together; must equal the plain runs
+    V.solve_batch(insts, max_size=args.max_size)  # warm
+    t0 = time.perf_counter()
+    batch = V.solve_batch(insts, max_size=args.max_size)
+    t_batch = time.perf_counter() - t0
+    exact = all(a[f] == b[f] for a, b in zip(plain, batch)
+                for f in ("status", "min_size", "formula", "distinct", "candidates", "level_winners"))
+    t_plain = step_times[-1]
+
+    bytes_exp = sum(r["bytes_expand"] for res in all_res for r in res)
+    ms_exp = sum(r["ms_expand"] for res in all_res for r in res)
+    launches = sum(r["expand_launches"] for res in all_res for r in res)
+    achieved = bytes_exp / (ms_exp * 1e-3) / 1e9 if ms_exp else None
+
+    out = {
+        "metric": "distinct CS/sec + instances/sec batched vs plain (bit-exact); p50/p99 solve latency",
+        "value": distinct / T,
+        "unit": "distinct CS/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * T / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": f"config 2 dedup stress: {args.instances} instances "
+                               f"(5 APs, 40 traces 20+/20-, len U[16,32], p=0.3, planted size 9 with U), "
+                               f"plain single-instance solves, max_formula_size {args.max_size}, "
+                               f"relaxed 128-bit hashed keys",
+                   "instances": args.instances, "max_formula_size": args.max_size,
+                   "parallelism": "single GPU"},
+        "candidates_per_s": cands / T,
+        "instances_per_s_plain": len(insts) / t_plain,
+        "instances_per_s_batched": len(insts) / t_batch,
+        "batched_vs_plain": t_plain / t_batch,
+        "batched_bit_exact": exact,
+        "latency_ms_p50": pct(lat, 0.5), "latency_ms_p99": pct(lat, 0.99),
+        "statuses": {V.status_string(k): v for k, v in statuses.items()},
+        "roofline": {"bound": "hbm", "kernel": "k_expand", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": None,
+                     "launches": launches, "ms_per_launch": ms_exp / launches if launches else None,
+                     "alg_bytes_per_launch": bytes_exp / launches if launches else None},
+        "device": dev,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(insts, args.cpu_max_size, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

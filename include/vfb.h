@@ -106,6 +106,10 @@ typedef struct {
     uint64_t expand_launches;
     uint64_t table_slots;  /* final hash-set capacity */
     uint64_t retries;      /* epochs re-run after a table or list grew */
+    uint64_t bytes_expand; /* algorithmic bytes of the combine+key+dedup kernel:
+                              (a*8T + 16) per candidate, a = 2 for & and |, else 1 */
+    uint64_t bytes_store;  /* algorithmic bytes of survivor storage: 8T + 24 per survivor */
+    uint64_t fresh_total;  /* candidates whose key was new to the set at expand time */
 } vfb_result;
 
 void vfb_options_default(vfb_options* opt);

@@ -38,7 +38,8 @@ class Result(C.Structure):
                 ("ms_total", C.c_double), ("ms_expand", C.c_double),
                 ("ms_resolve", C.c_double), ("ms_store", C.c_double),
                 ("expand_launches", C.c_uint64), ("table_slots", C.c_uint64),
-                ("retries", C.c_uint64)]
+                ("retries", C.c_uint64), ("bytes_expand", C.c_uint64),
+                ("bytes_store", C.c_uint64), ("fresh_total", C.c_uint64)]
 
 
 # every symbol declared in include/vfb.h
@@ -199,7 +200,8 @@ def _todict(r):
         "ms_total": r.ms_total, "ms_expand": r.ms_expand,
         "ms_resolve": r.ms_resolve, "ms_store": r.ms_store,
         "expand_launches": r.expand_launches, "table_slots": r.table_slots,
-        "retries": r.retries,
+        "retries": r.retries, "bytes_expand": r.bytes_expand,
+        "bytes_store": r.bytes_store, "fresh_total": r.fresh_total,
     }
 
 

@@ -459,7 +459,7 @@ void Engine::prepare_level(Inst& I) {
     // scratch sized from the previous level's ratios, grown on overflow
     const uint64_t N = I.N;
     const uint64_t nwords = (N + 63) / 64;
-    const uint64_t nblk = (nwords + kScanWords - 1) / kScanWords;
+    const uint64_t nblk = (nwords + kChunkWords - 1) / kChunkWords;
     uint64_t fcap = std::min<uint64_t>(N, std::max<uint64_t>(1 << 16, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
     if (fcap > I.fresh_cap) {
         ensure(I, I.fresh_ord, fcap * 8);
@@ -725,7 +725,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
                 CK(launch_resolve(d_desc_, (const Seg*)a, (int)fsegs.size(), ftot, grid, st_));
             }
             if (btot) {
-                int grid = (int)std::min<uint64_t>(btot, (uint64_t)cus_ * 8);
+                int grid = (int)std::min<uint64_t>((btot + 3) / 4, (uint64_t)cus_ * 8);
                 CK(launch_count(d_desc_, (const Seg*)(a + off1), (int)bsegs.size(), btot, grid, st_));
             }
             CK(launch_scan(d_desc_, (const uint32_t*)(a + off2), (int)ids.size(), st_));
@@ -748,6 +748,11 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             const int s = I.level + 1;
             R.level_candidates[s] = I.N;
             R.level_winners[s] = I.ctr.winners;
+            R.fresh_total += I.ctr.fresh_count;
+            for (const Unit& u : I.units[s]) {
+                const uint64_t a = (u.op == VFB_OP_AND || u.op == VFB_OP_OR) ? 2 : 1;
+                R.bytes_expand += u.count * (a * 8 * (uint64_t)I.h.T + 16);
+            }
             if (opt_.verbose)
                 std::fprintf(stderr, "Cost %-2d | candidates: %-14llu | fresh: %-12llu | distinct: %-12llu | table: %llu\n", s,
                              (unsigned long long)I.N, (unsigned long long)I.ctr.fresh_count,
@@ -765,6 +770,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             R.candidates[s] = I.N;
             R.distinct[s] = I.ctr.winners;
             R.levels_done = s;
+            R.bytes_store += I.ctr.winners * (8 * (uint64_t)I.h.T + 24);
             if (s >= I.max_size) {
                 finish(I, VFB_NOT_FOUND);
                 continue;
@@ -790,12 +796,12 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             for (size_t k = 0; k < store.size(); ++k)
                 if (store[k]->nrows[store[k]->level + 1] && store[k]->h.nblk) {
                     segs.push_back(Seg{(uint32_t)k, 0, tot});
-                    tot += store[k]->h.nblk;
+                    tot += store[k]->h.nblk * (uint64_t)((store[k]->h.T + kStoreTraces - 1) / kStoreTraces);
                 }
             if (tot) {
                 Seg* d = (Seg*)aux(segs.size() * sizeof(Seg));
                 h2d(d, segs.data(), segs.size() * sizeof(Seg));
-                int grid = (int)std::min<uint64_t>(tot, (uint64_t)cus_ * 8);
+                int grid = (int)std::min<uint64_t>((tot + 3) / 4, (uint64_t)cus_ * 8);
                 timed_begin();
                 CK(launch_store(d_desc_, d, (int)segs.size(), tot, grid, st_));
                 if (opt_.timing) {

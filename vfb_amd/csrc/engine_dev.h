@@ -26,8 +26,9 @@ constexpr int kMaxS = VFB_MAX_SIZE;
 constexpr int kMaxUnits = 4 + 3 * (kMaxS / 2);   // ~,X,F,G + (&,|,U) per split
 constexpr uint64_t kEmpty = ~0ull;
 constexpr uint64_t kMatchBit = 1ull << 63;
-constexpr int kScanWords = 4096;       // bitmap words per scan block (262144 ordinals)
+constexpr int kChunkWords = 64;        // bitmap words per scan chunk (4096 ordinals)
 constexpr int kProbeLimit = 4096;
+constexpr int kStoreTraces = 8;       // traces per k_store work item
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 

@@ -370,33 +370,22 @@ __global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D,
     }
 }
 
-__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
-    // 256 threads = 4 wavefronts
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[w] = v;
-    __syncthreads();
-    return sh[0] + sh[1] + sh[2] + sh[3];
-}
-
-// popcount of each 4096-word block of every instance's winner bitmap
+// popcount of each chunk (64 bitmap words = 4096 ordinals) of every
+// instance's winner bitmap; one wavefront per chunk
 __global__ __launch_bounds__(256) void k_count(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
-                                               int nseg, uint64_t total_blocks) {
-    __shared__ uint32_t sh[4];
-    for (uint64_t g = blockIdx.x; g < total_blocks; g += gridDim.x) {
+                                               int nseg, uint64_t total_chunks) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t wpb = blockDim.x >> 6;
+    const uint64_t stride = (uint64_t)gridDim.x * wpb;
+    for (uint64_t g = blockIdx.x * wpb + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); g < total_chunks;
+         g += stride) {
         const Seg& sg = segs[seg_find(segs, nseg, g)];
         const InstDesc& d = D[sg.inst];
-        const uint64_t b = g - sg.begin;
-        const uint64_t w0 = b * kScanWords;
-        uint32_t c = 0;
-        for (int i = threadIdx.x; i < kScanWords; i += 256) {
-            const uint64_t w = w0 + i;
-            if (w < d.nwords) c += __popcll(d.winbits[w]);
-        }
-        c = block_sum_u32(c, sh);
-        if (threadIdx.x == 0) d.blkcount[b] = c;
-        __syncthreads();
+        const uint64_t c = g - sg.begin;
+        const uint64_t w = c * kChunkWords + lane;
+        uint32_t v = w < d.nwords ? (uint32_t)__popcll(d.winbits[w]) : 0u;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) d.blkcount[c] = v;
     }
 }
 
@@ -409,12 +398,11 @@ __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
     return v;
 }
 
-// one workgroup per instance: exclusive scan of the block counts, the level's
+// one workgroup per instance: exclusive scan of the chunk counts, the level's
 // survivor total, and the rank of the returned formula among the survivors
 __global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, const uint32_t* __restrict__ insts) {
     __shared__ uint64_t wsum[4];
     __shared__ uint64_t carry;
-    __shared__ uint32_t sh[4];
     const InstDesc& d = D[insts[blockIdx.x]];
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
@@ -432,38 +420,32 @@ __global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, co
         if (threadIdx.x == 255) carry = pre + inc;
         __syncthreads();
     }
-    const uint64_t total = carry;
-    const uint64_t f = d.ctr->found;
-    uint64_t rank = 0;
-    if (f != kEmpty) {
-        const uint64_t fw = f >> 6, fb = fw / kScanWords;
-        uint32_t c = 0;
-        for (uint64_t x = fb * kScanWords + threadIdx.x; x < fw; x += 256) c += __popcll(d.winbits[x]);
-        c = block_sum_u32(c, sh);
-        rank = d.blkprefix[fb] + c + __popcll(d.winbits[fw] & ((1ull << (f & 63)) - 1ull));
-    }
-    if (threadIdx.x == 0) {
-        d.ctr->winners = total;
-        d.ctr->found_rank = rank;
+    if (threadIdx.x < 64) {
+        const uint64_t f = d.ctr->found;
+        uint64_t rank = 0;
+        if (f != kEmpty) {
+            const uint64_t fw = f >> 6, fc = fw / kChunkWords;
+            const uint64_t w = fc * kChunkWords + lane;
+            uint64_t bits = w < fw ? d.winbits[w] : (w == fw ? d.winbits[w] & ((1ull << (f & 63)) - 1ull) : 0ull);
+            uint32_t v = (uint32_t)__popcll(bits);
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            rank = d.blkprefix[fc] + v;
+        }
+        if (threadIdx.x == 0) {
+            d.ctr->winners = carry;
+            d.ctr->found_rank = rank;
+        }
     }
 }
 
-// recomputes the CS word of trace t for the candidate with level-local ordinal o
-__device__ __forceinline__ uint64_t cs_of(const InstDesc& d, const Unit& u, uint64_t o, int t) {
-    const uint64_t k = o - u.base;
-    const uint64_t m = d.lenmask[t];
-    const unsigned long long* L = d.lvl[u.lvl_l];
-    if (u.op == VFB_OP_NOT || u.op == VFB_OP_NEXT || u.op == VFB_OP_FINALLY || u.op == VFB_OP_GLOBALLY)
-        return apply_op(u.op, L[(uint64_t)t * u.nl + k], 0, m);
-    const unsigned long long* R = d.lvl[u.lvl_r];
-    if (u.op == VFB_OP_UNTIL) {
-        const uint64_t p = k >> 1, l = p / u.nr, r = p - l * u.nr;
-        const uint64_t a = L[(uint64_t)t * u.nl + l], b = R[(uint64_t)t * u.nr + r];
-        return (k & 1) ? until_op(b, a) : until_op(a, b);
-    }
-    const uint64_t l = k / u.nr, r = k - l * u.nr;
-    return apply_op(u.op, L[(uint64_t)t * u.nl + l], R[(uint64_t)t * u.nr + r], m);
-}
+// operands of the candidate with level-local ordinal o
+struct Cand {
+    int op;
+    const unsigned long long* L;
+    const unsigned long long* R;
+    uint64_t nl, nr, l, r;
+    bool swap;
+};
 
 __device__ __forceinline__ int unit_of(const InstDesc& d, uint64_t o) {
     int lo = 0, hi = d.nunits - 1;
@@ -471,52 +453,85 @@ __device__ __forceinline__ int unit_of(const InstDesc& d, uint64_t o) {
         const int mid = (lo + hi + 1) >> 1;
         if (d.units[mid].base <= o) lo = mid; else hi = mid - 1;
     }
-    // skip empty units that share a base with the next one
-    while (lo + 1 < d.nunits && d.units[lo].count == 0) ++lo;
     return lo;
 }
 
-// stable compaction of the survivors: 64 bitmap words per round are expanded
-// into an LDS list of ordinals, then 256 lanes write consecutive rows
+__device__ __forceinline__ Cand decode(const InstDesc& d, uint64_t o) {
+    const Unit& u = d.units[unit_of(d, o)];
+    Cand c;
+    c.op = u.op;
+    c.L = d.lvl[u.lvl_l];
+    c.R = d.lvl[u.lvl_r];
+    c.nl = u.nl;
+    c.nr = u.nr;
+    c.swap = false;
+    const uint64_t k = o - u.base;
+    if (u.op == VFB_OP_NOT || u.op == VFB_OP_NEXT || u.op == VFB_OP_FINALLY || u.op == VFB_OP_GLOBALLY) {
+        c.l = k;
+        c.r = 0;
+        c.R = c.L;
+    } else {
+        const uint64_t p = u.op == VFB_OP_UNTIL ? k >> 1 : k;
+        c.l = p / u.nr;
+        c.r = p - c.l * u.nr;
+        c.swap = u.op == VFB_OP_UNTIL && (k & 1);
+    }
+    return c;
+}
+
+__device__ __forceinline__ uint64_t cand_word(const Cand& c, int t, uint64_t m) {
+    const uint64_t a = c.L[(uint64_t)t * c.nl + c.l];
+    if (c.op == VFB_OP_NOT || c.op == VFB_OP_NEXT || c.op == VFB_OP_FINALLY || c.op == VFB_OP_GLOBALLY)
+        return apply_op(c.op, a, 0, m);
+    const uint64_t b = c.R[(uint64_t)t * c.nr + c.r];
+    if (c.op == VFB_OP_UNTIL) return c.swap ? until_op(b, a) : until_op(a, b);
+    return apply_op(c.op, a, b, m);
+}
+
+// stable compaction of the survivors.  Work item = (chunk of 4096 ordinals,
+// slice of kStoreTraces traces), one wavefront each: the chunk's set bits
+// become an LDS list of 12-bit offsets, then the 64 lanes write consecutive
+// rows of the slice's columns (coalesced stores).  Slice 0 also writes the
+// parent ordinals.
 __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
-                                               int nseg, uint64_t total_blocks) {
-    __shared__ uint64_t list[64 * 64];
-    __shared__ uint32_t cnt;
+                                               int nseg, uint64_t total_items) {
+    __shared__ uint16_t lists[4][64 * kChunkWords];
     const int lane = threadIdx.x & 63;
-    for (uint64_t g = blockIdx.x; g < total_blocks; g += gridDim.x) {
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint16_t* list = lists[wv];
+    const uint64_t wpb = blockDim.x >> 6;
+    const uint64_t stride = (uint64_t)gridDim.x * wpb;
+    for (uint64_t g = blockIdx.x * wpb + wv; g < total_items; g += stride) {
         const Seg& sg = segs[seg_find(segs, nseg, g)];
         const InstDesc& d = D[sg.inst];
-        const uint64_t b = g - sg.begin;
-        if (d.blkcount[b] == 0) continue;
-        uint64_t rank = d.blkprefix[b];
-        const uint64_t w0 = b * kScanWords;
-        const uint64_t w1 = w0 + kScanWords < d.nwords ? w0 + kScanWords : d.nwords;
-        for (uint64_t wr = w0; wr < w1; wr += 64) {
-            if (threadIdx.x < 64) {
-                const uint64_t w = wr + lane;
-                uint64_t bits = w < w1 ? d.winbits[w] : 0ull;
-                const uint64_t pc = __popcll(bits);
-                const uint64_t inc = wave_incl_scan(pc);
-                uint64_t pos = inc - pc;
-                while (bits) {
-                    const int q = __ffsll((unsigned long long)bits) - 1;
-                    bits &= bits - 1;
-                    list[pos++] = w * 64 + (uint64_t)q;
-                }
-                if (lane == 63) cnt = (uint32_t)inc;
-            }
-            __syncthreads();
-            const uint32_t n = cnt;
-            for (uint32_t i = threadIdx.x; i < n; i += 256) {
-                const uint64_t o = list[i];
-                const Unit& u = d.units[unit_of(d, o)];
-                const uint64_t row = rank + i;
-                for (int t = 0; t < d.T; ++t) d.out_rows[(uint64_t)t * d.out_n + row] = cs_of(d, u, o, t);
-                d.out_par[row] = o;
-            }
-            rank += n;
-            __syncthreads();
+        const uint32_t nsl = (uint32_t)(d.T + kStoreTraces - 1) / kStoreTraces;
+        const uint64_t item = g - sg.begin;
+        const uint64_t c = item / nsl;
+        const int t0 = (int)(item - c * nsl) * kStoreTraces;
+        const int t1 = t0 + kStoreTraces < d.T ? t0 + kStoreTraces : d.T;
+        const uint32_t n = d.blkcount[c];
+        if (n == 0) continue;
+        const uint64_t w = c * kChunkWords + lane;
+        uint64_t bits = w < d.nwords ? d.winbits[w] : 0ull;
+        const uint64_t pc = __popcll(bits);
+        uint32_t pos = (uint32_t)(wave_incl_scan(pc) - pc);
+        while (bits) {
+            const int q = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            list[pos++] = (uint16_t)(lane * 64 + q);
         }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const uint64_t rank0 = d.blkprefix[c];
+        const uint64_t obase = c * (uint64_t)(kChunkWords * 64);
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint64_t o = obase + list[i];
+            const Cand cd = decode(d, o);
+            const uint64_t row = rank0 + i;
+            for (int t = t0; t < t1; ++t) d.out_rows[(uint64_t)t * d.out_n + row] = cand_word(cd, t, d.lenmask[t]);
+            if (t0 == 0) d.out_par[row] = o;
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
