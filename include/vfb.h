@@ -83,7 +83,10 @@ typedef struct {
     int relaxed_type;      /* key when total trace length > 126: 1, 2 or 3 (LTL_BS.cu:39-92) */
     int verbose;           /* 1: per-level progress lines on stderr */
     int timing;            /* 1: time each kernel class with HIP events */
-    uint64_t hbm_budget;   /* bytes of device memory per solve; 0 = 90% of free */
+    uint64_t hbm_budget;   /* bytes of device memory per solve; 0 = the process arena
+                              (85% of the HBM free at first use) */
+    uint64_t max_rows;     /* cap on stored CS rows, the reference's LTLCacheCapacity
+                              (LTL_BS.cu:1182); 0 = limited by hbm_budget only */
 } vfb_options;
 
 typedef struct {
@@ -110,6 +113,11 @@ typedef struct {
                               (a*8T + 16) per candidate, a = 2 for & and |, else 1 */
     uint64_t bytes_store;  /* algorithmic bytes of survivor storage: 8T + 24 per survivor */
     uint64_t fresh_total;  /* candidates whose key was new to the set at expand time */
+    int otf_level;         /* level whose rows were truncated (onTheFly, LTL_BS.cu:516-522);
+                              the next level was then checked only and the search
+                              stopped (status VFB_CAPACITY if nothing matched); 0: none */
+    int pad;
+    uint64_t otf_stored;   /* rows of otf_level that were stored */
 } vfb_result;
 
 void vfb_options_default(vfb_options* opt);

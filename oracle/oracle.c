@@ -21,7 +21,9 @@
  *   - Among candidates with equal keys the lowest canonical ordinal survives,
  *     and the returned formula is the lowest-ordinal matching survivor
  *     (the reference takes whichever thread's atomicCAS lands first).
- *   - A full cache is reported as status 3 instead of switching to onTheFly.
+ *   - The row cap truncates the overflowing level in ordinal order after the
+ *     whole level was deduplicated and checked (the reference stops deduping
+ *     mid-level at a device-dependent point); the next level is check-only.
  */
 #define _POSIX_C_SOURCE 200809L
 #include "oracle.h"
@@ -326,7 +328,25 @@ typedef struct {
     int level;
     uint64_t local_ord;
     int done;            /* 1 found, 3 capacity, -1 alloc failure */
+    int check_only;      /* onTheFly level: target check only */
+    int max_size;
+    int otf;             /* rows were dropped at the cap */
 } enum_ctx;
+
+static void set_found(enum_ctx* c, const uint64_t* cs, int op, int l, int r, uint64_t ord)
+{
+    /* build the formula text from the candidate's parents */
+    rowstore* st = &c->st;
+    if (!rows_push(st, cs, op, l, r)) { c->done = -1; return; }
+    sbuf b = { c->out->formula, 0, sizeof c->out->formula, 0 };
+    c->out->formula[0] = 0;
+    row_to_string(st, (int)st->rows - 1, c->names, &b);
+    st->rows--;
+    c->out->min_size = c->level;
+    c->out->top_op = op;
+    c->out->found_ordinal = ord;
+    c->done = 1;
+}
 
 /* processes one candidate in canonical order; returns nonzero to stop */
 static int candidate(enum_ctx* c, const uint64_t* cs, int op, int l, int r)
@@ -335,6 +355,11 @@ static int candidate(enum_ctx* c, const uint64_t* cs, int op, int l, int r)
     int T = in->T;
     uint64_t ord = c->local_ord++;
     c->out->candidates[c->level]++;
+    if (c->check_only) {
+        /* processOnTheFly, LTL_BS.cu:331-349 */
+        if (matches_target(cs, T, in->P)) { set_found(c, cs, op, l, r, ord); return 1; }
+        return 0;
+    }
     int fresh;
     if (c->key_mode == ORA_KEY_FULLROW) {
         fresh = rowset_find_or_add(&c->rs, &c->st, cs, 0, 0);
@@ -345,26 +370,17 @@ static int candidate(enum_ctx* c, const uint64_t* cs, int op, int l, int r)
     }
     if (fresh < 0) { c->done = -1; return 1; }
     if (!fresh) return 0;
-    if (matches_target(cs, T, in->P)) {
-        /* build the formula text from the candidate's parents */
-        rowstore* st = &c->st;
-        if (!rows_push(st, cs, op, l, r)) { c->done = -1; return 1; }
-        sbuf b = { c->out->formula, 0, sizeof c->out->formula, 0 };
-        c->out->formula[0] = 0;
-        row_to_string(st, (int)st->rows - 1, c->names, &b);
-        st->rows--;
-        c->out->min_size = c->level;
-        c->out->top_op = op;
-        c->out->found_ordinal = ord;
-        c->done = 1;
-        return 1;
+    if (matches_target(cs, T, in->P)) { set_found(c, cs, op, l, r, ord); return 1; }
+    c->out->distinct[c->level]++;
+    if (c->st.rows >= c->max_rows && c->level < c->max_size) {
+        /* cache full: the survivor is checked but not stored (LTL_BS.cu:516-522) */
+        if (!c->otf) { c->otf = 1; c->out->otf_level = c->level; }
+        return 0;
     }
-    if (c->st.rows >= c->max_rows) { c->done = 3; return 1; }
     if (c->key_mode == ORA_KEY_FULLROW) {
         if (rowset_find_or_add(&c->rs, &c->st, cs, (int64_t)c->st.rows, 1) < 0) { c->done = -1; return 1; }
     }
     if (!rows_push(&c->st, cs, op, l, r)) { c->done = -1; return 1; }
-    c->out->distinct[c->level]++;
     return 0;
 }
 
@@ -379,6 +395,7 @@ int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
     memset(&c, 0, sizeof c);
     c.in = in; c.key_mode = key_mode; c.names = names; c.out = out;
     c.max_rows = max_rows ? max_rows : UINT64_MAX;
+    c.max_size = max_size;
     c.st.T = T;
     if (key_mode != ORA_KEY_FULLROW && !keyset_init(&c.ks, 1024)) return -1;
 
@@ -414,7 +431,9 @@ int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
     if (!c.done) out->levels_done = 1;
     start[2] = c.st.rows;
 
+    int last_round = 0;
     for (int s = 2; s <= max_size && !c.done; ++s) {
+        if (c.otf) { last_round = 1; c.check_only = 1; }
         c.level = s;
         c.local_ord = 0;
         const size_t u0 = start[s - 1], u1 = start[s];
@@ -461,8 +480,13 @@ int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
                 }
         }
 #undef ROW
-        if (!c.done) out->levels_done = s;
+        if (!c.done && !last_round) out->levels_done = s;
         start[s + 1] = c.st.rows;
+        if (out->otf_level == s) out->otf_stored = start[s + 1] - start[s];
+        if (last_round) {
+            if (!c.done) c.done = 3;
+            break;
+        }
     }
 
     free(cs);
@@ -471,7 +495,7 @@ int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
     free(c.st.cs); free(c.st.left); free(c.st.right); free(c.st.op);
     if (c.done == -1) { out->status = -1; return -1; }
     if (c.done == 1) { out->status = 0; return 0; }
-    if (c.done == 3) { out->status = 3; return 3; }
+    if (c.done == 3) { out->status = 3; snprintf(out->formula, sizeof out->formula, "Not found !"); return 3; }
     out->status = 1;
     snprintf(out->formula, sizeof out->formula, "Not found !");
     return 1;

@@ -48,6 +48,9 @@ typedef struct {
     uint64_t distinct[ORA_MAX_SIZE + 1];   /* survivors stored per level (found level: before the match) */
     uint64_t candidates[ORA_MAX_SIZE + 1]; /* candidates generated per level (found level: through the match) */
     uint64_t found_ordinal; /* level-local ordinal of the returned formula */
+    int otf_level;          /* level whose rows were truncated at max_rows (0: none) */
+    int pad;
+    uint64_t otf_stored;    /* rows of that level that were stored */
     char formula[8192];     /* ltl_nocache.cu:289-317 syntax */
 } ora_result;
 
@@ -59,6 +62,12 @@ void ora_key(const uint64_t* cs, const int* len, int T, int relaxed_type,
              uint64_t* hi, uint64_t* lo);
 
 /* ---- enumerators ---- */
+/* max_rows (0 = unlimited) caps the stored rows, the reference's
+   LTLCacheCapacity (LTL_BS.cu:1182): survivors of the level that overflows
+   are still checked, rows past the cap are dropped in ordinal order, the next
+   level is checked without dedup or storage over the stored rows (onTheFly,
+   LTL_BS.cu:331-349, 516-522) and the search stops there (lastRound,
+   LTL_BS.cu:1229-1233, 1490-1491) with status 3 if nothing matched. */
 int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
                   uint64_t max_rows, const char* const* names, ora_result* out);
 int ora_bruteforce(const ora_instance* in, int max_size,

@@ -27,6 +27,7 @@ class OraResult(C.Structure):
                 ("distinct", C.c_uint64 * (MAX_SIZE + 1)),
                 ("candidates", C.c_uint64 * (MAX_SIZE + 1)),
                 ("found_ordinal", C.c_uint64),
+                ("otf_level", C.c_int), ("pad", C.c_int), ("otf_stored", C.c_uint64),
                 ("formula", C.c_char * 8192)]
 
 
@@ -110,6 +111,7 @@ def _result(r):
         "levels_done": r.levels_done,
         "distinct": list(r.distinct), "candidates": list(r.candidates),
         "found_ordinal": r.found_ordinal,
+        "otf_level": r.otf_level, "otf_stored": r.otf_stored,
         "formula": r.formula.decode(),
     }
 

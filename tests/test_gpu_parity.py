@@ -96,3 +96,30 @@ def test_batch_equals_plain():
     for k, (a, b) in enumerate(zip(plain, batch)):
         for f in ("status", "min_size", "formula", "distinct", "candidates", "level_winners"):
             assert a[f] == b[f], (k, f, a[f], b[f])
+
+
+@pytest.mark.parametrize("cfg,k,max_rows", [(2, 2, 3000), (2, 2, 20000), (2, 3, 100000), (2, 21, 500),
+                                            (1, 0, 40), (3, 4, 5000), (2, 5, 130)])
+def test_overflow_onthefly(cfg, k, max_rows):
+    """Row cap (the reference's cache capacity): the overflowing level is stored
+    up to the cap in ordinal order, the next level is check-only."""
+    inst = V.Instance.generate(cfg, k)
+    names, lens, atoms, P = inst.arrays()
+    ms = 10
+    print(f"[parity] overflow cfg{cfg} #{k} max_rows={max_rows}", flush=True)
+    want = O.enumerate_cs(O.Instance(names, lens, atoms, P), ms, 3, max_rows)
+    got = V.solve(inst, max_size=ms, max_rows=max_rows)
+    for f in ("status", "min_size", "formula", "distinct", "candidates", "otf_level", "otf_stored"):
+        assert got[f] == want[f], (f, got[f], want[f])
+    if got["status"] == V.OK:
+        assert separates(got["formula"], names, lens, atoms, P)
+
+
+def test_batch_512_mix_equals_plain():
+    insts = [V.Instance.generate(4, k) for k in range(512)]
+    plain = [V.solve(i, max_size=16) for i in insts]
+    batch = V.solve_batch(insts, max_size=16)
+    bad = [k for k, (a, b) in enumerate(zip(plain, batch))
+           if any(a[f] != b[f] for f in ("status", "min_size", "formula", "distinct", "candidates",
+                                         "level_winners", "otf_level", "otf_stored"))]
+    assert not bad, [(k, plain[k]["status"], batch[k]["status"]) for k in bad[:10]]

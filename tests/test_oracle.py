@@ -225,3 +225,24 @@ def test_num_formulas_closed_form():
         nf = O.num_formulas(A, 6)
         for s in range(1, 7):
             assert nf[s][7] == _count_trees(A, s), (A, s)
+
+
+def test_onthefly_semantics():
+    rng = random.Random(9)
+    inst = _rand_inst(rng, 3, 8, 4, 10)
+    inst = O.Instance(inst.names, inst.lens + [inst.lens[0]], [row + [row[0]] for row in inst.atoms], 1)
+    full = O.enumerate_cs(inst, 7, 3)
+    assert full["status"] == 1 and full["otf_level"] == 0
+    # a cap above everything stored changes nothing
+    big = O.enumerate_cs(inst, 7, 3, 10 ** 9)
+    assert big == full
+    # cap inside level 4: levels 1-4 deduped, 5 check-only, then stop
+    cap = sum(full["distinct"][:4]) + 7
+    r = O.enumerate_cs(inst, 7, 3, cap)
+    assert r["status"] == 3 and r["otf_level"] == 4 and r["otf_stored"] == 7
+    assert r["distinct"][:5] == full["distinct"][:5]
+    assert r["levels_done"] == 4 and r["distinct"][5] == 0
+    # the check-only level enumerates pairs over the stored rows only
+    n = [0] + full["distinct"][1:4] + [7]
+    want = 4 * n[4] + 4 * sum(n[i] * n[4 - i] for i in range(1, 5) if 2 * i <= 4)
+    assert r["candidates"][5] == want

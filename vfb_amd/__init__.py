@@ -23,7 +23,7 @@ OP_NAMES = ["~", "&", "|", "X", "F", "G", "U", "atom"]
 
 class Options(C.Structure):
     _fields_ = [("max_size", C.c_int), ("relaxed_type", C.c_int), ("verbose", C.c_int),
-                ("timing", C.c_int), ("hbm_budget", C.c_uint64)]
+                ("timing", C.c_int), ("hbm_budget", C.c_uint64), ("max_rows", C.c_uint64)]
 
 
 class Result(C.Structure):
@@ -39,7 +39,8 @@ class Result(C.Structure):
                 ("ms_resolve", C.c_double), ("ms_store", C.c_double),
                 ("expand_launches", C.c_uint64), ("table_slots", C.c_uint64),
                 ("retries", C.c_uint64), ("bytes_expand", C.c_uint64),
-                ("bytes_store", C.c_uint64), ("fresh_total", C.c_uint64)]
+                ("bytes_store", C.c_uint64), ("fresh_total", C.c_uint64),
+                ("otf_level", C.c_int), ("pad", C.c_int), ("otf_stored", C.c_uint64)]
 
 
 # every symbol declared in include/vfb.h
@@ -175,7 +176,7 @@ class Instance:
             raise VfbError(f"write_json: {status_string(st)}")
 
 
-def _opts(max_size, relaxed_type, timing, budget, verbose):
+def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0):
     o = Options()
     lib().vfb_options_default(C.byref(o))
     o.max_size = max_size
@@ -183,6 +184,7 @@ def _opts(max_size, relaxed_type, timing, budget, verbose):
     o.timing = 1 if timing else 0
     o.verbose = 1 if verbose else 0
     o.hbm_budget = int(budget)
+    o.max_rows = int(max_rows)
     return o
 
 
@@ -202,12 +204,13 @@ def _todict(r):
         "expand_launches": r.expand_launches, "table_slots": r.table_slots,
         "retries": r.retries, "bytes_expand": r.bytes_expand,
         "bytes_store": r.bytes_store, "fresh_total": r.fresh_total,
+        "otf_level": r.otf_level, "otf_stored": r.otf_stored,
     }
 
 
-def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbose=False):
+def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbose=False, max_rows=0):
     """Plain single-instance solve (the drop-in for LTLI)."""
-    o = _opts(max_size, relaxed_type, timing, budget, verbose)
+    o = _opts(max_size, relaxed_type, timing, budget, verbose, max_rows)
     r = Result()
     lib().vfb_solve(inst.handle, C.byref(o), C.byref(r))
     try:
@@ -216,10 +219,10 @@ def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbo
         lib().vfb_result_free(C.byref(r))
 
 
-def solve_batch(insts, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0):
+def solve_batch(insts, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, max_rows=0):
     """Many instances stepped together; each result equals solve() alone."""
     n = len(insts)
-    o = _opts(max_size, relaxed_type, timing, budget, False)
+    o = _opts(max_size, relaxed_type, timing, budget, False, max_rows)
     hs = (C.c_void_p * n)(*[i.handle.value for i in insts])
     rs = (Result * n)()
     lib().vfb_solve_batch(hs, n, C.byref(o), rs)

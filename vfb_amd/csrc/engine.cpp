@@ -1,16 +1,21 @@
 // Host driver of the CS enumeration: the level scheduler of LTLI
 // (LTL_BS.cu:1069-1532) rebuilt around whole-level epochs.
 //
-// For every active instance a step builds size level s:
+// A step builds one size level s for every active instance:
 //   1. candidate units in the reference's order (~, & splits, | splits, X, F,
 //      G, U splits; LTL_BS.cu:1236-1488) and fused tasks over them;
-//   2. k_expand over all tasks of all instances (one launch);
-//   3. sync: on table / fresh-list overflow the set is rebuilt larger and the
-//      epoch re-run (results never depend on capacity);
-//   4. k_resolve, k_count, k_scan; sync: survivors, lowest matching ordinal;
-//   5. found -> formula rebuilt from parent ordinals; else k_store writes the
-//      level's survivors in ordinal order into a new level block.
-// All device memory of a solve comes from one arena (a single hipMalloc).
+//   2. one packed upload (descriptors, counters, tasks), then k_expand over all
+//      tasks of all instances, k_fresh_prefix, k_resolve, k_count and k_scan,
+//      all enqueued back to back; ONE sync;
+//   3. instances whose hash set or fresh list overflowed get a larger set and
+//      re-run the epoch (results never depend on capacity);
+//   4. found -> formula rebuilt from parent ordinals; else k_store writes the
+//      level's survivors, in ordinal order, into a new level block (async).
+// Overflow (the reference's onTheFly / lastRound, LTL_BS.cu:331-349, 516-522,
+// 1223-1233, 1490-1491): when the stored rows would pass max_rows or the HBM
+// budget, the level is stored up to that point in ordinal order and the next
+// level is only checked against the target, then the search stops.
+// All device memory of a solve comes from one process-wide arena.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,11 +40,13 @@ hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, hip
 hipError_t launch_rehash(const Slot* old, uint64_t old_cap, Slot* tab, uint64_t mask, uint64_t keep_below,
                          unsigned int* fail, int grid, hipStream_t st);
 hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, int grid, hipStream_t st);
-hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, uint64_t total, int grid, hipStream_t st);
-hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_blocks, int grid,
+hipError_t launch_fresh_prefix(const InstDesc* D, Seg* segs, int n, unsigned long long* total, hipStream_t st);
+hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const unsigned long long* total, int grid,
+                          hipStream_t st);
+hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_chunks, int grid,
                         hipStream_t st);
 hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStream_t st);
-hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_blocks, int grid,
+hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, int grid,
                         hipStream_t st);
 
 struct DeviceError : std::runtime_error {
@@ -59,6 +66,8 @@ static uint64_t next_pow2(uint64_t x) {
     return p;
 }
 
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
 // ---------------------------------------------------------------------------
 // device arena: one hipMalloc, first-fit free list with coalescing
 // ---------------------------------------------------------------------------
@@ -70,15 +79,13 @@ class Arena {
     }
     ~Arena() { (void)hipFree(base_); }
     void* alloc(size_t bytes) {
-        bytes = (bytes + 255) & ~size_t(255);
-        if (bytes == 0) bytes = 256;
+        bytes = std::max<size_t>(align256(bytes), 256);
         for (auto it = free_.begin(); it != free_.end(); ++it) {
             if (it->second >= bytes) {
                 size_t off = it->first, sz = it->second;
                 free_.erase(it);
                 if (sz > bytes) free_[off + bytes] = sz - bytes;
                 used_[off] = bytes;
-                in_use_ += bytes;
                 return (char*)base_ + off;
             }
         }
@@ -91,7 +98,6 @@ class Arena {
         if (it == used_.end()) return;
         size_t sz = it->second;
         used_.erase(it);
-        in_use_ -= sz;
         auto nx = free_.lower_bound(off);
         if (nx != free_.end() && off + sz == nx->first) { sz += nx->second; nx = free_.erase(nx); }
         if (nx != free_.begin()) {
@@ -100,13 +106,16 @@ class Arena {
         }
         free_[off] = sz;
     }
-    size_t in_use() const { return in_use_; }
+    size_t largest_free() const {
+        size_t m = 0;
+        for (auto& kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
     size_t size() const { return size_; }
 
   private:
     void* base_ = nullptr;
     size_t size_;
-    size_t in_use_ = 0;
     std::map<size_t, size_t> free_;
     std::map<size_t, size_t> used_;
 };
@@ -116,8 +125,10 @@ struct Buf {
     size_t bytes = 0;
 };
 
+// budget (the solve's own limit) or arena (memory shared with a batch) exhausted
 struct CapacityError : std::runtime_error {
-    explicit CapacityError(const std::string& s) : std::runtime_error(s) {}
+    bool arena;
+    CapacityError(const std::string& s, bool a) : std::runtime_error(s), arena(a) {}
 };
 
 // ---------------------------------------------------------------------------
@@ -127,12 +138,16 @@ struct Inst {
     const vfb_instance* in = nullptr;
     vfb_result* res = nullptr;
     int max_size = 0;
-    uint64_t budget = 0, used = 0;
+    uint64_t budget = 0, used = 0, max_rows = 0;
     InstDesc h{};
     bool done = false;
-    int level = 0;                   // last completed level
-    uint64_t gbase = 0;              // global ordinal of the next level's first candidate
-    std::vector<Buf> rows, pars;     // per level
+    bool deferred = false;      // batch only: re-solved alone after the batch
+    bool check_only = false;    // this level is the onTheFly last round
+    bool next_check_only = false;
+    int level = 0;              // last completed level
+    uint64_t gbase = 0;         // global ordinal of the next level's first candidate
+    uint64_t stored = 0;        // rows stored over all levels
+    std::vector<Buf> rows;      // per level: T columns of nrows words, then nrows parent ordinals
     std::vector<uint32_t> nrows;
     std::vector<std::vector<Unit>> units;
     Buf table;
@@ -140,11 +155,9 @@ struct Inst {
     Buf fresh_ord, fresh_slot, winbits, blkcount, blkprefix;
     uint64_t fresh_cap = 0, nwords_cap = 0, nblk_cap = 0;
     double win_ratio = 1.0, fresh_ratio = 1.0;
-    // current epoch
-    uint64_t N = 0;
+    uint64_t N = 0;             // candidates of the level being built
     std::vector<Task> tasks;
     Ctr ctr{};
-    bool redo = false;
 };
 
 class Engine {
@@ -156,45 +169,50 @@ class Engine {
 
   private:
     vfb_options opt_;
+    bool batch_ = false;
     hipStream_t st_ = nullptr;
-    hipEvent_t ev_[2] = {nullptr, nullptr};
     int cus_ = 256;
     std::unique_ptr<Arena> arena_;
-    InstDesc* d_desc_ = nullptr;
-    size_t desc_cap_ = 0;
-    Ctr* d_ctr_ = nullptr;
-    size_t ctr_cap_ = 0;
-    void* d_aux_ = nullptr;
-    size_t aux_cap_ = 0;
+    char* d_step_ = nullptr;      // per-phase packed descriptors / counters / tasks / segments
+    size_t step_cap_ = 0;
     unsigned int* d_flag_ = nullptr;
-    char* pin_ = nullptr;        // pinned staging for host->device uploads,
-    size_t pin_cap_ = 0;         // recycled only at stream sync points
-    size_t pin_used_ = 0;
+    char* pin_ = nullptr;         // pinned staging for uploads, recycled at sync points
+    size_t pin_cap_ = 0, pin_used_ = 0;
+    std::vector<hipEvent_t> ev_pool_;
+    struct Timed {
+        hipEvent_t a, b;
+        int kind;                 // 0 expand, 1 resolve, 2 store
+        std::vector<Inst*> who;
+    };
+    std::vector<Timed> timed_;
 
     void sync();
     void h2d(void* dst, const void* src, size_t bytes);
-
+    char* step_buf(size_t bytes);
     void dalloc(Inst& I, Buf& b, size_t bytes);
     void dfree(Inst& I, Buf& b);
     void ensure(Inst& I, Buf& b, size_t bytes);
-    void* aux(size_t bytes);
-    void setup(Inst& I, uint32_t idx);
-    void prepare_level(Inst& I);
+    void release_all(Inst& I);
+    void setup(Inst& I);
+    void build_units(Inst& I);
+    void prepare_scratch(Inst& I);
     void ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild);
-    void upload_descs(std::vector<Inst*>& v);
     void finish(Inst& I, vfb_status st);
+    void defer(Inst& I);
+    bool on_capacity(Inst& I, const CapacityError& e);
     std::string node_string(Inst& I, int level, uint64_t row);
     std::string cand_string(Inst& I, int level, uint64_t o, int* top_op);
     void fill(void* p, uint64_t words, uint64_t v);
-    float timed_begin();
-    void timed_end(double* acc);
+    void time_begin(hipEvent_t* a);
+    void time_end(hipEvent_t a, int kind, const std::vector<Inst*>& who);
+    void collect_times();
+    void epoch(std::vector<Inst*>& v);
+    void store(std::vector<Inst*>& v);
 };
 
 Engine::Engine(size_t arena_bytes) {
     vfb_options_default(&opt_);
     CK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
-    CK(hipEventCreate(&ev_[0]));
-    CK(hipEventCreate(&ev_[1]));
     int dev = 0;
     CK(hipGetDevice(&dev));
     CK(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
@@ -203,22 +221,92 @@ Engine::Engine(size_t arena_bytes) {
 }
 
 Engine::~Engine() {
-    if (d_desc_) (void)hipFree(d_desc_);
-    if (d_ctr_) (void)hipFree(d_ctr_);
-    if (d_aux_) (void)hipFree(d_aux_);
+    (void)hipStreamSynchronize(st_);
+    if (d_step_) (void)hipFree(d_step_);
     if (d_flag_) (void)hipFree(d_flag_);
     if (pin_) (void)hipHostFree(pin_);
+    for (auto e : ev_pool_) (void)hipEventDestroy(e);
+    for (auto& t : timed_) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     arena_.reset();
-    if (ev_[0]) (void)hipEventDestroy(ev_[0]);
-    if (ev_[1]) (void)hipEventDestroy(ev_[1]);
     if (st_) (void)hipStreamDestroy(st_);
 }
 
+void Engine::sync() {
+    CK(hipStreamSynchronize(st_));
+    pin_used_ = 0;
+    collect_times();
+}
+
+void Engine::h2d(void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    const size_t need = align256(bytes);
+    if (pin_used_ + need > pin_cap_) {
+        sync();
+        if (need > pin_cap_) {
+            if (pin_) CK(hipHostFree(pin_));
+            pin_ = nullptr;
+            pin_cap_ = std::max(need, std::max<size_t>(pin_cap_ * 2, 1 << 22));
+            CK(hipHostMalloc((void**)&pin_, pin_cap_, hipHostMallocDefault));
+        }
+    }
+    std::memcpy(pin_ + pin_used_, src, bytes);
+    CK(hipMemcpyAsync(dst, pin_ + pin_used_, bytes, hipMemcpyHostToDevice, st_));
+    pin_used_ += need;
+}
+
+// Device scratch for one phase's packed upload.  Phases run in stream order,
+// so a later phase may overwrite it; growing it waits for the stream first.
+char* Engine::step_buf(size_t bytes) {
+    if (bytes > step_cap_) {
+        sync();
+        if (d_step_) CK(hipFree(d_step_));
+        step_cap_ = std::max(bytes, step_cap_ * 2);
+        CK(hipMalloc((void**)&d_step_, step_cap_));
+    }
+    return d_step_;
+}
+
+void Engine::time_begin(hipEvent_t* a) {
+    if (!opt_.timing) return;
+    if (ev_pool_.empty()) {
+        hipEvent_t e;
+        CK(hipEventCreate(&e));
+        ev_pool_.push_back(e);
+    }
+    *a = ev_pool_.back();
+    ev_pool_.pop_back();
+    CK(hipEventRecord(*a, st_));
+}
+
+void Engine::time_end(hipEvent_t a, int kind, const std::vector<Inst*>& who) {
+    if (!opt_.timing) return;
+    hipEvent_t b;
+    if (ev_pool_.empty()) CK(hipEventCreate(&b));
+    else { b = ev_pool_.back(); ev_pool_.pop_back(); }
+    CK(hipEventRecord(b, st_));
+    timed_.push_back(Timed{a, b, kind, who});
+}
+
+// reads finished event pairs (called after a stream sync, so no extra waits)
+void Engine::collect_times() {
+    for (auto& t : timed_) {
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, t.a, t.b));
+        for (Inst* I : t.who) {
+            double* acc = t.kind == 0 ? &I->res->ms_expand : t.kind == 1 ? &I->res->ms_resolve : &I->res->ms_store;
+            *acc += ms;
+        }
+        ev_pool_.push_back(t.a);
+        ev_pool_.push_back(t.b);
+    }
+    timed_.clear();
+}
+
 void Engine::dalloc(Inst& I, Buf& b, size_t bytes) {
-    size_t rb = (bytes + 255) & ~size_t(255);
-    if (I.used + rb > I.budget) throw CapacityError("per-solve HBM budget exhausted");
+    size_t rb = std::max<size_t>(align256(bytes), 256);
+    if (I.used + rb > I.budget) throw CapacityError("per-solve HBM budget exhausted", false);
     void* p = arena_->alloc(rb);
-    if (!p) throw CapacityError("device arena exhausted");
+    if (!p) throw CapacityError("device arena exhausted", batch_);
     b.p = p;
     b.bytes = rb;
     I.used += rb;
@@ -238,55 +326,21 @@ void Engine::ensure(Inst& I, Buf& b, size_t bytes) {
     dalloc(I, b, bytes);
 }
 
-void Engine::sync() {
-    CK(hipStreamSynchronize(st_));
-    pin_used_ = 0;
-}
-
-void Engine::h2d(void* dst, const void* src, size_t bytes) {
-    if (!bytes) return;
-    const size_t need = (bytes + 255) & ~size_t(255);
-    if (pin_used_ + need > pin_cap_) {
-        sync();
-        if (need > pin_cap_) {
-            if (pin_) CK(hipHostFree(pin_));
-            pin_ = nullptr;
-            pin_cap_ = std::max(need, std::max<size_t>(pin_cap_ * 2, 1 << 22));
-            CK(hipHostMalloc((void**)&pin_, pin_cap_, hipHostMallocDefault));
-        }
-    }
-    std::memcpy(pin_ + pin_used_, src, bytes);
-    CK(hipMemcpyAsync(dst, pin_ + pin_used_, bytes, hipMemcpyHostToDevice, st_));
-    pin_used_ += need;
-}
-
-void* Engine::aux(size_t bytes) {
-    if (bytes > aux_cap_) {
-        sync();
-        if (d_aux_) CK(hipFree(d_aux_));
-        aux_cap_ = std::max(bytes, aux_cap_ * 2);
-        CK(hipMalloc(&d_aux_, aux_cap_));
-    }
-    return d_aux_;
+void Engine::release_all(Inst& I) {
+    for (auto& b : I.rows) dfree(I, b);
+    dfree(I, I.table);
+    dfree(I, I.fresh_ord);
+    dfree(I, I.fresh_slot);
+    dfree(I, I.winbits);
+    dfree(I, I.blkcount);
+    dfree(I, I.blkprefix);
+    I.fresh_cap = I.nwords_cap = I.nblk_cap = 0;
+    I.cap = 0;
 }
 
 void Engine::fill(void* p, uint64_t words, uint64_t v) {
     int grid = (int)std::min<uint64_t>((words + 255) / 256, (uint64_t)cus_ * 8);
     CK(launch_fill((unsigned long long*)p, words, v, std::max(grid, 1), st_));
-}
-
-float Engine::timed_begin() {
-    if (opt_.timing) CK(hipEventRecord(ev_[0], st_));
-    return 0.f;
-}
-
-void Engine::timed_end(double* acc) {
-    if (!opt_.timing) return;
-    CK(hipEventRecord(ev_[1], st_));
-    CK(hipEventSynchronize(ev_[1]));
-    float ms = 0.f;
-    CK(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
-    *acc += ms;
 }
 
 // key parameters per trace (LTL_BS.cu:30-110), computed once per instance
@@ -302,7 +356,7 @@ static void key_params(const vfb_instance& in, int relaxed_type, InstDesc& h) {
     }
     h.half = T / 2;
     h.stride = 1;
-    if (lenSum <= 126) {
+    if (lenSum <= 126) {                         // exact concatenation, :94-106
         h.key_kind = KEY_FIELD;
         int j = 0;
         for (int t = 0; t < T; ++t) {
@@ -311,7 +365,7 @@ static void key_params(const vfb_instance& in, int relaxed_type, InstDesc& h) {
             h.kmask[t] = len_mask(in.lens[t]);
             j += in.lens[t];
         }
-    } else if (relaxed_type == 1) {
+    } else if (relaxed_type == 1) {              // strided samples, :45-59
         h.key_kind = KEY_STRIDE;
         h.stride = lenSum / 126;
         int j = 0;
@@ -322,7 +376,7 @@ static void key_params(const vfb_instance& in, int relaxed_type, InstDesc& h) {
             h.kbits[t] = (int16_t)use;
             j += n;
         }
-    } else if (relaxed_type == 2) {
+    } else if (relaxed_type == 2) {              // leading bits per trace, :61-76
         h.key_kind = KEY_FIELD;
         int j = 0;
         for (int t = 0; t < T; ++t) {
@@ -333,12 +387,12 @@ static void key_params(const vfb_instance& in, int relaxed_type, InstDesc& h) {
             h.kmask[t] = len_mask(use);
             j += mb;
         }
-    } else {
+    } else {                                     // hashed, :78-88
         h.key_kind = KEY_MIX;
     }
 }
 
-void Engine::setup(Inst& I, uint32_t idx) {
+void Engine::setup(Inst& I) {
     const vfb_instance& in = *I.in;
     const int A = in.A(), T = in.T();
     std::memset(&I.h, 0, sizeof(I.h));
@@ -346,7 +400,6 @@ void Engine::setup(Inst& I, uint32_t idx) {
     I.h.P = in.P;
     key_params(in, opt_.relaxed_type, I.h);
     I.rows.assign(kMaxS + 2, Buf{});
-    I.pars.assign(kMaxS + 2, Buf{});
     I.nrows.assign(kMaxS + 2, 0);
     I.units.assign(kMaxS + 2, {});
     vfb_result& R = *I.res;
@@ -373,18 +426,20 @@ void Engine::setup(Inst& I, uint32_t idx) {
     }
     R.level_winners[1] = (uint64_t)A;
     R.level_candidates[1] = (uint64_t)A;
+    R.levels_done = 1;
     std::vector<uint64_t> col((size_t)A * T);
     for (int a = 0; a < A; ++a)
         for (int t = 0; t < T; ++t) col[(size_t)t * A + a] = in.atoms[(size_t)a * T + t] & len_mask(in.lens[t]);
     dalloc(I, I.rows[1], col.size() * 8);
     h2d(I.rows[1].p, col.data(), col.size() * 8);
     I.nrows[1] = (uint32_t)A;
+    I.stored = (uint64_t)A;
     I.level = 1;
     I.gbase = (uint64_t)A;
-    (void)idx;
 }
 
-void Engine::prepare_level(Inst& I) {
+// candidate units of the next level in canonical order, and fused tasks
+void Engine::build_units(Inst& I) {
     const int s = I.level + 1;
     auto& U = I.units[s];
     U.clear();
@@ -411,7 +466,6 @@ void Engine::prepare_level(Inst& I) {
     for (int i = 1; 2 * i <= s - 1; ++i) add(VFB_OP_UNTIL, i, s - 1 - i);
     I.N = base;
 
-    // fused tasks
     I.tasks.clear();
     const int nsplit = (s - 1) / 2;
     if (I.nrows[s - 1]) {
@@ -421,10 +475,10 @@ void Engine::prepare_level(Inst& I) {
         t.nl = I.nrows[s - 1];
         t.nr = 1;
         t.nchunk = 1;
-        t.base[0] = U[0].base;
-        t.base[1] = U[1 + 2 * nsplit].base;
-        t.base[2] = U[2 + 2 * nsplit].base;
-        t.base[3] = U[3 + 2 * nsplit].base;
+        t.base[0] = U[0].base;                   // ~
+        t.base[1] = U[1 + 2 * nsplit].base;      // X
+        t.base[2] = U[2 + 2 * nsplit].base;      // F
+        t.base[3] = U[3 + 2 * nsplit].base;      // G
         I.tasks.push_back(t);
     }
     for (int i = 1; i <= nsplit; ++i) {
@@ -439,13 +493,13 @@ void Engine::prepare_level(Inst& I) {
         t.nchunk = (t.nr + 63) / 64;
         t.base[0] = U[i].base;                   // & split i
         t.base[1] = U[nsplit + i].base;          // | split i
-        t.base[2] = U[4 + 2 * nsplit + i - 1].base;  // U split i
+        t.base[2] = U[3 + 2 * nsplit + i].base;  // U split i
         I.tasks.push_back(t);
     }
 
-    // descriptor fields for this epoch
     InstDesc& h = I.h;
     h.level = s;
+    h.check_only = I.check_only ? 1 : 0;
     h.nunits = (int)U.size();
     for (size_t k = 0; k < U.size(); ++k) h.units[k] = U[k];
     for (int l = 1; l < s; ++l) {
@@ -455,12 +509,17 @@ void Engine::prepare_level(Inst& I) {
     h.epoch_base = I.gbase;
     h.ord_lo = 0;
     h.ord_hi = I.N;
+}
 
-    // scratch sized from the previous level's ratios, grown on overflow
+// scratch and hash set for a deduplicating level, sized from the previous
+// level's ratios and grown on overflow
+void Engine::prepare_scratch(Inst& I) {
+    InstDesc& h = I.h;
     const uint64_t N = I.N;
     const uint64_t nwords = (N + 63) / 64;
     const uint64_t nblk = (nwords + kChunkWords - 1) / kChunkWords;
-    uint64_t fcap = std::min<uint64_t>(N, std::max<uint64_t>(1 << 16, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
+    uint64_t fcap = std::min<uint64_t>(
+        N, std::max<uint64_t>(1 << 16, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
     if (fcap > I.fresh_cap) {
         ensure(I, I.fresh_ord, fcap * 8);
         ensure(I, I.fresh_slot, fcap * 4);
@@ -483,16 +542,16 @@ void Engine::prepare_level(Inst& I) {
     h.blkcount = (uint32_t*)I.blkcount.p;
     h.blkprefix = (unsigned long long*)I.blkprefix.p;
     h.nblk = nblk;
-    fill(I.winbits.p, nwords, 0);
-
-    const uint64_t est_new = std::min<uint64_t>(N, std::max<uint64_t>(1 << 16, (uint64_t)(N * std::min(1.0, I.win_ratio * 2.0))));
+    const uint64_t est_new =
+        std::min<uint64_t>(N, std::max<uint64_t>(1 << 16, (uint64_t)(N * std::min(1.0, I.win_ratio * 1.3))));
     ensure_table(I, I.committed + est_new, false);
+    fill(I.winbits.p, nwords, 0);
 }
 
-// (re)builds the hash set so that it holds want_keys at load <= 1/2; keeps
-// only keys of completed epochs (ord < epoch_base)
+// (re)builds the hash set for want_keys keys at load <= 0.7; keeps only keys
+// of completed epochs (ord < epoch_base)
 void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
-    uint64_t need = next_pow2(std::max<uint64_t>(4096, 2 * want_keys));
+    uint64_t need = next_pow2(std::max<uint64_t>(4096, (uint64_t)(want_keys / 0.7) + 1));
     if (need > (1ull << 32)) need = 1ull << 32;
     if (I.table.p && need <= I.cap && !force_rebuild) {
         I.h.claim_limit = I.cap * 7 / 10 > I.committed ? I.cap * 7 / 10 - I.committed : 0;
@@ -503,14 +562,14 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     dalloc(I, nt, need * sizeof(Slot));
     fill(nt.p, need * 4, kEmpty);
     if (I.table.p) {
-        CK(hipMemsetAsync(d_flag_, 0, sizeof(unsigned int), st_));
+        unsigned int zero = 0;
+        h2d(d_flag_, &zero, sizeof zero);
         int grid = (int)std::min<uint64_t>((I.cap + 255) / 256, (uint64_t)cus_ * 8);
-        CK(launch_rehash((const Slot*)I.table.p, I.cap, (Slot*)nt.p, need - 1, I.h.epoch_base ? I.h.epoch_base : I.gbase,
-                         d_flag_, grid, st_));
+        CK(launch_rehash((const Slot*)I.table.p, I.cap, (Slot*)nt.p, need - 1, I.gbase, d_flag_, grid, st_));
         unsigned int fail = 0;
         CK(hipMemcpyAsync(&fail, d_flag_, sizeof fail, hipMemcpyDeviceToHost, st_));
         sync();
-        if (fail) throw CapacityError("hash set rebuild failed");
+        if (fail) throw DeviceError("hash set rebuild exceeded the probe limit");
         dfree(I, I.table);
     }
     I.table = nt;
@@ -521,32 +580,11 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     I.res->table_slots = need;
 }
 
-void Engine::upload_descs(std::vector<Inst*>& v) {
-    const size_t n = v.size();
-    if (n > desc_cap_) {
-        sync();
-        if (d_desc_) CK(hipFree(d_desc_));
-        CK(hipMalloc(&d_desc_, n * sizeof(InstDesc)));
-        desc_cap_ = n;
-    }
-    if (n > ctr_cap_) {
-        sync();
-        if (d_ctr_) CK(hipFree(d_ctr_));
-        CK(hipMalloc(&d_ctr_, n * sizeof(Ctr)));
-        ctr_cap_ = n;
-    }
-    std::vector<InstDesc> hd(n);
-    for (size_t k = 0; k < n; ++k) {
-        v[k]->h.ctr = d_ctr_ + k;
-        hd[k] = v[k]->h;
-    }
-    h2d(d_desc_, hd.data(), n * sizeof(InstDesc));
-}
-
 std::string Engine::node_string(Inst& I, int level, uint64_t row) {
     if (level == 1) return I.in->names[row];
     uint64_t o = 0;
-    CK(hipMemcpy(&o, (const uint64_t*)I.pars[level].p + row, 8, hipMemcpyDeviceToHost));
+    const uint64_t* par = (const uint64_t*)I.rows[level].p + (uint64_t)I.h.T * I.nrows[level];
+    CK(hipMemcpy(&o, par + row, 8, hipMemcpyDeviceToHost));
     return cand_string(I, level, o, nullptr);
 }
 
@@ -573,45 +611,153 @@ void Engine::finish(Inst& I, vfb_status st) {
     I.done = true;
     I.res->status = st;
     if (!I.res->formula) I.res->formula = strdup("Not found !");
-    for (auto* v : {&I.rows, &I.pars})
-        for (auto& b : *v) dfree(I, b);
-    dfree(I, I.table);
-    dfree(I, I.fresh_ord);
-    dfree(I, I.fresh_slot);
-    dfree(I, I.winbits);
-    dfree(I, I.blkcount);
-    dfree(I, I.blkprefix);
+    release_all(I);
+}
+
+void Engine::defer(Inst& I) {
+    release_all(I);
+    I.done = true;
+    I.deferred = true;
+}
+
+// true when the instance leaves the batch (arena memory is shared there)
+bool Engine::on_capacity(Inst& I, const CapacityError& e) {
+    if (e.arena) { defer(I); return true; }
+    return false;
+}
+
+void Engine::epoch(std::vector<Inst*>& v) {
+    const size_t n = v.size();
+    // packed upload: descriptors | counters | tasks | fresh segments | total | chunk segments | ids
+    std::vector<Task> tasks;
+    uint64_t waves = 0;
+    for (size_t k = 0; k < n; ++k)
+        for (Task t : v[k]->tasks) {
+            t.inst = (uint32_t)k;
+            t.wave_begin = waves;
+            waves += t.kind == 0 ? (t.nl + 63) / 64 : (uint64_t)t.nl * t.nchunk;
+            tasks.push_back(t);
+        }
+    std::vector<uint32_t> ids;
+    std::vector<Seg> bsegs;
+    uint64_t btot = 0;
+    for (size_t k = 0; k < n; ++k) {
+        if (v[k]->check_only) continue;
+        ids.push_back((uint32_t)k);
+        if (v[k]->h.nblk) { bsegs.push_back(Seg{(uint32_t)k, 0, btot}); btot += v[k]->h.nblk; }
+    }
+    const size_t o_ctr = align256(n * sizeof(InstDesc));
+    const size_t o_task = o_ctr + align256(n * sizeof(Ctr));
+    const size_t o_fseg = o_task + align256(std::max<size_t>(tasks.size(), 1) * sizeof(Task));
+    const size_t o_tot = o_fseg + align256(n * sizeof(Seg));
+    const size_t o_bseg = o_tot + 256;
+    const size_t o_ids = o_bseg + align256(std::max<size_t>(bsegs.size(), 1) * sizeof(Seg));
+    const size_t bytes = o_ids + align256(std::max<size_t>(ids.size(), 1) * 4);
+    char* d = step_buf(bytes);
+    std::vector<char> host(bytes, 0);
+    InstDesc* hd = (InstDesc*)host.data();
+    Ctr* hc = (Ctr*)(host.data() + o_ctr);
+    for (size_t k = 0; k < n; ++k) {
+        v[k]->h.ctr = (Ctr*)(d + o_ctr) + k;
+        hd[k] = v[k]->h;
+        hc[k] = Ctr{};
+        hc[k].found = kEmpty;
+    }
+    if (!tasks.empty()) std::memcpy(host.data() + o_task, tasks.data(), tasks.size() * sizeof(Task));
+    if (!bsegs.empty()) std::memcpy(host.data() + o_bseg, bsegs.data(), bsegs.size() * sizeof(Seg));
+    if (!ids.empty()) std::memcpy(host.data() + o_ids, ids.data(), ids.size() * 4);
+    h2d(d, host.data(), bytes);
+    const InstDesc* D = (const InstDesc*)d;
+
+    hipEvent_t ev = nullptr;
+    if (waves) {
+        time_begin(&ev);
+        int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * 8);
+        CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, grid, st_));
+        time_end(ev, 0, v);
+        for (auto* I : v) I->res->expand_launches++;
+    }
+    if (!ids.empty()) {
+        std::vector<Inst*> dd;
+        for (auto* I : v)
+            if (!I->check_only) dd.push_back(I);
+        time_begin(&ev);
+        unsigned long long* tot = (unsigned long long*)(d + o_tot);
+        CK(launch_fresh_prefix(D, (Seg*)(d + o_fseg), (int)n, tot, st_));
+        CK(launch_resolve(D, (const Seg*)(d + o_fseg), (int)n, tot, cus_ * 16, st_));
+        if (btot) {
+            int grid = (int)std::min<uint64_t>((btot + 3) / 4, (uint64_t)cus_ * 8);
+            CK(launch_count(D, (const Seg*)(d + o_bseg), (int)bsegs.size(), btot, grid, st_));
+        }
+        CK(launch_scan(D, (const uint32_t*)(d + o_ids), (int)ids.size(), st_));
+        time_end(ev, 1, dd);
+    }
+    std::vector<Ctr> back(n);
+    CK(hipMemcpyAsync(back.data(), d + o_ctr, n * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
+    sync();
+    for (size_t k = 0; k < n; ++k) v[k]->ctr = back[k];
+}
+
+void Engine::store(std::vector<Inst*>& v) {
+    const size_t n = v.size();
+    std::vector<Seg> segs;
+    uint64_t tot = 0;
+    for (size_t k = 0; k < n; ++k) {
+        Inst& I = *v[k];
+        if (!I.h.out_n || !I.h.nblk) continue;
+        segs.push_back(Seg{(uint32_t)k, 0, tot});
+        tot += I.h.nblk * (uint64_t)((I.h.T + kStoreTraces - 1) / kStoreTraces);
+    }
+    if (!tot) return;
+    const size_t o_seg = align256(n * sizeof(InstDesc));
+    const size_t bytes = o_seg + align256(segs.size() * sizeof(Seg));
+    char* d = step_buf(bytes);
+    std::vector<char> host(bytes, 0);
+    for (size_t k = 0; k < n; ++k) ((InstDesc*)host.data())[k] = v[k]->h;
+    std::memcpy(host.data() + o_seg, segs.data(), segs.size() * sizeof(Seg));
+    h2d(d, host.data(), bytes);
+    hipEvent_t ev = nullptr;
+    time_begin(&ev);
+    int grid = (int)std::min<uint64_t>((tot + 3) / 4, (uint64_t)cus_ * 8);
+    CK(launch_store((const InstDesc*)d, (const Seg*)(d + o_seg), (int)segs.size(), tot, grid, st_));
+    time_end(ev, 2, v);
 }
 
 void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
     opt_ = opt;
-    // setup: atoms, level-1 check, initial tables
+    batch_ = all.size() > 1;
     std::vector<Inst*> act;
     for (auto* I : all) {
         try {
-            setup(*I, 0);
+            setup(*I);
             if (I->done) continue;
             ensure_table(*I, (uint64_t)I->in->A() * 4, false);
             act.push_back(I);
-        } catch (CapacityError&) {
-            finish(*I, VFB_CAPACITY);
+        } catch (CapacityError& e) {
+            if (!on_capacity(*I, e)) finish(*I, VFB_CAPACITY);
         }
     }
+    // atoms into the hash sets (hashSetsInitialisation, LTL_BS.cu:1217)
     if (!act.empty()) {
-        for (auto* I : act) { I->h.epoch_base = 0; I->h.level = 1; }
-        upload_descs(act);
-        CK(hipMemsetAsync(d_ctr_, 0, act.size() * sizeof(Ctr), st_));
-        for (size_t k = 0; k < act.size(); ++k) {
-            act[k]->h.lvl[1] = (const unsigned long long*)act[k]->rows[1].p;
-            act[k]->h.nrows[1] = act[k]->nrows[1];
+        const size_t n = act.size();
+        const size_t o_ctr = align256(n * sizeof(InstDesc));
+        const size_t bytes = o_ctr + align256(n * sizeof(Ctr));
+        char* d = step_buf(bytes);
+        std::vector<char> host(bytes, 0);
+        for (size_t k = 0; k < n; ++k) {
+            Inst& I = *act[k];
+            I.h.lvl[1] = (const unsigned long long*)I.rows[1].p;
+            I.h.nrows[1] = I.nrows[1];
+            I.h.ctr = (Ctr*)(d + o_ctr) + k;
+            ((InstDesc*)host.data())[k] = I.h;
         }
-        upload_descs(act);
-        for (size_t k = 0; k < act.size(); ++k)
-            CK(launch_insert_atoms(d_desc_, (uint32_t)k, act[k]->nrows[1], st_));
-        std::vector<Ctr> hc(act.size());
-        CK(hipMemcpyAsync(hc.data(), d_ctr_, hc.size() * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
+        h2d(d, host.data(), bytes);
+        for (size_t k = 0; k < n; ++k)
+            CK(launch_insert_atoms((const InstDesc*)d, (uint32_t)k, act[k]->nrows[1], st_));
+        std::vector<Ctr> hc(n);
+        CK(hipMemcpyAsync(hc.data(), d + o_ctr, n * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
         sync();
-        for (size_t k = 0; k < act.size(); ++k) {
+        for (size_t k = 0; k < n; ++k) {
             if (hc[k].overflow) throw DeviceError("atom insertion overflow");
             act[k]->committed = hc[k].claims;
         }
@@ -622,57 +768,36 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
         std::vector<Inst*> step;
         for (auto* I : act) {
             if (I->done) continue;
-            try {
-                prepare_level(*I);
-                step.push_back(I);
-            } catch (CapacityError&) {
-                finish(*I, VFB_CAPACITY);
+            I->check_only = I->next_check_only;
+            build_units(*I);
+            if (!I->check_only) {
+                try {
+                    prepare_scratch(*I);
+                } catch (CapacityError& e) {
+                    if (on_capacity(*I, e)) continue;
+                    // no memory to deduplicate this level: check it only, then stop
+                    I->check_only = true;
+                    build_units(*I);
+                }
             }
+            step.push_back(I);
         }
         if (step.empty()) break;
 
-        // ---- 2-3. expand with retry on overflow ----
+        // ---- 2-3. epoch; re-run for instances whose set or list overflowed ----
         std::vector<Inst*> pend = step;
         for (int attempt = 0; !pend.empty(); ++attempt) {
-            if (attempt > 12) throw DeviceError("expand did not converge");
-            upload_descs(pend);
-            std::vector<Ctr> init(pend.size());
-            for (auto& c : init) { c = Ctr{}; c.found = kEmpty; }
-            h2d(d_ctr_, init.data(), init.size() * sizeof(Ctr));
-            std::vector<Task> tasks;
-            uint64_t waves = 0;
-            for (size_t k = 0; k < pend.size(); ++k)
-                for (Task t : pend[k]->tasks) {
-                    t.inst = (uint32_t)k;
-                    t.wave_begin = waves;
-                    waves += t.kind == 0 ? (t.nl + 63) / 64 : (uint64_t)t.nl * t.nchunk;
-                    tasks.push_back(t);
-                }
-            if (waves) {
-                Task* d_tasks = (Task*)aux(tasks.size() * sizeof(Task));
-                h2d(d_tasks, tasks.data(), tasks.size() * sizeof(Task));
-                int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * 8);
-                timed_begin();
-                CK(launch_expand(d_desc_, d_tasks, (int)tasks.size(), waves, grid, st_));
-                for (auto* I : pend) I->res->expand_launches++;
-                if (opt_.timing) {
-                    double ms = 0;
-                    timed_end(&ms);
-                    for (auto* I : pend) I->res->ms_expand += ms;
-                }
-            }
-            std::vector<Ctr> hc(pend.size());
-            CK(hipMemcpyAsync(hc.data(), d_ctr_, hc.size() * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
-            sync();
+            if (attempt > 12) throw DeviceError("epoch did not converge");
+            epoch(pend);
             std::vector<Inst*> again;
-            for (size_t k = 0; k < pend.size(); ++k) {
-                Inst& I = *pend[k];
-                I.ctr = hc[k];
-                if (!hc[k].overflow) continue;
+            for (auto* Ip : pend) {
+                Inst& I = *Ip;
+                if (!I.ctr.overflow) continue;
+                I.res->retries++;
                 try {
-                    I.res->retries++;
-                    if (hc[k].overflow & 2) {
-                        uint64_t fcap = std::min<uint64_t>(I.N, std::max<uint64_t>(I.fresh_cap * 4, hc[k].fresh_count + 4096));
+                    if (I.ctr.overflow & 2) {
+                        uint64_t fcap = std::min<uint64_t>(
+                            I.N, std::max<uint64_t>(I.fresh_cap * 4, I.ctr.fresh_count + 4096));
                         ensure(I, I.fresh_ord, fcap * 8);
                         ensure(I, I.fresh_slot, fcap * 4);
                         I.fresh_cap = fcap;
@@ -680,83 +805,54 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
                         I.h.fresh_slot = (uint32_t*)I.fresh_slot.p;
                         I.h.fresh_cap = fcap;
                     }
-                    uint64_t want = (hc[k].overflow & 5) ? (I.cap / 2) * 4 : I.cap / 2;
-                    ensure_table(I, want, true);
+                    uint64_t keys = (uint64_t)(I.cap * 0.7);
+                    ensure_table(I, (I.ctr.overflow & 5) ? keys * 4 : keys, true);
+                    fill(I.winbits.p, I.h.nwords, 0);
                     again.push_back(&I);
-                } catch (CapacityError&) {
-                    finish(I, VFB_CAPACITY);
+                } catch (CapacityError& e) {
+                    if (on_capacity(I, e)) continue;
+                    I.check_only = true;   // cannot grow: check the level only
+                    build_units(I);
+                    again.push_back(&I);
                 }
             }
             pend.swap(again);
         }
-        std::vector<Inst*> live;
-        for (auto* I : step)
-            if (!I->done) live.push_back(I);
-        if (live.empty()) break;
-        upload_descs(live);
-        {
-            // restore the counters of the successful expand (upload reset nothing,
-            // but instances re-run individually had their own slot index)
-            std::vector<Ctr> hc(live.size());
-            for (size_t k = 0; k < live.size(); ++k) hc[k] = live[k]->ctr;
-            h2d(d_ctr_, hc.data(), hc.size() * sizeof(Ctr));
-        }
 
-        // ---- 4. resolve, count, scan ----
-        std::vector<Seg> fsegs, bsegs;
-        uint64_t ftot = 0, btot = 0;
-        std::vector<uint32_t> ids;
-        for (size_t k = 0; k < live.size(); ++k) {
-            Inst& I = *live[k];
-            if (I.ctr.fresh_count) { fsegs.push_back(Seg{(uint32_t)k, 0, ftot}); ftot += I.ctr.fresh_count; }
-            if (I.h.nblk) { bsegs.push_back(Seg{(uint32_t)k, 0, btot}); btot += I.h.nblk; }
-            ids.push_back((uint32_t)k);
-        }
-        {
-            size_t off1 = fsegs.size() * sizeof(Seg);
-            size_t off2 = off1 + bsegs.size() * sizeof(Seg);
-            char* a = (char*)aux(off2 + ids.size() * 4 + 64);
-            if (!fsegs.empty()) h2d(a, fsegs.data(), off1);
-            if (!bsegs.empty()) h2d(a + off1, bsegs.data(), off2 - off1);
-            h2d(a + off2, ids.data(), ids.size() * 4);
-            timed_begin();
-            if (ftot) {
-                int grid = (int)std::min<uint64_t>((ftot + 255) / 256, (uint64_t)cus_ * 16);
-                CK(launch_resolve(d_desc_, (const Seg*)a, (int)fsegs.size(), ftot, grid, st_));
-            }
-            if (btot) {
-                int grid = (int)std::min<uint64_t>((btot + 3) / 4, (uint64_t)cus_ * 8);
-                CK(launch_count(d_desc_, (const Seg*)(a + off1), (int)bsegs.size(), btot, grid, st_));
-            }
-            CK(launch_scan(d_desc_, (const uint32_t*)(a + off2), (int)ids.size(), st_));
-            if (opt_.timing) {
-                double ms = 0;
-                timed_end(&ms);
-                for (auto* I : live) I->res->ms_resolve += ms;
-            }
-            std::vector<Ctr> hc(live.size());
-            CK(hipMemcpyAsync(hc.data(), d_ctr_, hc.size() * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
-            sync();
-            for (size_t k = 0; k < live.size(); ++k) live[k]->ctr = hc[k];
-        }
-
-        // ---- 5. found / store ----
-        std::vector<Inst*> store;
-        for (auto* Ip : live) {
+        // ---- 4. results, overflow decisions, stores ----
+        std::vector<Inst*> st;
+        for (auto* Ip : step) {
             Inst& I = *Ip;
+            if (I.done) continue;
             vfb_result& R = *I.res;
             const int s = I.level + 1;
             R.level_candidates[s] = I.N;
+            if (opt_.verbose)
+                std::fprintf(stderr, "Cost %-2d | %s | candidates: %-14llu | fresh: %-12llu | distinct: %-12llu | table: %llu\n",
+                             s, I.check_only ? "check" : "dedup", (unsigned long long)I.N,
+                             (unsigned long long)I.ctr.fresh_count, (unsigned long long)I.ctr.winners,
+                             (unsigned long long)I.cap);
+            if (I.check_only) {
+                // onTheFly last round (processOnTheFly, LTL_BS.cu:331-349)
+                if (I.ctr.found != kEmpty) {
+                    R.candidates[s] = I.ctr.found + 1;
+                    R.min_size = s;
+                    R.found_ordinal = I.ctr.found;
+                    std::string f = cand_string(I, s, I.ctr.found, &R.top_op);
+                    R.formula = strdup(f.c_str());
+                    finish(I, VFB_OK);
+                } else {
+                    R.candidates[s] = I.N;
+                    finish(I, VFB_CAPACITY);
+                }
+                continue;
+            }
             R.level_winners[s] = I.ctr.winners;
             R.fresh_total += I.ctr.fresh_count;
             for (const Unit& u : I.units[s]) {
                 const uint64_t a = (u.op == VFB_OP_AND || u.op == VFB_OP_OR) ? 2 : 1;
                 R.bytes_expand += u.count * (a * 8 * (uint64_t)I.h.T + 16);
             }
-            if (opt_.verbose)
-                std::fprintf(stderr, "Cost %-2d | candidates: %-14llu | fresh: %-12llu | distinct: %-12llu | table: %llu\n", s,
-                             (unsigned long long)I.N, (unsigned long long)I.ctr.fresh_count,
-                             (unsigned long long)I.ctr.winners, (unsigned long long)I.cap);
             if (I.ctr.found != kEmpty) {
                 R.candidates[s] = I.ctr.found + 1;
                 R.distinct[s] = I.ctr.found_rank;
@@ -770,55 +866,50 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             R.candidates[s] = I.N;
             R.distinct[s] = I.ctr.winners;
             R.levels_done = s;
-            R.bytes_store += I.ctr.winners * (8 * (uint64_t)I.h.T + 24);
             if (s >= I.max_size) {
                 finish(I, VFB_NOT_FOUND);
                 continue;
             }
-            try {
-                const uint64_t n = I.ctr.winners;
-                if (n > 0xffffffffull) throw CapacityError("level too large");
-                dalloc(I, I.rows[s], std::max<uint64_t>(n, 1) * I.h.T * 8);
-                dalloc(I, I.pars[s], std::max<uint64_t>(n, 1) * 8);
-                I.nrows[s] = (uint32_t)n;
+            // rows to keep: all survivors, or as many as max_rows / the budget allow
+            const uint64_t W = I.ctr.winners;
+            const uint64_t row_bytes = 8 * (uint64_t)I.h.T + 8;
+            uint64_t keep = W;
+            if (I.max_rows) keep = std::min(keep, I.max_rows > I.stored ? I.max_rows - I.stored : 0);
+            uint64_t avail = I.budget > I.used ? I.budget - I.used : 0;
+            if (!batch_) avail = std::min<uint64_t>(avail, arena_->largest_free());
+            keep = std::min<uint64_t>(keep, avail > 256 ? (avail - 256) / row_bytes : 0);
+            keep = std::min<uint64_t>(keep, 0xffffffffull);
+            if (keep < W) {
+                I.next_check_only = true;
+                R.otf_level = s;
+                R.otf_stored = keep;
+            }
+            R.bytes_store += keep * (8 * (uint64_t)I.h.T + 24);
+            if (keep == 0) {
+                I.nrows[s] = 0;
+                I.h.out_n = 0;
+            } else {
+                try {
+                    dalloc(I, I.rows[s], keep * row_bytes);
+                } catch (CapacityError& e) {
+                    if (on_capacity(I, e)) continue;
+                    throw DeviceError("row allocation failed after a fit check");
+                }
+                I.nrows[s] = (uint32_t)keep;
                 I.h.out_rows = (unsigned long long*)I.rows[s].p;
-                I.h.out_par = (unsigned long long*)I.pars[s].p;
-                I.h.out_n = n;
-                store.push_back(&I);
-            } catch (CapacityError&) {
-                finish(I, VFB_CAPACITY);
+                I.h.out_par = (unsigned long long*)I.rows[s].p + (uint64_t)I.h.T * keep;
+                I.h.out_n = keep;
+                st.push_back(&I);
             }
+            const uint64_t N = std::max<uint64_t>(I.N, 1);
+            I.win_ratio = (double)W / N;
+            I.fresh_ratio = (double)I.ctr.fresh_count / N;
+            I.committed += W;
+            I.stored += keep;
+            I.gbase += I.N;
+            I.level += 1;
         }
-        if (!store.empty()) {
-            upload_descs(store);
-            std::vector<Seg> segs;
-            uint64_t tot = 0;
-            for (size_t k = 0; k < store.size(); ++k)
-                if (store[k]->nrows[store[k]->level + 1] && store[k]->h.nblk) {
-                    segs.push_back(Seg{(uint32_t)k, 0, tot});
-                    tot += store[k]->h.nblk * (uint64_t)((store[k]->h.T + kStoreTraces - 1) / kStoreTraces);
-                }
-            if (tot) {
-                Seg* d = (Seg*)aux(segs.size() * sizeof(Seg));
-                h2d(d, segs.data(), segs.size() * sizeof(Seg));
-                int grid = (int)std::min<uint64_t>((tot + 3) / 4, (uint64_t)cus_ * 8);
-                timed_begin();
-                CK(launch_store(d_desc_, d, (int)segs.size(), tot, grid, st_));
-                if (opt_.timing) {
-                    double ms = 0;
-                    timed_end(&ms);
-                    for (auto* I : store) I->res->ms_store += ms;
-                }
-            }
-            for (auto* I : store) {
-                const uint64_t N = std::max<uint64_t>(I->N, 1);
-                I->win_ratio = (double)I->ctr.winners / N;
-                I->fresh_ratio = (double)I->ctr.fresh_count / N;
-                I->committed += I->ctr.winners;
-                I->gbase += I->N;
-                I->level += 1;
-            }
-        }
+        if (!st.empty()) store(st);
         std::vector<Inst*> nxt;
         for (auto* I : act)
             if (!I->done) nxt.push_back(I);
@@ -829,6 +920,12 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
 
 static std::mutex g_mu;
 static std::unique_ptr<Engine> g_engine;
+
+static void reset_result(vfb_result* r) {
+    std::free(r->formula);
+    std::memset(r, 0, sizeof(vfb_result));
+    r->status = VFB_NOT_FOUND;
+}
 
 static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const vfb_options* opt_in, vfb_result* out) {
     vfb_options opt;
@@ -844,20 +941,15 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
     std::vector<Inst*> run;
     for (size_t k = 0; k < n; ++k) {
         std::string why;
-        if (!insts[k] || validate(*insts[k], &why) != VFB_OK) {
-            out[k].status = VFB_INVALID_INPUT;
-            out[k].formula = strdup("see_the_error");
-            continue;
-        }
-        int ms = opt.max_size;
-        if (ms < 1 || ms > VFB_MAX_SIZE) {
+        if (!insts[k] || validate(*insts[k], &why) != VFB_OK || opt.max_size < 1 || opt.max_size > VFB_MAX_SIZE) {
             out[k].status = VFB_INVALID_INPUT;
             out[k].formula = strdup("see_the_error");
             continue;
         }
         st[k].in = insts[k];
         st[k].res = &out[k];
-        st[k].max_size = ms;
+        st[k].max_size = opt.max_size;
+        st[k].max_rows = opt.max_rows;
         run.push_back(&st[k]);
     }
     std::lock_guard<std::mutex> lk(g_mu);
@@ -873,18 +965,30 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
             size_t bytes = want ? std::min(want, (size_t)(fr * 0.95)) : (size_t)(fr * 0.85);
             g_engine = std::make_unique<Engine>(bytes);
         }
-        uint64_t per = opt.hbm_budget ? opt.hbm_budget : g_engine->arena_size();
+        uint64_t per = opt.hbm_budget ? std::min<uint64_t>(opt.hbm_budget, g_engine->arena_size())
+                                      : g_engine->arena_size();
         for (auto* I : run) I->budget = per;
         g_engine->run(run, opt);
-    } catch (CapacityError& e) {
-        for (auto* I : run)
-            if (!I->done) { I->res->status = VFB_CAPACITY; if (!I->res->formula) I->res->formula = strdup("Not found !"); }
+        // batch members that could not get arena memory are solved alone
+        for (auto* I : run) {
+            if (!I->deferred) continue;
+            Inst fresh;
+            fresh.in = I->in;
+            fresh.res = I->res;
+            fresh.max_size = I->max_size;
+            fresh.max_rows = I->max_rows;
+            fresh.budget = per;
+            reset_result(fresh.res);
+            std::vector<Inst*> one{&fresh};
+            g_engine->run(one, opt);
+        }
     } catch (std::exception& e) {
         std::fprintf(stderr, "vfb: device error: %s\n", e.what());
         for (auto* I : run) {
-            if (I->res->status != VFB_OK || !I->done) {
+            if (!I->done || I->res->status != VFB_OK) {
+                std::free(I->res->formula);
                 I->res->status = VFB_DEVICE_ERROR;
-                if (!I->res->formula) I->res->formula = strdup("see_the_error");
+                I->res->formula = strdup("see_the_error");
             }
         }
         g_engine.reset();

@@ -60,7 +60,8 @@ struct InstDesc {
     int T, P, key_kind, half;
     int stride;       // KEY_STRIDE sampling stride
     int level;        // level being built
-    int nunits, pad0;
+    int nunits;
+    int check_only;   // onTheFly level: target check only, no keys, no storage
     uint64_t lenmask[kMaxT];
     uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
     int16_t koff[kMaxT];        // first key bit of the trace's field / samples
@@ -82,7 +83,7 @@ struct InstDesc {
     uint64_t nblk;
     unsigned long long* out_rows;
     unsigned long long* out_par;
-    uint64_t out_n;
+    uint64_t out_n;             // rows stored of this level (survivors past it are dropped)
     Ctr* ctr;
     Unit units[kMaxUnits];
 };

@@ -209,7 +209,18 @@ __device__ __forceinline__ uint32_t epoch_enable(const InstDesc& d, const uint64
     return en;
 }
 
-template <int KEY>
+// onTheFly last round (processOnTheFly, LTL_BS.cu:331-349): no key, no
+// storage; the lowest matching ordinal wins
+__device__ __forceinline__ void check_candidates(const InstDesc& d, const uint64_t* ord, uint32_t mismatch,
+                                                 uint32_t enable) {
+    uint64_t best = kEmpty;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (((enable >> q) & 1u) && !((mismatch >> q) & 1u) && ord[q] < best) best = ord[q];
+    if (__ballot(best != kEmpty) && best != kEmpty) atomicMin(&d.ctr->found, (unsigned long long)best);
+}
+
+template <int KEY, bool CHECK>
 __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
     const uint64_t x = wl * 64 + (uint64_t)lane;
     const bool active = x < tk.nl;
@@ -228,18 +239,21 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
         const uint64_t want = t < d.P ? 1ull : 0ull;
         mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
               ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
-        key_acc<KEY>(k[0], c0, d, t);
-        key_acc<KEY>(k[1], c1, d, t);
-        key_acc<KEY>(k[2], c2, d, t);
-        key_acc<KEY>(k[3], c3, d, t);
+        if constexpr (!CHECK) {
+            key_acc<KEY>(k[0], c0, d, t);
+            key_acc<KEY>(k[1], c1, d, t);
+            key_acc<KEY>(k[2], c2, d, t);
+            key_acc<KEY>(k[3], c3, d, t);
+        }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) key_fin(k[q]);
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
-    finish_candidates(d, k, ord, mm, epoch_enable(d, ord, active), lane);
+    if constexpr (CHECK) check_candidates(d, ord, mm, epoch_enable(d, ord, active));
+    else finish_candidates(d, k, ord, mm, epoch_enable(d, ord, active), lane);
 }
 
-template <int KEY>
+template <int KEY, bool CHECK>
 __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
     const uint64_t l = wl / tk.nchunk;
     const uint64_t rc = wl - l * tk.nchunk;
@@ -261,17 +275,20 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
         const uint64_t want = t < d.P ? 1ull : 0ull;
         mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
               ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
-        key_acc<KEY>(k[0], c0, d, t);
-        key_acc<KEY>(k[1], c1, d, t);
-        key_acc<KEY>(k[2], c2, d, t);
-        key_acc<KEY>(k[3], c3, d, t);
+        if constexpr (!CHECK) {
+            key_acc<KEY>(k[0], c0, d, t);
+            key_acc<KEY>(k[1], c1, d, t);
+            key_acc<KEY>(k[2], c2, d, t);
+            key_acc<KEY>(k[3], c3, d, t);
+        }
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) key_fin(k[q]);
     const uint64_t pair = l * tk.nr + r;
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
-    finish_candidates(d, k, ord, mm, epoch_enable(d, ord, active), lane);
+    if constexpr (CHECK) check_candidates(d, ord, mm, epoch_enable(d, ord, active));
+    else finish_candidates(d, k, ord, mm, epoch_enable(d, ord, active), lane);
 }
 
 __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, const Task* __restrict__ tasks,
@@ -289,14 +306,17 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
         const Task& tk = tasks[lo];
         const InstDesc& d = D[tk.inst];
         const uint64_t wl = w - tk.wave_begin;
-        if (tk.kind == 0) {
-            if (d.key_kind == KEY_MIX) expand_unary<KEY_MIX>(d, tk, wl, lane);
-            else if (d.key_kind == KEY_FIELD) expand_unary<KEY_FIELD>(d, tk, wl, lane);
-            else expand_unary<KEY_STRIDE>(d, tk, wl, lane);
+        if (d.check_only) {
+            if (tk.kind == 0) expand_unary<KEY_MIX, true>(d, tk, wl, lane);
+            else expand_binary<KEY_MIX, true>(d, tk, wl, lane);
+        } else if (tk.kind == 0) {
+            if (d.key_kind == KEY_MIX) expand_unary<KEY_MIX, false>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_unary<KEY_FIELD, false>(d, tk, wl, lane);
+            else expand_unary<KEY_STRIDE, false>(d, tk, wl, lane);
         } else {
-            if (d.key_kind == KEY_MIX) expand_binary<KEY_MIX>(d, tk, wl, lane);
-            else if (d.key_kind == KEY_FIELD) expand_binary<KEY_FIELD>(d, tk, wl, lane);
-            else expand_binary<KEY_STRIDE>(d, tk, wl, lane);
+            if (d.key_kind == KEY_MIX) expand_binary<KEY_MIX, false>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_binary<KEY_FIELD, false>(d, tk, wl, lane);
+            else expand_binary<KEY_STRIDE, false>(d, tk, wl, lane);
         }
     }
 }
@@ -352,8 +372,26 @@ __device__ __forceinline__ int seg_find(const Seg* segs, int nseg, uint64_t g) {
     return lo;
 }
 
+// per-instance offsets of the fresh lists in one flat index space, and its
+// size, computed on the device so that the resolve launch needs no host sync
+__global__ void k_fresh_prefix(const InstDesc* __restrict__ D, Seg* segs, int n, unsigned long long* total) {
+    if (threadIdx.x != 0) return;
+    unsigned long long acc = 0;
+    for (int k = 0; k < n; ++k) {
+        const InstDesc& d = D[k];
+        segs[k].inst = (uint32_t)k;
+        segs[k].begin = acc;
+        if (!d.check_only) {
+            const unsigned long long c = d.ctr->fresh_count;
+            acc += c < d.fresh_cap ? c : d.fresh_cap;
+        }
+    }
+    *total = acc;
+}
+
 __global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
-                                                 int nseg, uint64_t total) {
+                                                 int nseg, const unsigned long long* __restrict__ total_p) {
+    const uint64_t total = *total_p;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
         const Seg& sg = segs[seg_find(segs, nseg, g)];
@@ -510,7 +548,8 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         const int t0 = (int)(item - c * nsl) * kStoreTraces;
         const int t1 = t0 + kStoreTraces < d.T ? t0 + kStoreTraces : d.T;
         const uint32_t n = d.blkcount[c];
-        if (n == 0) continue;
+        const uint64_t rank0 = d.blkprefix[c];
+        if (n == 0 || rank0 >= d.out_n) continue;
         const uint64_t w = c * kChunkWords + lane;
         uint64_t bits = w < d.nwords ? d.winbits[w] : 0ull;
         const uint64_t pc = __popcll(bits);
@@ -522,9 +561,9 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        const uint64_t rank0 = d.blkprefix[c];
         const uint64_t obase = c * (uint64_t)(kChunkWords * 64);
-        for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t lim = rank0 + n <= d.out_n ? n : (uint32_t)(d.out_n - rank0);
+        for (uint32_t i = lane; i < lim; i += 64) {
             const uint64_t o = obase + list[i];
             const Cand cd = decode(d, o);
             const uint64_t row = rank0 + i;
@@ -562,14 +601,20 @@ hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, 
     return hipGetLastError();
 }
 
-hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, uint64_t total, int grid, hipStream_t st) {
+hipError_t launch_fresh_prefix(const InstDesc* D, Seg* segs, int n, unsigned long long* total, hipStream_t st) {
+    hipLaunchKernelGGL(k_fresh_prefix, dim3(1), dim3(64), 0, st, D, segs, n, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const unsigned long long* total, int grid,
+                          hipStream_t st) {
     hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, st, D, segs, nseg, total);
     return hipGetLastError();
 }
 
-hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_blocks, int grid,
+hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_chunks, int grid,
                         hipStream_t st) {
-    hipLaunchKernelGGL(k_count, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_blocks);
+    hipLaunchKernelGGL(k_count, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_chunks);
     return hipGetLastError();
 }
 
@@ -578,9 +623,9 @@ hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_blocks, int grid,
+hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, int grid,
                         hipStream_t st) {
-    hipLaunchKernelGGL(k_store, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_blocks);
+    hipLaunchKernelGGL(k_store, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_items);
     return hipGetLastError();
 }
 
