@@ -57,6 +57,81 @@ def cpu_baseline(insts, max_size, budget_s):
                       f"({distinct} distinct CS in {t_total:.1f} s)"}
 
 
+def batched_mix(V, n, max_size):
+    """Config 4: n uneven instances, batched vs sequential plain solves."""
+    insts = [V.Instance.generate(4, k) for k in range(n)]
+    V.solve(insts[0], max_size=max_size)
+    t0 = time.perf_counter()
+    plain = [V.solve(i, max_size=max_size) for i in insts]
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch = V.solve_batch(insts, max_size=max_size)
+    t_batch = time.perf_counter() - t0
+    fields = ("status", "min_size", "formula", "distinct", "candidates", "level_winners", "otf_level", "otf_stored")
+    exact = all(a[f] == b[f] for a, b in zip(plain, batch) for f in fields)
+    lat = [r["ms_total"] for r in plain]
+    return {"instances": n, "max_formula_size": max_size,
+            "instances_per_s_plain": n / t_plain, "instances_per_s_batched": n / t_batch,
+            "batched_vs_plain": t_plain / t_batch, "bit_exact": exact,
+            "plain_latency_ms_p50": pct(lat, 0.5), "plain_latency_ms_p99": pct(lat, 0.99),
+            "distinct_cs": sum(sum(r["level_winners"]) for r in plain)}
+
+
+def served(V, rate, seconds, max_size, max_active, check):
+    """Config 5: open-loop Poisson arrivals of config-4-distributed requests."""
+    import random
+    import threading
+    rng = random.Random(5)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.expovariate(rate)
+        if t > seconds:
+            break
+        arrivals.append(t)
+    insts = [V.Instance.generate(5, k) for k in range(len(arrivals))]
+    results = [None] * len(insts)
+    with V.Server(max_size=max_size, max_active=max_active) as srv:
+        tickets = [None] * len(insts)
+        submitted = threading.Semaphore(0)
+
+        def collect():
+            for k in range(len(insts)):
+                submitted.acquire()
+                results[k] = srv.wait(tickets[k])
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+
+        def monitor():
+            while th.is_alive():
+                time.sleep(5)
+                done = sum(r is not None for r in results)
+                print(f"[bench] served {time.perf_counter() - t0:.0f}s: {done}/{len(insts)} done, "
+                      f"{srv.stats()}", file=sys.stderr, flush=True)
+        threading.Thread(target=monitor, daemon=True).start()
+        for k, a in enumerate(arrivals):
+            d = t0 + a - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[k] = srv.submit(insts[k])
+            submitted.release()
+        th.join()
+        elapsed = time.perf_counter() - t0
+        stats = srv.stats()
+    lat = [r["ms_total"] for r in results]
+    sample = list(range(0, len(insts), max(1, len(insts) // check)))[:check]
+    plain_t0 = time.perf_counter()
+    plain = {k: V.solve(insts[k], max_size=max_size) for k in sample}
+    plain_rate = len(sample) / (time.perf_counter() - plain_t0)
+    fields = ("status", "min_size", "formula", "distinct", "candidates", "level_winners")
+    exact = all(plain[k][f] == results[k][f] for k in sample for f in fields)
+    return {"rate_req_s": rate, "seconds": seconds, "requests": len(insts), "max_active": max_active,
+            "instances_per_s": len(insts) / elapsed, "latency_ms_p50": pct(lat, 0.5),
+            "latency_ms_p99": pct(lat, 0.99), "peak_active": stats["peak_active"],
+            "plain_sequential_instances_per_s": plain_rate, "bit_exact_vs_plain_sample": exact,
+            "checked": len(sample)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,10 +142,14 @@ def main():
     ap.add_argument("--cpu-max-size", type=int, default=14)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mix", type=int, default=512, help="config-4 instances for batched vs plain (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="config-5 served stream length (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-active", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(instances shard across GPUs with no exchange; see DESIGN.md)"}))
+                          "(instances shard across GPUs with no exchange)"}))
         sys.exit(2)
 
     import vfb_amd as V
@@ -151,7 +230,14 @@ def main():
                      "alg_bytes_per_launch": bytes_exp / launches if launches else None},
         "device": dev,
     }
+    if args.mix:
+        print("[bench] batched mix", file=sys.stderr, flush=True)
+        out["batched_mix"] = batched_mix(V, args.mix, 16)
+    if args.serve_seconds:
+        print("[bench] served stream", file=sys.stderr, flush=True)
+        out["served"] = served(V, args.serve_rate, args.serve_seconds, 16, args.serve_active, 256)
     if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(insts, args.cpu_max_size, args.cpu_seconds)
     print(json.dumps(out))
 

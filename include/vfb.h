@@ -134,6 +134,38 @@ vfb_status vfb_solve_batch(const vfb_instance* const* insts, size_t n,
 void vfb_result_free(vfb_result* res);
 const char* vfb_status_string(vfb_status s);
 
+/* ---- serving: continuous batching -------------------------------------- */
+
+/* A request loop on a background thread.  Every iteration builds one size
+   level for every active request in shared launches (requests at different
+   levels interleave), then admits queued requests while fewer than
+   max_active run and the device memory held by the running ones is below
+   hbm_budget (0 = the process arena).  A request's result is identical to
+   vfb_solve() of the same instance with the same options.  No reference
+   counterpart (the reference solves one instance per process). */
+typedef struct vfb_server vfb_server;
+
+vfb_status vfb_server_create(const vfb_options* opt, uint64_t hbm_budget, int max_active, vfb_server** out);
+
+/* Queues a copy of inst; the ticket identifies the request. */
+vfb_status vfb_server_submit(vfb_server* srv, const vfb_instance* inst, uint64_t* ticket);
+
+/* Non-blocking: when the request has finished, *done = 1, *out receives its
+   result (ms_total = completion minus submission) and the ticket is released;
+   otherwise *done = 0.  VFB_INVALID_INPUT for an unknown ticket. */
+vfb_status vfb_server_poll(vfb_server* srv, uint64_t ticket, vfb_result* out, int* done);
+
+/* Blocks until the request has finished, then as vfb_server_poll. */
+vfb_status vfb_server_wait(vfb_server* srv, uint64_t ticket, vfb_result* out);
+
+/* Requests waiting, running, and device bytes held by the running ones;
+   peak_active is the largest number of requests that ran together. */
+void vfb_server_stats(vfb_server* srv, uint64_t* queued, uint64_t* active, uint64_t* bytes,
+                      uint64_t* peak_active);
+
+/* Stops the loop (requests still queued or running are dropped). */
+void vfb_server_destroy(vfb_server* srv);
+
 /* Device name and compute-unit count, for reports.  Returns VFB_DEVICE_ERROR
    when no HIP device is usable. */
 vfb_status vfb_device_info(char* name, size_t name_cap, int* num_cus);

@@ -123,3 +123,25 @@ def test_batch_512_mix_equals_plain():
            if any(a[f] != b[f] for f in ("status", "min_size", "formula", "distinct", "candidates",
                                          "level_winners", "otf_level", "otf_stored"))]
     assert not bad, [(k, plain[k]["status"], batch[k]["status"]) for k in bad[:10]]
+
+
+def test_server_results_equal_plain():
+    """Requests arriving while others run are admitted as slots free up; each
+    result equals the plain solve of the same instance."""
+    import time
+    insts = [V.Instance.generate(5, k) for k in range(96)]
+    plain = [V.solve(i, max_size=16) for i in insts]
+    with V.Server(max_size=16, max_active=16) as srv:
+        tickets = []
+        for k, inst in enumerate(insts):
+            tickets.append(srv.submit(inst))
+            if k % 8 == 7:
+                time.sleep(0.002)        # arrivals spread over time
+        got = [srv.wait(t) for t in tickets]
+        stats = srv.stats()
+    assert stats["peak_active"] <= 16
+    for k, (a, b) in enumerate(zip(plain, got)):
+        for f in ("status", "min_size", "formula", "distinct", "candidates", "level_winners",
+                  "otf_level", "otf_stored"):
+            assert a[f] == b[f], (k, f, a[f], b[f])
+        assert b["ms_total"] > 0

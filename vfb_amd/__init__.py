@@ -50,6 +50,8 @@ EXPORTS = [
     "vfb_instance_atom_name", "vfb_instance_planted", "vfb_instance_max_size",
     "vfb_instance_free", "vfb_options_default", "vfb_solve", "vfb_solve_batch",
     "vfb_result_free", "vfb_status_string", "vfb_device_info",
+    "vfb_server_create", "vfb_server_submit", "vfb_server_poll", "vfb_server_wait",
+    "vfb_server_stats", "vfb_server_destroy",
 ]
 
 _lib = None
@@ -89,6 +91,14 @@ def lib():
     L.vfb_status_string.argtypes = [C.c_int]
     L.vfb_status_string.restype = C.c_char_p
     L.vfb_device_info.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_int)]
+    L.vfb_server_create.argtypes = [C.POINTER(Options), C.c_uint64, C.c_int, C.POINTER(vp)]
+    L.vfb_server_submit.argtypes = [vp, vp, C.POINTER(C.c_uint64)]
+    L.vfb_server_poll.argtypes = [vp, C.c_uint64, C.POINTER(Result), C.POINTER(C.c_int)]
+    L.vfb_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(Result)]
+    L.vfb_server_stats.argtypes = [vp] + [C.POINTER(C.c_uint64)] * 4
+    L.vfb_server_stats.restype = None
+    L.vfb_server_destroy.argtypes = [vp]
+    L.vfb_server_destroy.restype = None
     _lib = L
     return L
 
@@ -240,3 +250,59 @@ def device_info():
     if st != OK:
         return None
     return {"name": buf.value.decode(), "cus": cus.value}
+
+
+class Server:
+    """Continuous-batching request loop (vfb_server_*): requests are admitted
+    as slots and HBM budget free up, and running requests share launches."""
+
+    def __init__(self, max_size=MAX_SIZE, relaxed_type=3, budget=0, max_active=256, request_budget=0,
+                 max_rows=0):
+        o = _opts(max_size, relaxed_type, False, request_budget, False, max_rows)
+        h = C.c_void_p()
+        st = lib().vfb_server_create(C.byref(o), int(budget), int(max_active), C.byref(h))
+        if st != OK:
+            raise VfbError(f"server_create: {status_string(st)}")
+        self._h = h
+
+    def submit(self, inst):
+        t = C.c_uint64()
+        st = lib().vfb_server_submit(self._h, inst.handle, C.byref(t))
+        if st != OK:
+            raise VfbError(f"server_submit: {status_string(st)}")
+        return t.value
+
+    def poll(self, ticket):
+        r = Result()
+        done = C.c_int()
+        lib().vfb_server_poll(self._h, ticket, C.byref(r), C.byref(done))
+        if not done.value:
+            return None
+        try:
+            return _todict(r)
+        finally:
+            lib().vfb_result_free(C.byref(r))
+
+    def wait(self, ticket):
+        r = Result()
+        lib().vfb_server_wait(self._h, ticket, C.byref(r))
+        try:
+            return _todict(r)
+        finally:
+            lib().vfb_result_free(C.byref(r))
+
+    def stats(self):
+        v = [C.c_uint64() for _ in range(4)]
+        lib().vfb_server_stats(self._h, *[C.byref(x) for x in v])
+        return {"queued": v[0].value, "active": v[1].value, "bytes": v[2].value, "peak_active": v[3].value}
+
+    def close(self):
+        if self._h:
+            lib().vfb_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -16,199 +16,19 @@
 // budget, the level is stored up to that point in ordinal order and the next
 // level is only checked against the target, then the search stops.
 // All device memory of a solve comes from one process-wide arena.
-#include <hip/hip_runtime.h>
+#include "engine.h"
 
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
-#include <map>
-#include <memory>
-#include <mutex>
-#include <stdexcept>
-#include <string>
-#include <vector>
-
-#include "engine_dev.h"
-#include "vfb_internal.h"
 
 namespace vfb {
 
-hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int grid,
-                         hipStream_t st);
-hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, hipStream_t st);
-hipError_t launch_rehash(const Slot* old, uint64_t old_cap, Slot* tab, uint64_t mask, uint64_t keep_below,
-                         unsigned int* fail, int grid, hipStream_t st);
-hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, int grid, hipStream_t st);
-hipError_t launch_fresh_prefix(const InstDesc* D, Seg* segs, int n, unsigned long long* total, hipStream_t st);
-hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const unsigned long long* total, int grid,
-                          hipStream_t st);
-hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_chunks, int grid,
-                        hipStream_t st);
-hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStream_t st);
-hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, int grid,
-                        hipStream_t st);
-
-struct DeviceError : std::runtime_error {
-    explicit DeviceError(const std::string& s) : std::runtime_error(s) {}
-};
-
-#define CK(x)                                                                                   \
-    do {                                                                                        \
-        hipError_t e_ = (x);                                                                    \
-        if (e_ != hipSuccess)                                                                   \
-            throw DeviceError(std::string(#x) + ": " + hipGetErrorString(e_));                  \
-    } while (0)
-
-static uint64_t next_pow2(uint64_t x) {
-    uint64_t p = 1;
-    while (p < x) p <<= 1;
-    return p;
+static void drop_live(std::vector<Inst*>& live, Inst* I) {
+    live.erase(std::remove(live.begin(), live.end(), I), live.end());
 }
-
-static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
-
-// ---------------------------------------------------------------------------
-// device arena: one hipMalloc, first-fit free list with coalescing
-// ---------------------------------------------------------------------------
-class Arena {
-  public:
-    explicit Arena(size_t bytes) : size_(bytes) {
-        CK(hipMalloc(&base_, bytes));
-        free_[0] = bytes;
-    }
-    ~Arena() { (void)hipFree(base_); }
-    void* alloc(size_t bytes) {
-        bytes = std::max<size_t>(align256(bytes), 256);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= bytes) {
-                size_t off = it->first, sz = it->second;
-                free_.erase(it);
-                if (sz > bytes) free_[off + bytes] = sz - bytes;
-                used_[off] = bytes;
-                return (char*)base_ + off;
-            }
-        }
-        return nullptr;
-    }
-    void release(void* p) {
-        if (!p) return;
-        size_t off = (char*)p - (char*)base_;
-        auto it = used_.find(off);
-        if (it == used_.end()) return;
-        size_t sz = it->second;
-        used_.erase(it);
-        auto nx = free_.lower_bound(off);
-        if (nx != free_.end() && off + sz == nx->first) { sz += nx->second; nx = free_.erase(nx); }
-        if (nx != free_.begin()) {
-            auto pv = std::prev(nx);
-            if (pv->first + pv->second == off) { pv->second += sz; return; }
-        }
-        free_[off] = sz;
-    }
-    size_t largest_free() const {
-        size_t m = 0;
-        for (auto& kv : free_) m = std::max(m, kv.second);
-        return m;
-    }
-    size_t size() const { return size_; }
-
-  private:
-    void* base_ = nullptr;
-    size_t size_;
-    std::map<size_t, size_t> free_;
-    std::map<size_t, size_t> used_;
-};
-
-struct Buf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
-// budget (the solve's own limit) or arena (memory shared with a batch) exhausted
-struct CapacityError : std::runtime_error {
-    bool arena;
-    CapacityError(const std::string& s, bool a) : std::runtime_error(s), arena(a) {}
-};
-
-// ---------------------------------------------------------------------------
-// per-instance state
-// ---------------------------------------------------------------------------
-struct Inst {
-    const vfb_instance* in = nullptr;
-    vfb_result* res = nullptr;
-    int max_size = 0;
-    uint64_t budget = 0, used = 0, max_rows = 0;
-    InstDesc h{};
-    bool done = false;
-    bool deferred = false;      // batch only: re-solved alone after the batch
-    bool check_only = false;    // this level is the onTheFly last round
-    bool next_check_only = false;
-    int level = 0;              // last completed level
-    uint64_t gbase = 0;         // global ordinal of the next level's first candidate
-    uint64_t stored = 0;        // rows stored over all levels
-    std::vector<Buf> rows;      // per level: T columns of nrows words, then nrows parent ordinals
-    std::vector<uint32_t> nrows;
-    std::vector<std::vector<Unit>> units;
-    Buf table;
-    uint64_t cap = 0, committed = 0;
-    Buf fresh_ord, fresh_slot, winbits, blkcount, blkprefix;
-    uint64_t fresh_cap = 0, nwords_cap = 0, nblk_cap = 0;
-    double win_ratio = 1.0, fresh_ratio = 1.0;
-    uint64_t N = 0;             // candidates of the level being built
-    std::vector<Task> tasks;
-    Ctr ctr{};
-};
-
-class Engine {
-  public:
-    explicit Engine(size_t arena_bytes);
-    ~Engine();
-    void run(std::vector<Inst*>& insts, const vfb_options& opt);
-    size_t arena_size() const { return arena_->size(); }
-
-  private:
-    vfb_options opt_;
-    bool batch_ = false;
-    hipStream_t st_ = nullptr;
-    int cus_ = 256;
-    std::unique_ptr<Arena> arena_;
-    char* d_step_ = nullptr;      // per-phase packed descriptors / counters / tasks / segments
-    size_t step_cap_ = 0;
-    unsigned int* d_flag_ = nullptr;
-    char* pin_ = nullptr;         // pinned staging for uploads, recycled at sync points
-    size_t pin_cap_ = 0, pin_used_ = 0;
-    std::vector<hipEvent_t> ev_pool_;
-    struct Timed {
-        hipEvent_t a, b;
-        int kind;                 // 0 expand, 1 resolve, 2 store
-        std::vector<Inst*> who;
-    };
-    std::vector<Timed> timed_;
-
-    void sync();
-    void h2d(void* dst, const void* src, size_t bytes);
-    char* step_buf(size_t bytes);
-    void dalloc(Inst& I, Buf& b, size_t bytes);
-    void dfree(Inst& I, Buf& b);
-    void ensure(Inst& I, Buf& b, size_t bytes);
-    void release_all(Inst& I);
-    void setup(Inst& I);
-    void build_units(Inst& I);
-    void prepare_scratch(Inst& I);
-    void ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild);
-    void finish(Inst& I, vfb_status st);
-    void defer(Inst& I);
-    bool on_capacity(Inst& I, const CapacityError& e);
-    std::string node_string(Inst& I, int level, uint64_t row);
-    std::string cand_string(Inst& I, int level, uint64_t o, int* top_op);
-    void fill(void* p, uint64_t words, uint64_t v);
-    void time_begin(hipEvent_t* a);
-    void time_end(hipEvent_t a, int kind, const std::vector<Inst*>& who);
-    void collect_times();
-    void epoch(std::vector<Inst*>& v);
-    void store(std::vector<Inst*>& v);
-};
 
 Engine::Engine(size_t arena_bytes) {
     vfb_options_default(&opt_);
@@ -306,7 +126,26 @@ void Engine::dalloc(Inst& I, Buf& b, size_t bytes) {
     size_t rb = std::max<size_t>(align256(bytes), 256);
     if (I.used + rb > I.budget) throw CapacityError("per-solve HBM budget exhausted", false);
     void* p = arena_->alloc(rb);
-    if (!p) throw CapacityError("device arena exhausted", batch_);
+    while (!p && batch_) {
+        // Shared arena full: evict the youngest other instance that holds
+        // memory, so the oldest always progresses as it would alone.  The
+        // evicted one restarts later with the same result.
+        Inst* victim = nullptr;
+        bool older_holds = false;
+        for (Inst* v : live_) {
+            if (v == &I || v->done || v->used == 0) continue;
+            if (v->prio < I.prio) older_holds = true;
+            else if (!victim || v->prio > victim->prio) victim = v;
+        }
+        if (!victim) {
+            if (older_holds) throw CapacityError("device arena exhausted", true);
+            break;   // alone in the arena: the plain solve's limit applies
+        }
+        sync();
+        defer(*victim);
+        p = arena_->alloc(rb);
+    }
+    if (!p) throw CapacityError("device arena exhausted", false);
     b.p = p;
     b.bytes = rb;
     I.used += rb;
@@ -608,6 +447,7 @@ std::string Engine::cand_string(Inst& I, int level, uint64_t o, int* top_op) {
 }
 
 void Engine::finish(Inst& I, vfb_status st) {
+    drop_live(live_, &I);
     I.done = true;
     I.res->status = st;
     if (!I.res->formula) I.res->formula = strdup("Not found !");
@@ -615,6 +455,7 @@ void Engine::finish(Inst& I, vfb_status st) {
 }
 
 void Engine::defer(Inst& I) {
+    drop_live(live_, &I);
     release_all(I);
     I.done = true;
     I.deferred = true;
@@ -723,14 +564,37 @@ void Engine::store(std::vector<Inst*>& v) {
     time_end(ev, 2, v);
 }
 
-void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
+void Engine::begin(const vfb_options& opt, bool batch) {
     opt_ = opt;
-    batch_ = all.size() > 1;
+    batch_ = batch;
+}
+
+void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
+    begin(opt, all.size() > 1);
+    std::vector<Inst*> act;
+    start(all, act);
+    while (!act.empty()) step(act);
+    sync();
+    for (Inst* I : all) drop_live(live_, I);
+}
+
+void Engine::abandon(Inst& I) {
+    sync();
+    drop_live(live_, &I);
+    release_all(I);
+    I.done = true;
+}
+
+void Engine::start(std::vector<Inst*>& all, std::vector<Inst*>& out) {
     std::vector<Inst*> act;
     for (auto* I : all) {
+        live_.push_back(I);
         try {
             setup(*I);
-            if (I->done) continue;
+            if (I->done) {   // an atom separates the traces (LTL_BS.cu:1160-1162)
+                drop_live(live_, I);
+                continue;
+            }
             ensure_table(*I, (uint64_t)I->in->A() * 4, false);
             act.push_back(I);
         } catch (CapacityError& e) {
@@ -738,6 +602,7 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
         }
     }
     // atoms into the hash sets (hashSetsInitialisation, LTL_BS.cu:1217)
+    act.erase(std::remove_if(act.begin(), act.end(), [](Inst* I) { return I->done; }), act.end());
     if (!act.empty()) {
         const size_t n = act.size();
         const size_t o_ctr = align256(n * sizeof(InstDesc));
@@ -760,10 +625,13 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
         for (size_t k = 0; k < n; ++k) {
             if (hc[k].overflow) throw DeviceError("atom insertion overflow");
             act[k]->committed = hc[k].claims;
+            if (!act[k]->done) out.push_back(act[k]);
         }
     }
+}
 
-    while (!act.empty()) {
+void Engine::step(std::vector<Inst*>& act) {
+    {
         // ---- 1. units, tasks, scratch ----
         std::vector<Inst*> step;
         for (auto* I : act) {
@@ -782,17 +650,23 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             }
             step.push_back(I);
         }
-        if (step.empty()) break;
+        step.erase(std::remove_if(step.begin(), step.end(), [](Inst* I) { return I->done; }), step.end());
+        if (step.empty()) {
+            act.clear();
+            return;
+        }
 
         // ---- 2-3. epoch; re-run for instances whose set or list overflowed ----
         std::vector<Inst*> pend = step;
         for (int attempt = 0; !pend.empty(); ++attempt) {
             if (attempt > 12) throw DeviceError("epoch did not converge");
+            pend.erase(std::remove_if(pend.begin(), pend.end(), [](Inst* I) { return I->done; }), pend.end());
+            if (pend.empty()) break;
             epoch(pend);
             std::vector<Inst*> again;
             for (auto* Ip : pend) {
                 Inst& I = *Ip;
-                if (!I.ctr.overflow) continue;
+                if (I.done || !I.ctr.overflow) continue;
                 I.res->retries++;
                 try {
                     if (I.ctr.overflow & 2) {
@@ -879,26 +753,31 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             if (!batch_) avail = std::min<uint64_t>(avail, arena_->largest_free());
             keep = std::min<uint64_t>(keep, avail > 256 ? (avail - 256) / row_bytes : 0);
             keep = std::min<uint64_t>(keep, 0xffffffffull);
+            bool evicted = false;
+            while (keep > 0) {
+                try {
+                    dalloc(I, I.rows[s], keep * row_bytes);
+                    break;
+                } catch (CapacityError& e) {
+                    if (on_capacity(I, e)) { evicted = true; break; }
+                    // alone in a fragmented arena: keep what the largest block holds
+                    const uint64_t lf = arena_->largest_free();
+                    const uint64_t fit = lf > 256 ? (lf - 256) / row_bytes : 0;
+                    keep = std::min<uint64_t>(fit, keep - 1);
+                }
+            }
+            if (evicted) continue;
             if (keep < W) {
                 I.next_check_only = true;
                 R.otf_level = s;
                 R.otf_stored = keep;
             }
             R.bytes_store += keep * (8 * (uint64_t)I.h.T + 24);
-            if (keep == 0) {
-                I.nrows[s] = 0;
-                I.h.out_n = 0;
-            } else {
-                try {
-                    dalloc(I, I.rows[s], keep * row_bytes);
-                } catch (CapacityError& e) {
-                    if (on_capacity(I, e)) continue;
-                    throw DeviceError("row allocation failed after a fit check");
-                }
-                I.nrows[s] = (uint32_t)keep;
+            I.nrows[s] = (uint32_t)keep;
+            I.h.out_n = keep;
+            if (keep) {
                 I.h.out_rows = (unsigned long long*)I.rows[s].p;
                 I.h.out_par = (unsigned long long*)I.rows[s].p + (uint64_t)I.h.T * keep;
-                I.h.out_n = keep;
                 st.push_back(&I);
             }
             const uint64_t N = std::max<uint64_t>(I.N, 1);
@@ -909,17 +788,32 @@ void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
             I.gbase += I.N;
             I.level += 1;
         }
+        st.erase(std::remove_if(st.begin(), st.end(), [](Inst* I) { return I->done; }), st.end());
         if (!st.empty()) store(st);
-        std::vector<Inst*> nxt;
-        for (auto* I : act)
-            if (!I->done) nxt.push_back(I);
-        act.swap(nxt);
     }
-    sync();
+    std::vector<Inst*> nxt;
+    for (auto* I : act)
+        if (!I->done) nxt.push_back(I);
+    act.swap(nxt);
 }
 
 static std::mutex g_mu;
 static std::unique_ptr<Engine> g_engine;
+
+std::mutex& engine_mutex() { return g_mu; }
+
+Engine& engine_acquire(size_t want) {
+    if (!g_engine || (want && want > g_engine->arena_size() && !engine_pinned())) {
+        g_engine.reset();
+        size_t fr = 0, tot = 0;
+        CK(hipMemGetInfo(&fr, &tot));
+        size_t bytes = want ? std::min(want, (size_t)(fr * 0.95)) : (size_t)(fr * 0.85);
+        g_engine = std::make_unique<Engine>(bytes);
+    }
+    return *g_engine;
+}
+
+void engine_drop() { g_engine.reset(); }
 
 static void reset_result(vfb_result* r) {
     std::free(r->formula);
@@ -950,25 +844,15 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
         st[k].res = &out[k];
         st[k].max_size = opt.max_size;
         st[k].max_rows = opt.max_rows;
+        st[k].prio = k;
         run.push_back(&st[k]);
     }
     std::lock_guard<std::mutex> lk(g_mu);
     try {
-        size_t fr = 0, tot = 0;
-        CK(hipMemGetInfo(&fr, &tot));
-        // one arena per process, sized once (85% of free HBM, or the budget
-        // when that is larger than what the current arena holds)
-        size_t want = opt.hbm_budget ? (size_t)opt.hbm_budget : 0;
-        if (!g_engine || (want && want > g_engine->arena_size())) {
-            g_engine.reset();
-            CK(hipMemGetInfo(&fr, &tot));
-            size_t bytes = want ? std::min(want, (size_t)(fr * 0.95)) : (size_t)(fr * 0.85);
-            g_engine = std::make_unique<Engine>(bytes);
-        }
-        uint64_t per = opt.hbm_budget ? std::min<uint64_t>(opt.hbm_budget, g_engine->arena_size())
-                                      : g_engine->arena_size();
+        Engine& eng = engine_acquire(opt.hbm_budget);
+        uint64_t per = opt.hbm_budget ? std::min<uint64_t>(opt.hbm_budget, eng.arena_size()) : eng.arena_size();
         for (auto* I : run) I->budget = per;
-        g_engine->run(run, opt);
+        eng.run(run, opt);
         // batch members that could not get arena memory are solved alone
         for (auto* I : run) {
             if (!I->deferred) continue;
@@ -980,7 +864,7 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
             fresh.budget = per;
             reset_result(fresh.res);
             std::vector<Inst*> one{&fresh};
-            g_engine->run(one, opt);
+            eng.run(one, opt);
         }
     } catch (std::exception& e) {
         std::fprintf(stderr, "vfb: device error: %s\n", e.what());

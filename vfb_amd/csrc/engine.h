@@ -1,0 +1,213 @@
+// Internal: the CS enumeration engine shared by the solve entry points and
+// the continuous-batching server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "engine_dev.h"
+#include "vfb_internal.h"
+
+namespace vfb {
+
+hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int grid,
+                         hipStream_t st);
+hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, hipStream_t st);
+hipError_t launch_rehash(const Slot* old, uint64_t old_cap, Slot* tab, uint64_t mask, uint64_t keep_below,
+                         unsigned int* fail, int grid, hipStream_t st);
+hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, int grid, hipStream_t st);
+hipError_t launch_fresh_prefix(const InstDesc* D, Seg* segs, int n, unsigned long long* total, hipStream_t st);
+hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const unsigned long long* total, int grid,
+                          hipStream_t st);
+hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_chunks, int grid,
+                        hipStream_t st);
+hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStream_t st);
+hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, int grid,
+                        hipStream_t st);
+
+struct DeviceError : std::runtime_error {
+    explicit DeviceError(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define CK(x)                                                                                   \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess)                                                                   \
+            throw DeviceError(std::string(#x) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+static uint64_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// ---------------------------------------------------------------------------
+// device arena: one hipMalloc, first-fit free list with coalescing
+// ---------------------------------------------------------------------------
+class Arena {
+  public:
+    explicit Arena(size_t bytes) : size_(bytes) {
+        CK(hipMalloc(&base_, bytes));
+        free_[0] = bytes;
+    }
+    ~Arena() { (void)hipFree(base_); }
+    void* alloc(size_t bytes) {
+        bytes = std::max<size_t>(align256(bytes), 256);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                size_t off = it->first, sz = it->second;
+                free_.erase(it);
+                if (sz > bytes) free_[off + bytes] = sz - bytes;
+                used_[off] = bytes;
+                return (char*)base_ + off;
+            }
+        }
+        return nullptr;
+    }
+    void release(void* p) {
+        if (!p) return;
+        size_t off = (char*)p - (char*)base_;
+        auto it = used_.find(off);
+        if (it == used_.end()) return;
+        size_t sz = it->second;
+        used_.erase(it);
+        auto nx = free_.lower_bound(off);
+        if (nx != free_.end() && off + sz == nx->first) { sz += nx->second; nx = free_.erase(nx); }
+        if (nx != free_.begin()) {
+            auto pv = std::prev(nx);
+            if (pv->first + pv->second == off) { pv->second += sz; return; }
+        }
+        free_[off] = sz;
+    }
+    size_t largest_free() const {
+        size_t m = 0;
+        for (auto& kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
+    size_t size() const { return size_; }
+
+  private:
+    void* base_ = nullptr;
+    size_t size_;
+    std::map<size_t, size_t> free_;
+    std::map<size_t, size_t> used_;
+};
+
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// budget (the solve's own limit) or arena (memory shared with a batch) exhausted
+struct CapacityError : std::runtime_error {
+    bool arena;
+    CapacityError(const std::string& s, bool a) : std::runtime_error(s), arena(a) {}
+};
+
+// ---------------------------------------------------------------------------
+// per-instance state
+// ---------------------------------------------------------------------------
+struct Inst {
+    const vfb_instance* in = nullptr;
+    vfb_result* res = nullptr;
+    int max_size = 0;
+    uint64_t budget = 0, used = 0, max_rows = 0;
+    InstDesc h{};
+    bool done = false;
+    bool deferred = false;      // batch/server: evicted for memory, restarted later
+    uint64_t prio = 0;          // admission order: older instances evict younger ones
+    bool check_only = false;    // this level is the onTheFly last round
+    bool next_check_only = false;
+    int level = 0;              // last completed level
+    uint64_t gbase = 0;         // global ordinal of the next level's first candidate
+    uint64_t stored = 0;        // rows stored over all levels
+    std::vector<Buf> rows;      // per level: T columns of nrows words, then nrows parent ordinals
+    std::vector<uint32_t> nrows;
+    std::vector<std::vector<Unit>> units;
+    Buf table;
+    uint64_t cap = 0, committed = 0;
+    Buf fresh_ord, fresh_slot, winbits, blkcount, blkprefix;
+    uint64_t fresh_cap = 0, nwords_cap = 0, nblk_cap = 0;
+    double win_ratio = 1.0, fresh_ratio = 1.0;
+    uint64_t N = 0;             // candidates of the level being built
+    std::vector<Task> tasks;
+    Ctr ctr{};
+};
+
+class Engine {
+  public:
+    explicit Engine(size_t arena_bytes);
+    ~Engine();
+    void run(std::vector<Inst*>& insts, const vfb_options& opt);
+    // incremental use (server): begin() sets the options; start() sets up new
+    // instances and appends the unfinished ones to act; step() builds one
+    // level for every instance in act and removes those that finished
+    void begin(const vfb_options& opt, bool batch);
+    void start(std::vector<Inst*>& fresh, std::vector<Inst*>& act);
+    void step(std::vector<Inst*>& act);
+    void drain() { sync(); }
+    void abandon(Inst& I);
+    size_t arena_size() const { return arena_->size(); }
+
+  private:
+    vfb_options opt_;
+    bool batch_ = false;
+    hipStream_t st_ = nullptr;
+    int cus_ = 256;
+    std::unique_ptr<Arena> arena_;
+    char* d_step_ = nullptr;      // per-phase packed descriptors / counters / tasks / segments
+    size_t step_cap_ = 0;
+    unsigned int* d_flag_ = nullptr;
+    char* pin_ = nullptr;         // pinned staging for uploads, recycled at sync points
+    size_t pin_cap_ = 0, pin_used_ = 0;
+    std::vector<hipEvent_t> ev_pool_;
+    struct Timed {
+        hipEvent_t a, b;
+        int kind;                 // 0 expand, 1 resolve, 2 store
+        std::vector<Inst*> who;
+    };
+    std::vector<Timed> timed_;
+    std::vector<Inst*> live_;     // started, not finished: eviction candidates
+
+    void sync();
+    void h2d(void* dst, const void* src, size_t bytes);
+    char* step_buf(size_t bytes);
+    void dalloc(Inst& I, Buf& b, size_t bytes);
+    void dfree(Inst& I, Buf& b);
+    void ensure(Inst& I, Buf& b, size_t bytes);
+    void release_all(Inst& I);
+    void setup(Inst& I);
+    void build_units(Inst& I);
+    void prepare_scratch(Inst& I);
+    void ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild);
+    void finish(Inst& I, vfb_status st);
+    void defer(Inst& I);
+    bool on_capacity(Inst& I, const CapacityError& e);
+    std::string node_string(Inst& I, int level, uint64_t row);
+    std::string cand_string(Inst& I, int level, uint64_t o, int* top_op);
+    void fill(void* p, uint64_t words, uint64_t v);
+    void time_begin(hipEvent_t* a);
+    void time_end(hipEvent_t a, int kind, const std::vector<Inst*>& who);
+    void collect_times();
+    void epoch(std::vector<Inst*>& v);
+    void store(std::vector<Inst*>& v);
+};
+
+// The process-wide engine (its arena holds all device memory of every solve)
+// and the mutex that serialises its use.  engine_acquire() creates the engine,
+// or re-creates it with a larger arena when want_bytes exceeds the current one.
+std::mutex& engine_mutex();
+Engine& engine_acquire(size_t want_bytes);
+void engine_drop();
+bool engine_pinned();   // true while a server holds requests in the engine
+
+}  // namespace vfb

@@ -565,11 +565,11 @@ GenSpec spec_for(int config, Rng& r) {
 
 extern "C" vfb_status vfb_instance_generate(int config, uint64_t index, vfb_instance** out) {
     if (!out || config < 1 || config > 5) return VFB_INVALID_INPUT;
-    if (config == 5) config = 4;  // the served stream draws from the batched mix
     *out = nullptr;
     Rng rng{((uint64_t)config << 32) | index};
     const uint64_t seed = rng.s;
-    GenSpec g = spec_for(config, rng);
+    // the served stream (5) draws from the batched mix (4) with its own seeds
+    GenSpec g = spec_for(config == 5 ? 4 : config, rng);
     auto inst = std::make_unique<vfb_instance>();
     for (int a = 0; a < g.A; ++a) inst->names.push_back("a" + std::to_string(a));
     inst->default_max_size = g.max_size;
