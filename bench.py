@@ -180,7 +180,10 @@ def main():
         all_res.append(res)
         print(f"[bench] step {s} {dt:.3f}s", file=sys.stderr, flush=True)
     T = sum(step_times)
-    distinct = sum(sum(r["level_winners"]) for res in all_res for r in res)
+    # the reference's count: CS stored per level, and at the level of the
+    # returned formula the survivors ordered before it (StoredLTLs)
+    distinct = sum(sum(r["distinct"]) for res in all_res for r in res)
+    level_distinct = sum(sum(r["level_winners"]) for res in all_res for r in res)
     cands = sum(sum(r["level_candidates"]) for res in all_res for r in res)
     lat = [r["ms_total"] for res in all_res for r in res]
     plain = all_res[-1]
@@ -217,6 +220,7 @@ def main():
                    "instances": args.instances, "max_formula_size": args.max_size,
                    "parallelism": "single GPU"},
         "candidates_per_s": cands / T,
+        "level_distinct_per_s": level_distinct / T,
         "instances_per_s_plain": len(insts) / t_plain,
         "instances_per_s_batched": len(insts) / t_batch,
         "batched_vs_plain": t_plain / t_batch,

@@ -547,7 +547,7 @@ void Engine::store(std::vector<Inst*>& v) {
         Inst& I = *v[k];
         if (!I.h.out_n || !I.h.nblk) continue;
         segs.push_back(Seg{(uint32_t)k, 0, tot});
-        tot += I.h.nblk * (uint64_t)((I.h.T + kStoreTraces - 1) / kStoreTraces);
+        tot += I.h.nblk * kStoreParts * (uint64_t)((I.h.T + kStoreTraces - 1) / kStoreTraces);
     }
     if (!tot) return;
     const size_t o_seg = align256(n * sizeof(InstDesc));

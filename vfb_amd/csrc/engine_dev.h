@@ -29,6 +29,7 @@ constexpr uint64_t kMatchBit = 1ull << 63;
 constexpr int kChunkWords = 64;        // bitmap words per scan chunk (4096 ordinals)
 constexpr int kProbeLimit = 4096;
 constexpr int kStoreTraces = 8;       // traces per k_store work item
+constexpr int kStoreParts = 4;        // survivor subsets per chunk in k_store
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 

@@ -109,37 +109,58 @@ __device__ __forceinline__ uint64_t ld_any(const unsigned long long* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
+__device__ __forceinline__ uint64_t ld_dev(const unsigned long long* p) {
+    // device-coherent read of a slot's ordinal: a candidate that sees a smaller
+    // ordinal of this epoch skips its atomicMin (hot keys stay uncontended)
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 enum { PR_DUP = 0, PR_FRESH = 1, PR_FAIL = 2 };
 
 // Insert-or-find of a 128-bit key with lowest-ordinal tracking, lock-free:
 // hi and lo are each claimed by a 64-bit CAS from EMPTY; a prober decides on
 // the values the CAS returns, so every prober of a key stops at the same slot.
 // Keys from earlier epochs carry ord < epoch_base and are never modified.
-__device__ __forceinline__ int probe_insert(Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo,
-                                            uint64_t gord, uint64_t epoch_base,
-                                            uint32_t* slot_out, int* claimed) {
-    uint64_t i = slot_hash(hi, lo) & mask;
-    for (int n = 0; n < kProbeLimit; ++n, i = (i + 1) & mask) {
+// The first slot's fields (th, tl, to) are passed in, loaded ahead by the
+// caller.  PR_FRESH only for a candidate that may still hold the key's lowest
+// ordinal: one that sees a smaller ordinal of this epoch has certainly lost.
+__device__ __forceinline__ int probe_from(Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo, uint64_t gord,
+                                          uint64_t epoch_base, uint64_t i, uint64_t th, uint64_t tl,
+                                          uint64_t to, uint32_t* slot_out, int* claimed) {
+    for (int n = 0; n < kProbeLimit; ++n) {
         Slot* s = tab + i;
-        uint64_t th = ld_any(&s->hi);
-        uint64_t tl = ld_any(&s->lo);
-        const uint64_t to = ld_any(&s->ord);
         if (th == kEmpty) {
             th = atomicCAS(&s->hi, kEmpty, hi);
             if (th == kEmpty) { th = hi; *claimed += 1; }
         }
-        if (th != hi) continue;
-        if (tl == kEmpty) {
-            tl = atomicCAS(&s->lo, kEmpty, lo);
-            if (tl == kEmpty) tl = lo;
+        if (th == hi) {
+            if (tl == kEmpty) {
+                tl = atomicCAS(&s->lo, kEmpty, lo);
+                if (tl == kEmpty) tl = lo;
+            }
+            if (tl == lo) {
+                *slot_out = (uint32_t)i;
+                if (to < gord) return PR_DUP;      // older key, or a smaller ordinal of this epoch
+                const uint64_t old = atomicMin(&s->ord, (unsigned long long)gord);
+                return old < gord ? PR_DUP : PR_FRESH;
+            }
         }
-        if (tl != lo) continue;
-        *slot_out = (uint32_t)i;
-        if (to < epoch_base) return PR_DUP;
-        const uint64_t old = atomicMin(&s->ord, (unsigned long long)gord);
-        return old < epoch_base ? PR_DUP : PR_FRESH;
+        i = (i + 1) & mask;
+        s = tab + i;
+        th = ld_any(&s->hi);
+        tl = ld_any(&s->lo);
+        to = ld_dev(&s->ord);
     }
     return PR_FAIL;
+}
+
+__device__ __forceinline__ int probe_insert(Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo,
+                                            uint64_t gord, uint64_t epoch_base,
+                                            uint32_t* slot_out, int* claimed) {
+    const uint64_t i = slot_hash(hi, lo) & mask;
+    Slot* s = tab + i;
+    return probe_from(tab, mask, hi, lo, gord, epoch_base, i, ld_any(&s->hi), ld_any(&s->lo), ld_any(&s->ord),
+                      slot_out, claimed);
 }
 
 __device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
@@ -148,54 +169,112 @@ __device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Probes the candidates of this lane (up to 4) and appends the fresh ones.
+__device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t* total) {
+    const int lane = threadIdx.x & 63;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(x, off);
+        if (lane >= off) x += u;
+    }
+    *total = __builtin_amdgcn_readlane(x, 63);
+    return x - v;
+}
+
+// Probes the candidates of this lane (up to 4) as four interleaved state
+// machines -- per slot visited, one round of hi-CAS, one of lo-CAS and one of
+// atomicMin, each round issuing the atomics of all four candidates together so
+// that their latencies overlap -- then appends the fresh ones with one atomic
+// per wavefront.  Per-candidate semantics are those of probe_from().
 __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* key, const uint64_t* ord,
                                                   uint32_t mismatch, uint32_t enable, int lane) {
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
-    int claimed = 0;
-    int fail = 0;
+    Slot* tab = d.table;
+    const uint64_t mask = d.table_mask;
+    uint64_t idx[4], th[4], tl[4], to[4];
+    uint32_t act = enable, fresh = 0, claimed = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const bool en = (enable >> q) & 1u;
-        uint32_t slot = 0xffffffffu;
-        int r = PR_DUP;
-        if (en) {
-            const uint64_t gord = d.epoch_base + ord[q];
-            r = probe_insert(d.table, d.table_mask, key[q].hi, key[q].lo, gord, d.epoch_base, &slot, &claimed);
-            if (r == PR_FAIL) fail = 1;
+        idx[q] = slot_hash(key[q].hi, key[q].lo) & mask;
+        th[q] = tl[q] = to[q] = kEmpty;
+        if ((act >> q) & 1u) {
+            th[q] = ld_any(&tab[idx[q]].hi);
+            tl[q] = ld_any(&tab[idx[q]].lo);
+            to[q] = ld_dev(&tab[idx[q]].ord);
         }
-        const bool fresh = en && r == PR_FRESH;
-        const uint64_t bal = __ballot(fresh);
-        if (bal) {
-            const int leader = __ffsll((unsigned long long)bal) - 1;
-            uint64_t base = 0;
-            if (lane == leader) base = atomicAdd(&d.ctr->fresh_count, (unsigned long long)__popcll(bal));
-            base = bcast64(base, leader);
-            if (fresh) {
-                const uint64_t idx = base + __popcll(bal & lt_mask);
-                if (idx < d.fresh_cap) {
+    }
+    for (int n = 0; act && n < kProbeLimit; ++n) {
+        uint64_t r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (((act >> q) & 1u) && th[q] == kEmpty) r[q] = atomicCAS(&tab[idx[q]].hi, kEmpty, key[q].hi);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (((act >> q) & 1u) && th[q] == kEmpty) {
+                if (r[q] == kEmpty) { th[q] = key[q].hi; ++claimed; } else th[q] = r[q];
+            }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (((act >> q) & 1u) && th[q] == key[q].hi && tl[q] == kEmpty)
+                r[q] = atomicCAS(&tab[idx[q]].lo, kEmpty, key[q].lo);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (((act >> q) & 1u) && th[q] == key[q].hi && tl[q] == kEmpty) tl[q] = r[q] == kEmpty ? key[q].lo : r[q];
+        uint32_t need_min = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (((act >> q) & 1u) && th[q] == key[q].hi && tl[q] == key[q].lo) {
+                const uint64_t gord = d.epoch_base + ord[q];
+                if (to[q] < gord) act &= ~(1u << q);      // older key, or a smaller ordinal of this epoch
+                else need_min |= 1u << q;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if ((need_min >> q) & 1u) r[q] = atomicMin(&tab[idx[q]].ord, (unsigned long long)(d.epoch_base + ord[q]));
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if ((need_min >> q) & 1u) {
+                act &= ~(1u << q);
+                if (r[q] >= d.epoch_base + ord[q]) fresh |= 1u << q;
+            }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if ((act >> q) & 1u) {                      // mismatch: next slot
+                idx[q] = (idx[q] + 1) & mask;
+                th[q] = ld_any(&tab[idx[q]].hi);
+                tl[q] = ld_any(&tab[idx[q]].lo);
+                to[q] = ld_dev(&tab[idx[q]].ord);
+            }
+    }
+    // one fresh-list reservation and one claims update per wavefront
+    uint32_t ftot = 0, ctot = 0;
+    const uint32_t fbase = wave_excl_scan_u32((uint32_t)__popc(fresh), &ftot);
+    (void)wave_excl_scan_u32(claimed, &ctot);
+    const uint64_t fail = __ballot(act != 0);
+    if (ftot | ctot | (fail ? 1u : 0u)) {
+        unsigned long long base = 0;
+        if (lane == 0) {
+            if (ftot) base = atomicAdd(&d.ctr->fresh_count, (unsigned long long)ftot);
+            if (ctot) {
+                const unsigned long long before = atomicAdd(&d.ctr->claims, (unsigned long long)ctot);
+                if (before + ctot > d.claim_limit) atomicOr(&d.ctr->overflow, 4u);
+            }
+            if (fail) atomicOr(&d.ctr->overflow, 1u);
+        }
+        base = bcast64(base, 0) + fbase;
+        uint32_t k = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if ((fresh >> q) & 1u) {
+                const uint64_t e = base + k++;
+                if (e < d.fresh_cap) {
                     const uint64_t flag = ((mismatch >> q) & 1u) ? 0ull : kMatchBit;
-                    d.fresh_ord[idx] = (d.epoch_base + ord[q]) | flag;
-                    d.fresh_slot[idx] = slot;
+                    d.fresh_ord[e] = (d.epoch_base + ord[q]) | flag;
+                    d.fresh_slot[e] = (uint32_t)idx[q];
                 } else {
                     atomicOr(&d.ctr->overflow, 2u);
                 }
             }
         }
-    }
-    // claims, aggregated per wavefront, bound the table's load factor
-    uint32_t wclaims = 0;
-#pragma unroll
-    for (int q = 1; q <= 4; ++q) wclaims += __popcll(__ballot(claimed >= q));
-    if (wclaims) {
-        const int leader = __ffsll((unsigned long long)__ballot(true)) - 1;
-        if (lane == leader) {
-            const unsigned long long before = atomicAdd(&d.ctr->claims, (unsigned long long)wclaims);
-            if (before + wclaims > d.claim_limit) atomicOr(&d.ctr->overflow, 4u);
-        }
-    }
-    if (__ballot(fail)) {
-        if (fail) atomicOr(&d.ctr->overflow, 1u);
     }
 }
 
@@ -494,8 +573,7 @@ __device__ __forceinline__ int unit_of(const InstDesc& d, uint64_t o) {
     return lo;
 }
 
-__device__ __forceinline__ Cand decode(const InstDesc& d, uint64_t o) {
-    const Unit& u = d.units[unit_of(d, o)];
+__device__ __forceinline__ Cand decode_unit(const InstDesc& d, const Unit& u, uint64_t o) {
     Cand c;
     c.op = u.op;
     c.L = d.lvl[u.lvl_l];
@@ -517,6 +595,11 @@ __device__ __forceinline__ Cand decode(const InstDesc& d, uint64_t o) {
     return c;
 }
 
+__device__ __forceinline__ Cand decode_from(const InstDesc& d, uint64_t o, int u) {
+    while (u + 1 < d.nunits && d.units[u + 1].base <= o) ++u;
+    return decode_unit(d, d.units[u], o);
+}
+
 __device__ __forceinline__ uint64_t cand_word(const Cand& c, int t, uint64_t m) {
     const uint64_t a = c.L[(uint64_t)t * c.nl + c.l];
     if (c.op == VFB_OP_NOT || c.op == VFB_OP_NEXT || c.op == VFB_OP_FINALLY || c.op == VFB_OP_GLOBALLY)
@@ -527,7 +610,8 @@ __device__ __forceinline__ uint64_t cand_word(const Cand& c, int t, uint64_t m) 
 }
 
 // stable compaction of the survivors.  Work item = (chunk of 4096 ordinals,
-// slice of kStoreTraces traces), one wavefront each: the chunk's set bits
+// one of kStoreParts interleaved survivor subsets, slice of kStoreTraces
+// traces), one wavefront each: the chunk's set bits
 // become an LDS list of 12-bit offsets, then the 64 lanes write consecutive
 // rows of the slice's columns (coalesced stores).  Slice 0 also writes the
 // parent ordinals.
@@ -544,8 +628,10 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         const InstDesc& d = D[sg.inst];
         const uint32_t nsl = (uint32_t)(d.T + kStoreTraces - 1) / kStoreTraces;
         const uint64_t item = g - sg.begin;
-        const uint64_t c = item / nsl;
-        const int t0 = (int)(item - c * nsl) * kStoreTraces;
+        const uint64_t c = item / (nsl * kStoreParts);
+        const uint32_t rem = (uint32_t)(item - c * nsl * kStoreParts);
+        const uint32_t part = rem / nsl;
+        const int t0 = (int)(rem - part * nsl) * kStoreTraces;
         const int t1 = t0 + kStoreTraces < d.T ? t0 + kStoreTraces : d.T;
         const uint32_t n = d.blkcount[c];
         const uint64_t rank0 = d.blkprefix[c];
@@ -563,9 +649,11 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         const uint64_t obase = c * (uint64_t)(kChunkWords * 64);
         const uint32_t lim = rank0 + n <= d.out_n ? n : (uint32_t)(d.out_n - rank0);
-        for (uint32_t i = lane; i < lim; i += 64) {
+        // the chunk's 4096 ordinals span few units: find the first one once
+        const int u0 = unit_of(d, obase);
+        for (uint32_t i = part * 64 + lane; i < lim; i += 64 * kStoreParts) {
             const uint64_t o = obase + list[i];
-            const Cand cd = decode(d, o);
+            const Cand cd = decode_from(d, o, u0);
             const uint64_t row = rank0 + i;
             for (int t = t0; t < t1; ++t) d.out_rows[(uint64_t)t * d.out_n + row] = cand_word(cd, t, d.lenmask[t]);
             if (t0 == 0) d.out_par[row] = o;
