@@ -37,12 +37,18 @@ Engine::Engine(size_t arena_bytes) {
     CK(hipGetDevice(&dev));
     CK(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
     arena_ = std::make_unique<Arena>(arena_bytes);
-    CK(hipMalloc(&d_flag_, sizeof(unsigned int)));
+    // probe-limit failures of rehash / atom insertion: a device word the
+    // kernels set, copied back at the next stream sync after such a launch
+    CK(hipMalloc((void**)&d_flag_, sizeof(unsigned int)));
+    CK(hipMemset(d_flag_, 0, sizeof(unsigned int)));
+    CK(hipHostMalloc((void**)&h_flag_, sizeof(unsigned int), hipHostMallocDefault));
+    *h_flag_ = 0;
 }
 
 Engine::~Engine() {
     (void)hipStreamSynchronize(st_);
     if (d_step_) (void)hipFree(d_step_);
+    if (h_flag_) (void)hipHostFree(h_flag_);
     if (d_flag_) (void)hipFree(d_flag_);
     if (pin_) (void)hipHostFree(pin_);
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
@@ -52,9 +58,24 @@ Engine::~Engine() {
 }
 
 void Engine::sync() {
+    if (flag_dirty_) CK(hipMemcpyAsync(h_flag_, d_flag_, sizeof(unsigned int), hipMemcpyDeviceToHost, st_));
+    flag_dirty_ = false;
     CK(hipStreamSynchronize(st_));
     pin_used_ = 0;
     collect_times();
+    for (auto& pb : pending_free_) arena_->release(pb);
+    pending_free_.clear();
+    if (*h_flag_) throw DeviceError("hash set insertion exceeded the probe limit");
+}
+
+// a buffer still read by enqueued kernels goes back to the arena at the next
+// sync; its bytes stop counting against the instance now
+void Engine::release_after_sync(Inst& I, Buf& b) {
+    if (b.p) {
+        pending_free_.push_back(b.p);
+        I.used -= b.bytes;
+    }
+    b = Buf{};
 }
 
 void Engine::h2d(void* dst, const void* src, size_t bytes) {
@@ -358,7 +379,7 @@ void Engine::prepare_scratch(Inst& I) {
     const uint64_t nwords = (N + 63) / 64;
     const uint64_t nblk = (nwords + kChunkWords - 1) / kChunkWords;
     uint64_t fcap = std::min<uint64_t>(
-        N, std::max<uint64_t>(1 << 16, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
+        N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
     if (fcap > I.fresh_cap) {
         ensure(I, I.fresh_ord, fcap * 8);
         ensure(I, I.fresh_slot, fcap * 4);
@@ -382,7 +403,7 @@ void Engine::prepare_scratch(Inst& I) {
     h.blkprefix = (unsigned long long*)I.blkprefix.p;
     h.nblk = nblk;
     const uint64_t est_new =
-        std::min<uint64_t>(N, std::max<uint64_t>(1 << 16, (uint64_t)(N * std::min(1.0, I.win_ratio * 1.3))));
+        std::min<uint64_t>(N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.win_ratio * 1.3))));
     ensure_table(I, I.committed + est_new, false);
     fill(I.winbits.p, nwords, 0);
 }
@@ -391,6 +412,8 @@ void Engine::prepare_scratch(Inst& I) {
 // of completed epochs (ord < epoch_base)
 void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     uint64_t need = next_pow2(std::max<uint64_t>(4096, (uint64_t)(want_keys / 0.7) + 1));
+    // headroom while the set is small, so that small instances rarely rehash
+    if (need < (1ull << 24)) need = std::min<uint64_t>(need * 4, 1ull << 24);
     if (need > (1ull << 32)) need = 1ull << 32;
     if (I.table.p && need <= I.cap && !force_rebuild) {
         I.h.claim_limit = I.cap * 7 / 10 > I.committed ? I.cap * 7 / 10 - I.committed : 0;
@@ -401,15 +424,11 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     dalloc(I, nt, need * sizeof(Slot));
     fill(nt.p, need * 4, kEmpty);
     if (I.table.p) {
-        unsigned int zero = 0;
-        h2d(d_flag_, &zero, sizeof zero);
+        // the probe-limit flag is read at the next stream sync (check_flag)
         int grid = (int)std::min<uint64_t>((I.cap + 255) / 256, (uint64_t)cus_ * 8);
         CK(launch_rehash((const Slot*)I.table.p, I.cap, (Slot*)nt.p, need - 1, I.gbase, d_flag_, grid, st_));
-        unsigned int fail = 0;
-        CK(hipMemcpyAsync(&fail, d_flag_, sizeof fail, hipMemcpyDeviceToHost, st_));
-        sync();
-        if (fail) throw DeviceError("hash set rebuild exceeded the probe limit");
-        dfree(I, I.table);
+        flag_dirty_ = true;
+        release_after_sync(I, I.table);
     }
     I.table = nt;
     I.cap = need;
@@ -470,13 +489,21 @@ bool Engine::on_capacity(Inst& I, const CapacityError& e) {
 void Engine::epoch(std::vector<Inst*>& v) {
     const size_t n = v.size();
     // packed upload: descriptors | counters | tasks | fresh segments | total | chunk segments | ids
+    // small epochs use several lanes per candidate (shorter serial trace loop)
+    uint64_t waves1 = 0;
+    for (size_t k = 0; k < n; ++k)
+        for (const Task& t : v[k]->tasks)
+            waves1 += t.kind == 0 ? (t.nl + 63) / 64 : (uint64_t)t.nl * ((t.nr + 63) / 64);
+    const int wide = waves1 < kWideMaxWaves ? 1 : 0;
+    const uint64_t per_wave = wide ? 64 / kWideLanes : 64;
     std::vector<Task> tasks;
     uint64_t waves = 0;
     for (size_t k = 0; k < n; ++k)
         for (Task t : v[k]->tasks) {
             t.inst = (uint32_t)k;
             t.wave_begin = waves;
-            waves += t.kind == 0 ? (t.nl + 63) / 64 : (uint64_t)t.nl * t.nchunk;
+            t.nchunk = (uint32_t)((t.nr + per_wave - 1) / per_wave);
+            waves += t.kind == 0 ? (t.nl + per_wave - 1) / per_wave : (uint64_t)t.nl * t.nchunk;
             tasks.push_back(t);
         }
     std::vector<uint32_t> ids;
@@ -485,7 +512,8 @@ void Engine::epoch(std::vector<Inst*>& v) {
     for (size_t k = 0; k < n; ++k) {
         if (v[k]->check_only) continue;
         ids.push_back((uint32_t)k);
-        if (v[k]->h.nblk) { bsegs.push_back(Seg{(uint32_t)k, 0, btot}); btot += v[k]->h.nblk; }
+        // small bitmaps are counted inside k_scan
+        if (v[k]->h.nblk > (uint64_t)kSmallChunks) { bsegs.push_back(Seg{(uint32_t)k, 0, btot}); btot += v[k]->h.nblk; }
     }
     const size_t o_ctr = align256(n * sizeof(InstDesc));
     const size_t o_task = o_ctr + align256(n * sizeof(Ctr));
@@ -514,7 +542,7 @@ void Engine::epoch(std::vector<Inst*>& v) {
     if (waves) {
         time_begin(&ev);
         int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * 8);
-        CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, grid, st_));
+        CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, grid, st_));
         time_end(ev, 0, v);
         for (auto* I : v) I->res->expand_launches++;
     }
@@ -524,8 +552,13 @@ void Engine::epoch(std::vector<Inst*>& v) {
             if (!I->check_only) dd.push_back(I);
         time_begin(&ev);
         unsigned long long* tot = (unsigned long long*)(d + o_tot);
-        CK(launch_fresh_prefix(D, (Seg*)(d + o_fseg), (int)n, tot, st_));
-        CK(launch_resolve(D, (const Seg*)(d + o_fseg), (int)n, tot, cus_ * 16, st_));
+        // one instance: k_resolve reads its fresh count itself (segment 0 at 0)
+        const bool single = n == 1;
+        if (!single) CK(launch_fresh_prefix(D, (Seg*)(d + o_fseg), (int)n, tot, st_));
+        uint64_t fcap = 0;
+        for (auto* I : dd) fcap += I->h.fresh_cap;
+        const int rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>((fcap + 255) / 256, (uint64_t)cus_ * 16));
+        CK(launch_resolve(D, (const Seg*)(d + o_fseg), (int)n, tot, single ? 1 : 0, rgrid, st_));
         if (btot) {
             int grid = (int)std::min<uint64_t>((btot + 3) / 4, (uint64_t)cus_ * 8);
             CK(launch_count(D, (const Seg*)(d + o_bseg), (int)bsegs.size(), btot, grid, st_));
@@ -618,13 +651,10 @@ void Engine::start(std::vector<Inst*>& all, std::vector<Inst*>& out) {
         }
         h2d(d, host.data(), bytes);
         for (size_t k = 0; k < n; ++k)
-            CK(launch_insert_atoms((const InstDesc*)d, (uint32_t)k, act[k]->nrows[1], st_));
-        std::vector<Ctr> hc(n);
-        CK(hipMemcpyAsync(hc.data(), d + o_ctr, n * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
-        sync();
+            CK(launch_insert_atoms((const InstDesc*)d, (uint32_t)k, act[k]->nrows[1], d_flag_, st_));
+        flag_dirty_ = true;
         for (size_t k = 0; k < n; ++k) {
-            if (hc[k].overflow) throw DeviceError("atom insertion overflow");
-            act[k]->committed = hc[k].claims;
+            act[k]->committed = act[k]->nrows[1];   // at most A distinct atom keys
             if (!act[k]->done) out.push_back(act[k]);
         }
     }

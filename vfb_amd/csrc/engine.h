@@ -16,15 +16,15 @@
 
 namespace vfb {
 
-hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int grid,
+hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int wide, int grid,
                          hipStream_t st);
-hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, hipStream_t st);
+hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, unsigned int* fail, hipStream_t st);
 hipError_t launch_rehash(const Slot* old, uint64_t old_cap, Slot* tab, uint64_t mask, uint64_t keep_below,
                          unsigned int* fail, int grid, hipStream_t st);
 hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, int grid, hipStream_t st);
 hipError_t launch_fresh_prefix(const InstDesc* D, Seg* segs, int n, unsigned long long* total, hipStream_t st);
-hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const unsigned long long* total, int grid,
-                          hipStream_t st);
+hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const unsigned long long* total, int single,
+                          int grid, hipStream_t st);
 hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_chunks, int grid,
                         hipStream_t st);
 hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStream_t st);
@@ -166,7 +166,10 @@ class Engine {
     std::unique_ptr<Arena> arena_;
     char* d_step_ = nullptr;      // per-phase packed descriptors / counters / tasks / segments
     size_t step_cap_ = 0;
-    unsigned int* d_flag_ = nullptr;
+    unsigned int* d_flag_ = nullptr;   // probe-limit failure of rehash / atom insertion
+    unsigned int* h_flag_ = nullptr;   // its pinned host copy
+    bool flag_dirty_ = false;
+    std::vector<void*> pending_free_;
     char* pin_ = nullptr;         // pinned staging for uploads, recycled at sync points
     size_t pin_cap_ = 0, pin_used_ = 0;
     std::vector<hipEvent_t> ev_pool_;
@@ -183,6 +186,7 @@ class Engine {
     char* step_buf(size_t bytes);
     void dalloc(Inst& I, Buf& b, size_t bytes);
     void dfree(Inst& I, Buf& b);
+    void release_after_sync(Inst& I, Buf& b);
     void ensure(Inst& I, Buf& b, size_t bytes);
     void release_all(Inst& I);
     void setup(Inst& I);

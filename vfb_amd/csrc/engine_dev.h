@@ -30,6 +30,10 @@ constexpr int kChunkWords = 64;        // bitmap words per scan chunk (4096 ordi
 constexpr int kProbeLimit = 4096;
 constexpr int kStoreTraces = 8;       // traces per k_store work item
 constexpr int kStoreParts = 4;        // survivor subsets per chunk in k_store
+constexpr int kWideLanes = 8;         // lanes per candidate in k_expand's small-level mode
+constexpr uint64_t kWideMaxWaves = 2048; // levels below this many wavefronts use it
+constexpr int kTraceGroup = 8;        // traces whose operand loads k_expand issues together
+constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are counted in k_scan
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 
@@ -96,7 +100,7 @@ struct Task {
     uint32_t inst;
     uint8_t kind, lvl_l, lvl_r, pad;
     uint32_t nl, nr;
-    uint32_t nchunk;            // ceil(nr / 64)
+    uint32_t nchunk;            // ceil(nr / candidates per wavefront)
     uint32_t pad1;
     uint64_t base[4];
     uint64_t wave_begin;        // prefix over tasks, in wavefronts

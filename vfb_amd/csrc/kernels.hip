@@ -299,44 +299,137 @@ __device__ __forceinline__ void check_candidates(const InstDesc& d, const uint64
     if (__ballot(best != kEmpty) && best != kEmpty) atomicMin(&d.ctr->found, (unsigned long long)best);
 }
 
-template <int KEY, bool CHECK>
+// One candidate per lane (small-level mode): probe, then the fresh-list
+// append and claims update with one atomic each per wavefront.
+__device__ __forceinline__ void finish_one(const InstDesc& d, const Key& key, uint64_t ord, uint32_t mismatch,
+                                           bool en, int lane) {
+    uint32_t slot = 0xffffffffu;
+    int claimed = 0;
+    int r = PR_DUP;
+    if (en) r = probe_insert(d.table, d.table_mask, key.hi, key.lo, d.epoch_base + ord, d.epoch_base, &slot, &claimed);
+    const bool fresh = r == PR_FRESH;
+    uint32_t ftot = 0, ctot = 0;
+    const uint32_t fpos = wave_excl_scan_u32(fresh ? 1u : 0u, &ftot);
+    (void)wave_excl_scan_u32((uint32_t)claimed, &ctot);
+    const uint64_t fail = __ballot(r == PR_FAIL);
+    if (ftot | ctot | (fail ? 1u : 0u)) {
+        unsigned long long base = 0;
+        if (lane == 0) {
+            if (ftot) base = atomicAdd(&d.ctr->fresh_count, (unsigned long long)ftot);
+            if (ctot) {
+                const unsigned long long before = atomicAdd(&d.ctr->claims, (unsigned long long)ctot);
+                if (before + ctot > d.claim_limit) atomicOr(&d.ctr->overflow, 4u);
+            }
+            if (fail) atomicOr(&d.ctr->overflow, 1u);
+        }
+        base = bcast64(base, 0) + fpos;
+        if (fresh) {
+            if (base < d.fresh_cap) {
+                d.fresh_ord[base] = (d.epoch_base + ord) | (mismatch ? 0ull : kMatchBit);
+                d.fresh_slot[base] = slot;
+            } else {
+                atomicOr(&d.ctr->overflow, 2u);
+            }
+        }
+    }
+}
+
+// Combines the per-lane partial keys / mismatch bits of the G lanes that
+// share a candidate (G > 1: each lane folded every G-th trace), then probes
+// (or, in a check-only level, checks) the four candidates.
+template <int KEY, bool CHECK, int G>
+__device__ __forceinline__ void finish_group(const InstDesc& d, Key* k, const uint64_t* ord, uint32_t mm,
+                                             uint32_t enable, int lane) {
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint64_t h = __shfl_xor(k[q].hi, off), l = __shfl_xor(k[q].lo, off);
+                if constexpr (KEY == KEY_MIX) { k[q].hi ^= h; k[q].lo ^= l; }
+                else { k[q].hi |= h; k[q].lo |= l; }
+            }
+            mm |= __shfl_xor(mm, off);
+        }
+    }
+    const int sub = lane % G;
+    if constexpr (CHECK) {
+        check_candidates(d, ord, mm, sub == 0 ? enable : 0u);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) key_fin(k[q]);
+    if constexpr (G == 1) {
+        finish_candidates(d, k, ord, mm, enable, lane);
+    } else {
+        // lane `sub` (< 4) of the group probes candidate `sub` alone
+        const int q = sub & 3;
+        // mask blend rather than k[q], which would force the array to scratch
+        Key k1{0, 0};
+        uint64_t o1 = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint64_t m = 0ull - (uint64_t)(q == i);
+            k1.hi |= k[i].hi & m;
+            k1.lo |= k[i].lo & m;
+            o1 |= ord[i] & m;
+        }
+        const bool en1 = sub < 4 && ((enable >> q) & 1u);
+        finish_one(d, k1, o1, (mm >> q) & 1u, en1, lane);
+    }
+}
+
+// ~, X, F, G of one row.  G lanes per row; lane `sub` folds traces
+// sub, sub+G, ... in groups whose loads are issued before their first use.
+template <int KEY, bool CHECK, int G>
 __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
-    const uint64_t x = wl * 64 + (uint64_t)lane;
+    constexpr int W = 64 / G;
+    const int sub = lane % G;
+    const uint64_t x = wl * W + (uint64_t)(lane / G);
     const bool active = x < tk.nl;
     const uint64_t xr = active ? x : 0;
     const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
     Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     uint32_t mm = 0;
     const int T = d.T;
-    for (int t = 0; t < T; ++t) {
-        const uint64_t a = L[(uint64_t)t * tk.nl + xr];
-        const uint64_t m = d.lenmask[t];
-        const uint64_t c0 = ~a & m;                      // ~
-        const uint64_t c1 = a >> 1;                      // X
-        const uint64_t c2 = suffix_or(a);                // F
-        const uint64_t c3 = a & ~suffix_or(c0);          // G
-        const uint64_t want = t < d.P ? 1ull : 0ull;
-        mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
-              ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
-        if constexpr (!CHECK) {
-            key_acc<KEY>(k[0], c0, d, t);
-            key_acc<KEY>(k[1], c1, d, t);
-            key_acc<KEY>(k[2], c2, d, t);
-            key_acc<KEY>(k[3], c3, d, t);
+    for (int t0 = sub; t0 < T; t0 += G * kTraceGroup) {
+        uint64_t av[kTraceGroup];
+#pragma unroll
+        for (int j = 0; j < kTraceGroup; ++j)
+            if (t0 + j * G < T) av[j] = L[(uint64_t)(t0 + j * G) * tk.nl + xr];
+#pragma unroll
+        for (int j = 0; j < kTraceGroup; ++j) {
+            const int t = t0 + j * G;
+            if (t >= T) break;
+            const uint64_t a = av[j];
+            const uint64_t m = d.lenmask[t];
+            const uint64_t c0 = ~a & m;                      // ~
+            const uint64_t c1 = a >> 1;                      // X
+            const uint64_t c2 = suffix_or(a);                // F
+            const uint64_t c3 = a & ~suffix_or(c0);          // G
+            const uint64_t want = t < d.P ? 1ull : 0ull;
+            mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
+                  ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
+            if constexpr (!CHECK) {
+                key_acc<KEY>(k[0], c0, d, t);
+                key_acc<KEY>(k[1], c1, d, t);
+                key_acc<KEY>(k[2], c2, d, t);
+                key_acc<KEY>(k[3], c3, d, t);
+            }
         }
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) key_fin(k[q]);
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
-    if constexpr (CHECK) check_candidates(d, ord, mm, epoch_enable(d, ord, active));
-    else finish_candidates(d, k, ord, mm, epoch_enable(d, ord, active), lane);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, epoch_enable(d, ord, active), lane);
 }
 
-template <int KEY, bool CHECK>
+// &, |, l U r, r U l of one pair (left row uniform per wavefront)
+template <int KEY, bool CHECK, int G>
 __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    constexpr int W = 64 / G;
+    const int sub = lane % G;
     const uint64_t l = wl / tk.nchunk;
     const uint64_t rc = wl - l * tk.nchunk;
-    const uint64_t r = rc * 64 + (uint64_t)lane;
+    const uint64_t r = rc * W + (uint64_t)(lane / G);
     const bool active = r < tk.nr;
     const uint64_t rr = active ? r : 0;
     const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
@@ -344,32 +437,43 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     uint32_t mm = 0;
     const int T = d.T;
-    for (int t = 0; t < T; ++t) {
-        const uint64_t a = L[(uint64_t)t * tk.nl + l];
-        const uint64_t b = R[(uint64_t)t * tk.nr + rr];
-        const uint64_t c0 = a & b;
-        const uint64_t c1 = a | b;
-        const uint64_t c2 = until_op(a, b);
-        const uint64_t c3 = until_op(b, a);
-        const uint64_t want = t < d.P ? 1ull : 0ull;
-        mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
-              ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
-        if constexpr (!CHECK) {
-            key_acc<KEY>(k[0], c0, d, t);
-            key_acc<KEY>(k[1], c1, d, t);
-            key_acc<KEY>(k[2], c2, d, t);
-            key_acc<KEY>(k[3], c3, d, t);
+    for (int t0 = sub; t0 < T; t0 += G * kTraceGroup) {
+        uint64_t av[kTraceGroup], bv[kTraceGroup];
+#pragma unroll
+        for (int j = 0; j < kTraceGroup; ++j)
+            if (t0 + j * G < T) {
+                av[j] = L[(uint64_t)(t0 + j * G) * tk.nl + l];
+                bv[j] = R[(uint64_t)(t0 + j * G) * tk.nr + rr];
+            }
+#pragma unroll
+        for (int j = 0; j < kTraceGroup; ++j) {
+            const int t = t0 + j * G;
+            if (t >= T) break;
+            const uint64_t a = av[j], b = bv[j];
+            const uint64_t c0 = a & b;
+            const uint64_t c1 = a | b;
+            const uint64_t c2 = until_op(a, b);
+            const uint64_t c3 = until_op(b, a);
+            const uint64_t want = t < d.P ? 1ull : 0ull;
+            mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
+                  ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
+            if constexpr (!CHECK) {
+                key_acc<KEY>(k[0], c0, d, t);
+                key_acc<KEY>(k[1], c1, d, t);
+                key_acc<KEY>(k[2], c2, d, t);
+                key_acc<KEY>(k[3], c3, d, t);
+            }
         }
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) key_fin(k[q]);
     const uint64_t pair = l * tk.nr + r;
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
-    if constexpr (CHECK) check_candidates(d, ord, mm, epoch_enable(d, ord, active));
-    else finish_candidates(d, k, ord, mm, epoch_enable(d, ord, active), lane);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, epoch_enable(d, ord, active), lane);
 }
 
+// G = lanes per candidate: 1 for large levels (throughput), kWideLanes for
+// small ones (eight times more wavefronts, an eighth of the serial trace loop)
+template <int G>
 __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, const Task* __restrict__ tasks,
                                                 int ntasks, uint64_t total_waves) {
     const int lane = (int)(threadIdx.x & 63);
@@ -386,16 +490,16 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
         const InstDesc& d = D[tk.inst];
         const uint64_t wl = w - tk.wave_begin;
         if (d.check_only) {
-            if (tk.kind == 0) expand_unary<KEY_MIX, true>(d, tk, wl, lane);
-            else expand_binary<KEY_MIX, true>(d, tk, wl, lane);
+            if (tk.kind == 0) expand_unary<KEY_MIX, true, G>(d, tk, wl, lane);
+            else expand_binary<KEY_MIX, true, G>(d, tk, wl, lane);
         } else if (tk.kind == 0) {
-            if (d.key_kind == KEY_MIX) expand_unary<KEY_MIX, false>(d, tk, wl, lane);
-            else if (d.key_kind == KEY_FIELD) expand_unary<KEY_FIELD, false>(d, tk, wl, lane);
-            else expand_unary<KEY_STRIDE, false>(d, tk, wl, lane);
+            if (d.key_kind == KEY_MIX) expand_unary<KEY_MIX, false, G>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_unary<KEY_FIELD, false, G>(d, tk, wl, lane);
+            else expand_unary<KEY_STRIDE, false, G>(d, tk, wl, lane);
         } else {
-            if (d.key_kind == KEY_MIX) expand_binary<KEY_MIX, false>(d, tk, wl, lane);
-            else if (d.key_kind == KEY_FIELD) expand_binary<KEY_FIELD, false>(d, tk, wl, lane);
-            else expand_binary<KEY_STRIDE, false>(d, tk, wl, lane);
+            if (d.key_kind == KEY_MIX) expand_binary<KEY_MIX, false, G>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_binary<KEY_FIELD, false, G>(d, tk, wl, lane);
+            else expand_binary<KEY_STRIDE, false, G>(d, tk, wl, lane);
         }
     }
 }
@@ -403,25 +507,23 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
 // Inserts the rows of level 1 (the atoms) with ordinals 0..A-1 so that later
 // levels deduplicate against them (hashSetsInitialisation, LTL_BS.cu:136-160).
 template <int KEY>
-__device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n) {
+__device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned int* fail) {
     const unsigned long long* L = d.lvl[1];
     Key k{0, 0};
     for (int t = 0; t < d.T; ++t) key_acc<KEY>(k, L[(uint64_t)t * n + x], d, t);
     key_fin(k);
     uint32_t slot;
     int claimed = 0;
-    const int r = probe_insert(d.table, d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed);
-    if (r == PR_FAIL) atomicOr(&d.ctr->overflow, 1u);
-    if (claimed) atomicAdd(&d.ctr->claims, 1ull);
+    if (probe_insert(d.table, d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed) == PR_FAIL) atomicOr(fail, 1u);
 }
 
-__global__ void k_insert_atoms(const InstDesc* __restrict__ D, uint32_t inst, uint32_t n) {
+__global__ void k_insert_atoms(const InstDesc* __restrict__ D, uint32_t inst, uint32_t n, unsigned int* fail) {
     const uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (x >= n) return;
     const InstDesc& d = D[inst];
-    if (d.key_kind == KEY_MIX) insert_row<KEY_MIX>(d, x, n);
-    else if (d.key_kind == KEY_FIELD) insert_row<KEY_FIELD>(d, x, n);
-    else insert_row<KEY_STRIDE>(d, x, n);
+    if (d.key_kind == KEY_MIX) insert_row<KEY_MIX>(d, x, n, fail);
+    else if (d.key_kind == KEY_FIELD) insert_row<KEY_FIELD>(d, x, n, fail);
+    else insert_row<KEY_STRIDE>(d, x, n, fail);
 }
 
 // Moves the keys of completed epochs (ord < keep_below) into a fresh table.
@@ -469,8 +571,15 @@ __global__ void k_fresh_prefix(const InstDesc* __restrict__ D, Seg* segs, int n,
 }
 
 __global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
-                                                 int nseg, const unsigned long long* __restrict__ total_p) {
-    const uint64_t total = *total_p;
+                                                 int nseg, const unsigned long long* __restrict__ total_p,
+                                                 int single) {
+    uint64_t total;
+    if (single) {
+        const uint64_t c = D[0].ctr->fresh_count;
+        total = c < D[0].fresh_cap ? c : D[0].fresh_cap;
+    } else {
+        total = *total_p;
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
         const Seg& sg = segs[seg_find(segs, nseg, g)];
@@ -522,8 +631,17 @@ __global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, co
     __shared__ uint64_t carry;
     const InstDesc& d = D[insts[blockIdx.x]];
     if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (d.nblk <= (uint64_t)kSmallChunks) {
+        // small bitmap: count the chunks here (one wavefront per chunk)
+        for (uint64_t c = w; c < d.nblk; c += 4) {
+            const uint64_t word = c * kChunkWords + lane;
+            uint32_t v = word < d.nwords ? (uint32_t)__popcll(d.winbits[word]) : 0u;
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) d.blkcount[c] = v;
+        }
+    }
+    __syncthreads();
     for (uint64_t base = 0; base < d.nblk; base += 256) {
         const uint64_t i = base + threadIdx.x;
         const uint64_t v = i < d.nblk ? d.blkcount[i] : 0;
@@ -666,14 +784,17 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
 // launch wrappers (host)
 // ---------------------------------------------------------------------------
 
-hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int grid,
+hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int wide, int grid,
                          hipStream_t st) {
-    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
+    if (wide)
+        hipLaunchKernelGGL(k_expand<kWideLanes>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
+    else
+        hipLaunchKernelGGL(k_expand<1>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
     return hipGetLastError();
 }
 
-hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, hipStream_t st) {
-    hipLaunchKernelGGL(k_insert_atoms, dim3((n + 63) / 64), dim3(64), 0, st, D, inst, n);
+hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, unsigned int* fail, hipStream_t st) {
+    hipLaunchKernelGGL(k_insert_atoms, dim3((n + 63) / 64), dim3(64), 0, st, D, inst, n, fail);
     return hipGetLastError();
 }
 
@@ -694,9 +815,9 @@ hipError_t launch_fresh_prefix(const InstDesc* D, Seg* segs, int n, unsigned lon
     return hipGetLastError();
 }
 
-hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const unsigned long long* total, int grid,
-                          hipStream_t st) {
-    hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, st, D, segs, nseg, total);
+hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const unsigned long long* total, int single,
+                          int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_resolve, dim3(grid), dim3(256), 0, st, D, segs, nseg, total, single);
     return hipGetLastError();
 }
 
