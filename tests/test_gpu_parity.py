@@ -99,7 +99,8 @@ def test_batch_equals_plain():
 
 
 @pytest.mark.parametrize("cfg,k,max_rows", [(2, 2, 3000), (2, 2, 20000), (2, 3, 100000), (2, 21, 500),
-                                            (1, 0, 40), (3, 4, 5000), (2, 5, 130)])
+                                            (1, 0, 40), (3, 4, 5000), (2, 5, 130), (1, 1, 60), (1, 2, 200),
+                                            (4, 7, 90), (4, 14, 400), (4, 20, 1500), (3, 9, 300)])
 def test_overflow_onthefly(cfg, k, max_rows):
     """Row cap (the reference's cache capacity): the overflowing level is stored
     up to the cap in ordinal order, the next level is check-only."""
@@ -145,3 +146,17 @@ def test_server_results_equal_plain():
                   "otf_level", "otf_stored"):
             assert a[f] == b[f], (k, f, a[f], b[f])
         assert b["ms_total"] > 0
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_overflow_random(seed):
+    """Check-only levels over random instances of every length, incl. 64."""
+    rng = random.Random(500 + seed)
+    inst = _random_instance(rng, rng.randint(1, 4), rng.randint(2, 20), 1, 64 if seed % 3 == 0 else 12)
+    names, lens, atoms, P = inst.arrays()
+    cap = rng.randint(8, 200)
+    print(f"[parity] overflow random seed {seed} cap {cap}", flush=True)
+    want = O.enumerate_cs(O.Instance(names, lens, atoms, P), 8, 3, cap)
+    got = V.solve(inst, max_size=8, max_rows=cap)
+    for f in ("status", "min_size", "formula", "distinct", "candidates", "otf_level", "otf_stored"):
+        assert got[f] == want[f], (f, got[f], want[f])

@@ -59,6 +59,19 @@ __device__ __forceinline__ uint64_t apply_op(int op, uint64_t a, uint64_t b, uin
     }
 }
 
+// bit 0 of l U r: some k with r_k and l_j for all j < k; the first set bit
+// of r is the best witness (the doubling of until_op evaluated at time 0)
+__device__ __forceinline__ bool until_bit0(uint64_t l, uint64_t r) {
+    if (r == 0) return false;
+    const uint64_t below = (r & (0ull - r)) - 1ull;   // bits under the lowest set bit of r
+    return (l & below) == below;
+}
+
+__device__ __forceinline__ uint32_t bit0_mismatch(uint64_t b0, uint64_t b1, bool b2, bool b3, uint64_t want) {
+    return (uint32_t)(b0 ^ want) | ((uint32_t)(b1 ^ want) << 1) | ((uint32_t)((uint64_t)b2 ^ want) << 2) |
+           ((uint32_t)((uint64_t)b3 ^ want) << 3);
+}
+
 struct Key {
     uint64_t hi, lo;
 };
@@ -403,11 +416,16 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
             if (t >= T) break;
             const uint64_t a = av[j];
             const uint64_t m = d.lenmask[t];
+            const uint64_t want = t < d.P ? 1ull : 0ull;
+            if constexpr (CHECK) {
+                // only bit 0 (time 0) matters: closed forms of the operators there
+                mm |= bit0_mismatch(~a & 1ull, (a >> 1) & 1ull, a != 0, a == m, want);
+                continue;
+            }
             const uint64_t c0 = ~a & m;                      // ~
             const uint64_t c1 = a >> 1;                      // X
             const uint64_t c2 = suffix_or(a);                // F
             const uint64_t c3 = a & ~suffix_or(c0);          // G
-            const uint64_t want = t < d.P ? 1ull : 0ull;
             mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
                   ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
             if constexpr (!CHECK) {
@@ -450,11 +468,15 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
             const int t = t0 + j * G;
             if (t >= T) break;
             const uint64_t a = av[j], b = bv[j];
+            const uint64_t want = t < d.P ? 1ull : 0ull;
+            if constexpr (CHECK) {
+                mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
+                continue;
+            }
             const uint64_t c0 = a & b;
             const uint64_t c1 = a | b;
             const uint64_t c2 = until_op(a, b);
             const uint64_t c3 = until_op(b, a);
-            const uint64_t want = t < d.P ? 1ull : 0ull;
             mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
                   ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
             if constexpr (!CHECK) {
