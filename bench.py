@@ -132,6 +132,27 @@ def served(V, rate, seconds, max_size, max_active, check):
             "checked": len(sample)}
 
 
+def limits(V, ks, max_size):
+    """Config 3 (10 APs, 63 traces of length 48-64, max size 18): large levels,
+    where the combine+dedup kernel runs at its bandwidth roof."""
+    out = []
+    for k in ks:
+        inst = V.Instance.generate(3, k)
+        V.solve(inst, max_size=max_size)            # warm
+        t0 = time.perf_counter()
+        r = V.solve(inst, max_size=max_size, timing=True)
+        dt = time.perf_counter() - t0
+        ach = r["bytes_expand"] / (r["ms_expand"] * 1e-3) / 1e9 if r["ms_expand"] else None
+        out.append({"instance": k, "status": V.status_string(r["status"]), "min_size": r["min_size"],
+                    "seconds": dt, "distinct_cs": sum(r["distinct"]),
+                    "distinct_per_s": sum(r["distinct"]) / dt,
+                    "candidates_per_s": sum(r["level_candidates"]) / dt,
+                    "onthefly_level": r["otf_level"],
+                    "k_expand_ms": r["ms_expand"], "k_expand_launches": r["expand_launches"],
+                    "k_expand_alg_GBps": ach, "k_expand_frac_of_hbm": ach / HBM_PEAK_GBS if ach else None})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +167,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="config-5 served stream length (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-active", type=int, default=128)
+    ap.add_argument("--limits", default="4,10", help="config-3 instances to time (empty: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -234,6 +256,9 @@ def main():
                      "alg_bytes_per_launch": bytes_exp / launches if launches else None},
         "device": dev,
     }
+    if args.limits:
+        print("[bench] limits config", file=sys.stderr, flush=True)
+        out["limits"] = limits(V, [int(x) for x in args.limits.split(",")], 18)
     if args.mix:
         print("[bench] batched mix", file=sys.stderr, flush=True)
         out["batched_mix"] = batched_mix(V, args.mix, 16)

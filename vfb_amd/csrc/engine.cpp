@@ -536,6 +536,7 @@ void Engine::epoch(std::vector<Inst*>& v) {
     if (!bsegs.empty()) std::memcpy(host.data() + o_bseg, bsegs.data(), bsegs.size() * sizeof(Seg));
     if (!ids.empty()) std::memcpy(host.data() + o_ids, ids.data(), ids.size() * 4);
     h2d(d, host.data(), bytes);
+    epoch_single_ = n == 1 ? v[0] : nullptr;
     const InstDesc* D = (const InstDesc*)d;
 
     hipEvent_t ev = nullptr;
@@ -583,6 +584,17 @@ void Engine::store(std::vector<Inst*>& v) {
         tot += I.h.nblk * kStoreParts * (uint64_t)((I.h.T + kStoreTraces - 1) / kStoreTraces);
     }
     if (!tot) return;
+    hipEvent_t ev = nullptr;
+    const int grid = (int)std::min<uint64_t>((tot + 3) / 4, (uint64_t)cus_ * 8);
+    if (n == 1 && epoch_single_ == v[0]) {
+        // the epoch left this instance's descriptor at d_step_[0]: no upload
+        const Inst& I = *v[0];
+        const StorePatch p{I.h.out_rows, I.h.out_par, I.h.out_n};
+        time_begin(&ev);
+        CK(launch_store((const InstDesc*)d_step_, nullptr, 0, tot, p, grid, st_));
+        time_end(ev, 2, v);
+        return;
+    }
     const size_t o_seg = align256(n * sizeof(InstDesc));
     const size_t bytes = o_seg + align256(segs.size() * sizeof(Seg));
     char* d = step_buf(bytes);
@@ -590,10 +602,9 @@ void Engine::store(std::vector<Inst*>& v) {
     for (size_t k = 0; k < n; ++k) ((InstDesc*)host.data())[k] = v[k]->h;
     std::memcpy(host.data() + o_seg, segs.data(), segs.size() * sizeof(Seg));
     h2d(d, host.data(), bytes);
-    hipEvent_t ev = nullptr;
+    epoch_single_ = nullptr;
     time_begin(&ev);
-    int grid = (int)std::min<uint64_t>((tot + 3) / 4, (uint64_t)cus_ * 8);
-    CK(launch_store((const InstDesc*)d, (const Seg*)(d + o_seg), (int)segs.size(), tot, grid, st_));
+    CK(launch_store((const InstDesc*)d, (const Seg*)(d + o_seg), (int)segs.size(), tot, StorePatch{}, grid, st_));
     time_end(ev, 2, v);
 }
 

@@ -28,8 +28,8 @@ hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const un
 hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_chunks, int grid,
                         hipStream_t st);
 hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStream_t st);
-hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, int grid,
-                        hipStream_t st);
+hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch,
+                        int grid, hipStream_t st);
 
 struct DeviceError : std::runtime_error {
     explicit DeviceError(const std::string& s) : std::runtime_error(s) {}
@@ -180,6 +180,7 @@ class Engine {
     };
     std::vector<Timed> timed_;
     std::vector<Inst*> live_;     // started, not finished: eviction candidates
+    Inst* epoch_single_ = nullptr; // the one instance whose descriptor is at d_step_[0]
 
     void sync();
     void h2d(void* dst, const void* src, size_t bytes);

@@ -106,6 +106,14 @@ struct Task {
     uint64_t wave_begin;        // prefix over tasks, in wavefronts
 };
 
+// Output of a level's store, passed by value when one instance is stored
+// (its descriptor is then the one already on the device from the epoch).
+struct StorePatch {
+    unsigned long long* out_rows;
+    unsigned long long* out_par;
+    uint64_t out_n;
+};
+
 // Segment of a flat index space belonging to one instance.
 struct Seg {
     uint32_t inst, pad;

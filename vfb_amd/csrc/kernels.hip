@@ -756,7 +756,7 @@ __device__ __forceinline__ uint64_t cand_word(const Cand& c, int t, uint64_t m) 
 // rows of the slice's columns (coalesced stores).  Slice 0 also writes the
 // parent ordinals.
 __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
-                                               int nseg, uint64_t total_items) {
+                                               int nseg, uint64_t total_items, StorePatch patch) {
     __shared__ uint16_t lists[4][64 * kChunkWords];
     const int lane = threadIdx.x & 63;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -764,10 +764,14 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
     const uint64_t wpb = blockDim.x >> 6;
     const uint64_t stride = (uint64_t)gridDim.x * wpb;
     for (uint64_t g = blockIdx.x * wpb + wv; g < total_items; g += stride) {
-        const Seg& sg = segs[seg_find(segs, nseg, g)];
-        const InstDesc& d = D[sg.inst];
+        // nseg == 0: one instance, descriptor 0, outputs from `patch`
+        const int si = nseg ? seg_find(segs, nseg, g) : 0;
+        const InstDesc& d = D[nseg ? segs[si].inst : 0];
+        const uint64_t item = g - (nseg ? segs[si].begin : 0);
+        unsigned long long* const out_rows = nseg ? d.out_rows : patch.out_rows;
+        unsigned long long* const out_par = nseg ? d.out_par : patch.out_par;
+        const uint64_t out_n = nseg ? d.out_n : patch.out_n;
         const uint32_t nsl = (uint32_t)(d.T + kStoreTraces - 1) / kStoreTraces;
-        const uint64_t item = g - sg.begin;
         const uint64_t c = item / (nsl * kStoreParts);
         const uint32_t rem = (uint32_t)(item - c * nsl * kStoreParts);
         const uint32_t part = rem / nsl;
@@ -775,7 +779,7 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         const int t1 = t0 + kStoreTraces < d.T ? t0 + kStoreTraces : d.T;
         const uint32_t n = d.blkcount[c];
         const uint64_t rank0 = d.blkprefix[c];
-        if (n == 0 || rank0 >= d.out_n) continue;
+        if (n == 0 || rank0 >= out_n) continue;
         const uint64_t w = c * kChunkWords + lane;
         uint64_t bits = w < d.nwords ? d.winbits[w] : 0ull;
         const uint64_t pc = __popcll(bits);
@@ -788,15 +792,15 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         const uint64_t obase = c * (uint64_t)(kChunkWords * 64);
-        const uint32_t lim = rank0 + n <= d.out_n ? n : (uint32_t)(d.out_n - rank0);
+        const uint32_t lim = rank0 + n <= out_n ? n : (uint32_t)(out_n - rank0);
         // the chunk's 4096 ordinals span few units: find the first one once
         const int u0 = unit_of(d, obase);
         for (uint32_t i = part * 64 + lane; i < lim; i += 64 * kStoreParts) {
             const uint64_t o = obase + list[i];
             const Cand cd = decode_from(d, o, u0);
             const uint64_t row = rank0 + i;
-            for (int t = t0; t < t1; ++t) d.out_rows[(uint64_t)t * d.out_n + row] = cand_word(cd, t, d.lenmask[t]);
-            if (t0 == 0) d.out_par[row] = o;
+            for (int t = t0; t < t1; ++t) out_rows[(uint64_t)t * out_n + row] = cand_word(cd, t, d.lenmask[t]);
+            if (t0 == 0) out_par[row] = o;
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -854,9 +858,9 @@ hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, int grid,
-                        hipStream_t st) {
-    hipLaunchKernelGGL(k_store, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_items);
+hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch,
+                        int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_store, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_items, patch);
     return hipGetLastError();
 }
 
