@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-active", type=int, default=128)
     ap.add_argument("--limits", default="4,10", help="config-3 instances to time (empty: skip)")
+    ap.add_argument("--only", choices=["limits", "mix"], default=None,
+                    help="run one section alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -180,6 +182,12 @@ def main():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
     print(f"[bench] device {dev}", file=sys.stderr, flush=True)
+    if args.only == "limits":
+        print(json.dumps({"limits": limits(V, [int(x) for x in args.limits.split(",")], 18)}))
+        return
+    if args.only == "mix":
+        print(json.dumps({"batched_mix": batched_mix(V, args.mix, 16)}))
+        return
     insts = [V.Instance.generate(2, k) for k in range(args.instances)]
 
     def step():

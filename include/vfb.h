@@ -98,8 +98,11 @@ typedef struct {
                                               survivors ordered before the returned formula */
     uint64_t candidates[VFB_MAX_SIZE + 1]; /* candidates per level; at the found level,
                                               through the returned formula */
-    uint64_t level_winners[VFB_MAX_SIZE + 1]; /* distinct CS per processed level, all of it */
-    uint64_t level_candidates[VFB_MAX_SIZE + 1]; /* candidates per processed level, all of it */
+    uint64_t level_winners[VFB_MAX_SIZE + 1]; /* distinct CS per processed level, all of it;
+                                                 at the found level, of its evaluated part */
+    uint64_t level_candidates[VFB_MAX_SIZE + 1]; /* candidates evaluated per level: all of a
+                                                    completed level; at the found level, those
+                                                    of the passes run (vfb_tuning.pass_min) */
     uint64_t found_ordinal; /* canonical ordinal of the returned formula in its level */
     char* formula;         /* ltl_nocache.cu:289-317 syntax; "Not found !" if none */
     double ms_total;       /* wall time of the solve (host clock) */
@@ -165,6 +168,45 @@ void vfb_server_stats(vfb_server* srv, uint64_t* queued, uint64_t* active, uint6
 
 /* Stops the loop (requests still queued or running are dropped). */
 void vfb_server_destroy(vfb_server* srv);
+
+/* ---- tuning and the autotuner ------------------------------------------ */
+
+/* Launch and scheduling parameters of the process-wide engine.  They change
+   speed only: every result is identical under every valid setting.  No
+   reference counterpart (the reference hard-codes 1024-thread blocks and one
+   launch per temp-buffer chunk, LTL_BS.cu:383-446, 1226-1493). */
+typedef struct {
+    uint64_t pass_min;             /* levels with at least this many candidates are built
+                                      in three passes in canonical order (~ X F G, & |, U)
+                                      and stop after the pass that settles a match;
+                                      0 = always one fused pass */
+    uint32_t wide_max_waves;       /* an expansion of fewer wavefronts gives each
+                                      candidate 8 lanes (0 = never) */
+    uint32_t expand_blocks_per_cu; /* 256-thread workgroups per CU, expand launch, 1..64 */
+    uint32_t resolve_blocks_per_cu;/* same, winner resolution, 1..64 */
+    uint32_t store_blocks_per_cu;  /* same, survivor storage, 1..64 */
+} vfb_tuning;
+
+void vfb_tuning_default(vfb_tuning* t);
+/* Current settings of the process (the defaults until vfb_set_tuning). */
+void vfb_get_tuning(vfb_tuning* t);
+/* VFB_INVALID_INPUT (and no change) for a value out of range. */
+vfb_status vfb_set_tuning(const vfb_tuning* t);
+/* Text file of "key value" lines, with the device it was tuned on. */
+vfb_status vfb_tuning_save(const vfb_tuning* t, const char* path);
+vfb_status vfb_tuning_load(const char* path, vfb_tuning* t);
+
+/* Project autotuner: coordinate search over the vfb_tuning fields, timing
+   `reps` runs of the sample per setting -- vfb_solve_batch over all of it
+   (batched = 1) or vfb_solve of each instance in turn (batched = 0) -- and
+   keeping the fastest.  Every setting's results are checked against those of
+   the first; a difference returns VFB_DEVICE_ERROR.  On return the best
+   setting is active, stored in *best, and saved to cache_path if not NULL;
+   *ms_default / *ms_best receive the sample's time under the starting and
+   the chosen settings. */
+vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, const vfb_options* opt,
+                        int batched, int reps, const char* cache_path, vfb_tuning* best,
+                        double* ms_default, double* ms_best);
 
 /* Device name and compute-unit count, for reports.  Returns VFB_DEVICE_ERROR
    when no HIP device is usable. */

@@ -1,0 +1,101 @@
+"""GPU parity under every kind of tuning setting: launch shapes and the pass
+split change speed only, never a result.
+
+pass_min=1 builds every level with a binary operator in three passes
+(~ X F G, then & |, then U) that stop at the pass settling a match; the
+results must still equal the oracle's bit for bit, including the per-level
+counts and the returned formula.
+"""
+import random
+
+import pytest
+
+import oracle_lib as O
+import vfb_amd as V
+from ltl_semantics import separates
+
+pytestmark = pytest.mark.gpu
+
+SETTINGS = {
+    "split_all_levels": dict(pass_min=1),
+    "split_all_levels_wide": dict(pass_min=1, wide_max_waves=1 << 30),
+    "never_wide_small_grids": dict(pass_min=0, wide_max_waves=0, expand_blocks_per_cu=1,
+                                   resolve_blocks_per_cu=1, store_blocks_per_cu=1),
+    "big_grids": dict(pass_min=1 << 16, expand_blocks_per_cu=64, resolve_blocks_per_cu=64,
+                      store_blocks_per_cu=64),
+}
+
+FIELDS = ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "otf_level", "otf_stored")
+
+
+@pytest.fixture(params=sorted(SETTINGS))
+def tuned(request):
+    old = V.set_tuning(**SETTINGS[request.param])
+    yield request.param
+    V.set_tuning(**old)
+
+
+def _cases():
+    out = [(1, k, 10, 0) for k in range(3)] + [(2, k, 8, 0) for k in range(4)] + [(3, 1, 6, 0)]
+    out += [(2, 2, 10, 20000), (4, 14, 10, 400), (1, 0, 10, 40)]
+    return out
+
+
+@pytest.mark.parametrize("cfg,k,ms,rows", _cases())
+def test_tuned_parity(tuned, cfg, k, ms, rows):
+    inst = V.Instance.generate(cfg, k)
+    names, lens, atoms, P = inst.arrays()
+    print(f"[parity] tuning {tuned} cfg{cfg} #{k} max {ms} rows {rows}", flush=True)
+    want = O.enumerate_cs(O.Instance(names, lens, atoms, P), ms, 3, rows)
+    got = V.solve(inst, max_size=ms, max_rows=rows)
+    for f in FIELDS:
+        assert got[f] == want[f], (tuned, f, got[f], want[f])
+    if got["status"] == V.OK:
+        assert separates(got["formula"], names, lens, atoms, P)
+
+
+def test_tuned_random(tuned):
+    rng = random.Random(4242)
+    for seed in range(6):
+        T = rng.randint(2, 24)
+        lens = [rng.randint(1, 64 if seed % 2 else 10) for _ in range(T)]
+        A = rng.randint(1, 4)
+        atoms = [[sum((rng.random() < 0.5) << j for j in range(lens[t])) for t in range(T)] for _ in range(A)]
+        P = rng.randint(1, T - 1)
+        inst = V.Instance.from_bits([f"p{a}" for a in range(A)], lens, atoms, P)
+        print(f"[parity] tuning {tuned} random {seed}", flush=True)
+        want = O.enumerate_cs(O.Instance([f"p{a}" for a in range(A)], lens, atoms, P), 7, 3)
+        got = V.solve(inst, max_size=7)
+        for f in FIELDS[:6]:
+            assert got[f] == want[f], (tuned, seed, f, got[f], want[f])
+
+
+def test_split_batch_equals_plain():
+    """Instances at different passes of their levels share launches."""
+    old = V.set_tuning(pass_min=1)
+    try:
+        insts = [V.Instance.generate(4, k) for k in range(64)]
+        plain = [V.solve(i, max_size=12) for i in insts]
+        batch = V.solve_batch(insts, max_size=12)
+        for k, (a, b) in enumerate(zip(plain, batch)):
+            for f in FIELDS + ("level_winners", "level_candidates"):
+                assert a[f] == b[f], (k, f, a[f], b[f])
+    finally:
+        V.set_tuning(**old)
+
+
+def test_split_saves_work_at_found_level():
+    """With the split, a level settled by its first or second pass evaluates
+    fewer candidates than the fused pass; the answer is unchanged."""
+    inst = V.Instance.generate(2, 3)          # its formula's root is ~ (first pass)
+    old = V.set_tuning(pass_min=0)
+    try:
+        fused = V.solve(inst, max_size=14)
+        V.set_tuning(pass_min=1)
+        split = V.solve(inst, max_size=14)
+    finally:
+        V.set_tuning(**old)
+    s = fused["min_size"]
+    assert split["formula"] == fused["formula"] and split["distinct"] == fused["distinct"]
+    assert split["level_candidates"][s] < fused["level_candidates"][s]
+    assert split["level_candidates"][:s] == fused["level_candidates"][:s]

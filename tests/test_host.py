@@ -134,3 +134,35 @@ def test_parser_roundtrip():
     for f in ["a0", "~(a1)", "(a0) & (X(X1))", "((a0) U (G(a1))) | (F(~(X1)))"]:
         tree = parse(f, names)
         assert tree
+
+
+def test_tuning_set_get_and_range_checks():
+    """vfb_set_tuning accepts in-range settings, rejects others unchanged."""
+    base = V.get_tuning()
+    assert base == V.tuning_default() or set(base) == set(V.TUNING_FIELDS)
+    try:
+        V.set_tuning(pass_min=1, wide_max_waves=0, expand_blocks_per_cu=3)
+        got = V.get_tuning()
+        assert (got["pass_min"], got["wide_max_waves"], got["expand_blocks_per_cu"]) == (1, 0, 3)
+        for bad in ({"expand_blocks_per_cu": 0}, {"resolve_blocks_per_cu": 65}, {"store_blocks_per_cu": 0}):
+            with pytest.raises(V.VfbError):
+                V.set_tuning(**bad)
+            assert V.get_tuning() == got
+    finally:
+        V.set_tuning(**base)
+    assert V.get_tuning() == base
+
+
+def test_tuning_file_roundtrip(tmp_path):
+    t = dict(V.tuning_default(), pass_min=12345, wide_max_waves=77, store_blocks_per_cu=16)
+    path = tmp_path / "tuning.txt"
+    V.save_tuning(t, path)
+    text = path.read_text()
+    assert "pass_min 12345" in text and text.startswith("#")
+    assert V.load_tuning(path) == t
+    # missing keys keep their defaults; an out-of-range value is refused
+    path.write_text("wide_max_waves 5\n")
+    assert V.load_tuning(path) == dict(V.tuning_default(), wide_max_waves=5)
+    path.write_text("expand_blocks_per_cu 0\n")
+    with pytest.raises(V.VfbError):
+        V.load_tuning(path)

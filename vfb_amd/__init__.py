@@ -43,6 +43,15 @@ class Result(C.Structure):
                 ("otf_level", C.c_int), ("pad", C.c_int), ("otf_stored", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("pass_min", C.c_uint64), ("wide_max_waves", C.c_uint32),
+                ("expand_blocks_per_cu", C.c_uint32), ("resolve_blocks_per_cu", C.c_uint32),
+                ("store_blocks_per_cu", C.c_uint32)]
+
+
+TUNING_FIELDS = [f for f, _ in Tuning._fields_]
+
+
 # every symbol declared in include/vfb.h
 EXPORTS = [
     "vfb_instance_load_json", "vfb_instance_from_bits", "vfb_instance_generate",
@@ -52,6 +61,8 @@ EXPORTS = [
     "vfb_result_free", "vfb_status_string", "vfb_device_info",
     "vfb_server_create", "vfb_server_submit", "vfb_server_poll", "vfb_server_wait",
     "vfb_server_stats", "vfb_server_destroy",
+    "vfb_tuning_default", "vfb_get_tuning", "vfb_set_tuning", "vfb_tuning_save",
+    "vfb_tuning_load", "vfb_autotune",
 ]
 
 _lib = None
@@ -99,6 +110,16 @@ def lib():
     L.vfb_server_stats.restype = None
     L.vfb_server_destroy.argtypes = [vp]
     L.vfb_server_destroy.restype = None
+    TP = C.POINTER(Tuning)
+    L.vfb_tuning_default.argtypes = [TP]
+    L.vfb_tuning_default.restype = None
+    L.vfb_get_tuning.argtypes = [TP]
+    L.vfb_get_tuning.restype = None
+    L.vfb_set_tuning.argtypes = [TP]
+    L.vfb_tuning_save.argtypes = [TP, C.c_char_p]
+    L.vfb_tuning_load.argtypes = [C.c_char_p, TP]
+    L.vfb_autotune.argtypes = [C.POINTER(vp), C.c_size_t, C.POINTER(Options), C.c_int, C.c_int,
+                               C.c_char_p, TP, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -241,6 +262,61 @@ def solve_batch(insts, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0
         out.append(_todict(rs[k]))
         lib().vfb_result_free(C.byref(rs[k]))
     return out
+
+
+def _tdict(t):
+    return {f: getattr(t, f) for f in TUNING_FIELDS}
+
+
+def tuning_default():
+    t = Tuning()
+    lib().vfb_tuning_default(C.byref(t))
+    return _tdict(t)
+
+
+def get_tuning():
+    """Launch / pass parameters of the process (speed only, never results)."""
+    t = Tuning()
+    lib().vfb_get_tuning(C.byref(t))
+    return _tdict(t)
+
+
+def set_tuning(**kw):
+    """Changes the given vfb_tuning fields; returns the previous settings."""
+    old = get_tuning()
+    t = Tuning(**{**old, **kw})
+    st = lib().vfb_set_tuning(C.byref(t))
+    if st != OK:
+        raise VfbError(f"set_tuning: {status_string(st)}")
+    return old
+
+
+def save_tuning(tuning, path):
+    st = lib().vfb_tuning_save(C.byref(Tuning(**tuning)), str(path).encode())
+    if st != OK:
+        raise VfbError(f"save_tuning: {status_string(st)}")
+
+
+def load_tuning(path):
+    t = Tuning()
+    st = lib().vfb_tuning_load(str(path).encode(), C.byref(t))
+    if st != OK:
+        raise VfbError(f"load_tuning: {status_string(st)}")
+    return _tdict(t)
+
+
+def autotune(insts, max_size=MAX_SIZE, batched=True, reps=2, cache=None, relaxed_type=3):
+    """Runs the library's autotuner on a sample; the best setting stays active."""
+    n = len(insts)
+    o = _opts(max_size, relaxed_type, False, 0, False)
+    hs = (C.c_void_p * n)(*[i.handle.value for i in insts])
+    best = Tuning()
+    t0, t1 = C.c_double(), C.c_double()
+    st = lib().vfb_autotune(hs, n, C.byref(o), 1 if batched else 0, reps,
+                            str(cache).encode() if cache else None, C.byref(best), C.byref(t0), C.byref(t1))
+    if st != OK:
+        raise VfbError(f"autotune: {status_string(st)}")
+    return {"tuning": _tdict(best), "ms_default": t0.value, "ms_best": t1.value}
 
 
 def device_info():

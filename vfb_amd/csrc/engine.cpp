@@ -43,6 +43,7 @@ Engine::Engine(size_t arena_bytes) {
     CK(hipMemset(d_flag_, 0, sizeof(unsigned int)));
     CK(hipHostMalloc((void**)&h_flag_, sizeof(unsigned int), hipHostMallocDefault));
     *h_flag_ = 0;
+    tun_ = current_tuning();
 }
 
 Engine::~Engine() {
@@ -367,8 +368,46 @@ void Engine::build_units(Inst& I) {
         h.nrows[l] = I.nrows[l];
     }
     h.epoch_base = I.gbase;
-    h.ord_lo = 0;
-    h.ord_hi = I.N;
+    h.found_below = ~0ull;
+    h.mark = 1;
+}
+
+// Passes of a split level, in canonical order: 1 = ~ X F G (unary task),
+// 2 = & | (binary tasks), 3 = U (binary tasks).  After pass p every candidate
+// below pass_prefix(p) has been evaluated, so a match there is settled: no
+// later candidate can take its key or precede it.
+static uint64_t pass_prefix(const Inst& I, int pass) {
+    const auto& U = I.units[I.level + 1];
+    const size_t nsplit = (U.size() - 4) / 3;
+    if (pass == 1) return U[1].base;                  // end of ~
+    if (pass == 2) return U[2 * nsplit + 4].base;     // end of G
+    return I.N;
+}
+
+// whether unit q of the level belongs to pass p (0: fused, all units)
+static bool unit_in_pass(size_t q, size_t nsplit, int pass) {
+    if (pass == 0) return true;
+    const bool unary = q == 0 || (q >= 2 * nsplit + 1 && q <= 2 * nsplit + 3);
+    const bool andor = q >= 1 && q <= 2 * nsplit;
+    if (pass == 1) return unary;
+    if (pass == 2) return andor;
+    if (pass == 3) return !unary && !andor;
+    return false;
+}
+
+// candidates and algorithmic bytes (a*8T + 16 per candidate, no key slot in a
+// check-only level) of pass p
+static void pass_work(const Inst& I, int pass, uint64_t* cand, uint64_t* bytes) {
+    const auto& U = I.units[I.level + 1];
+    const size_t nsplit = (U.size() - 4) / 3;
+    *cand = *bytes = 0;
+    for (size_t q = 0; q < U.size(); ++q) {
+        if (!unit_in_pass(q, nsplit, pass)) continue;
+        const Unit& u = U[q];
+        const uint64_t a = (u.op == VFB_OP_AND || u.op == VFB_OP_OR) ? 2 : 1;
+        *cand += u.count;
+        *bytes += u.count * (a * 8 * (uint64_t)I.h.T + (I.check_only ? 0 : 16));
+    }
 }
 
 // scratch and hash set for a deduplicating level, sized from the previous
@@ -490,27 +529,41 @@ void Engine::epoch(std::vector<Inst*>& v) {
     const size_t n = v.size();
     // packed upload: descriptors | counters | tasks | fresh segments | total | chunk segments | ids
     // small epochs use several lanes per candidate (shorter serial trace loop)
+    // tasks of each instance's current pass
+    auto in_pass = [](const Inst& I, Task& t) {
+        switch (I.pass) {
+            case 0: t.part = PART_ALL; return true;
+            case 1: return t.kind == 0;
+            case 2: t.part = PART_ANDOR; return t.kind == 1;
+            case 3: t.part = PART_UNTIL; return t.kind == 1;
+            default: return false;
+        }
+    };
     uint64_t waves1 = 0;
     for (size_t k = 0; k < n; ++k)
-        for (const Task& t : v[k]->tasks)
-            waves1 += t.kind == 0 ? (t.nl + 63) / 64 : (uint64_t)t.nl * ((t.nr + 63) / 64);
-    const int wide = waves1 < kWideMaxWaves ? 1 : 0;
+        for (Task t : v[k]->tasks)
+            if (in_pass(*v[k], t)) waves1 += t.kind == 0 ? (t.nl + 63) / 64 : (uint64_t)t.nl * ((t.nr + 63) / 64);
+    const int wide = waves1 < tun_.wide_max_waves ? 1 : 0;
     const uint64_t per_wave = wide ? 64 / kWideLanes : 64;
     std::vector<Task> tasks;
     uint64_t waves = 0;
     for (size_t k = 0; k < n; ++k)
         for (Task t : v[k]->tasks) {
+            if (!in_pass(*v[k], t)) continue;
             t.inst = (uint32_t)k;
             t.wave_begin = waves;
             t.nchunk = (uint32_t)((t.nr + per_wave - 1) / per_wave);
             waves += t.kind == 0 ? (t.nl + per_wave - 1) / per_wave : (uint64_t)t.nl * t.nchunk;
             tasks.push_back(t);
         }
+    // the last pass of a level (0, 3 or 4) marks winners and scans; passes 1
+    // and 2 only report a match below their settled prefix
+    auto last_pass = [](const Inst& I) { return I.pass == 0 || I.pass >= 3; };
     std::vector<uint32_t> ids;
     std::vector<Seg> bsegs;
     uint64_t btot = 0;
     for (size_t k = 0; k < n; ++k) {
-        if (v[k]->check_only) continue;
+        if (v[k]->check_only || !last_pass(*v[k])) continue;
         ids.push_back((uint32_t)k);
         // small bitmaps are counted inside k_scan
         if (v[k]->h.nblk > (uint64_t)kSmallChunks) { bsegs.push_back(Seg{(uint32_t)k, 0, btot}); btot += v[k]->h.nblk; }
@@ -527,10 +580,17 @@ void Engine::epoch(std::vector<Inst*>& v) {
     InstDesc* hd = (InstDesc*)host.data();
     Ctr* hc = (Ctr*)(host.data() + o_ctr);
     for (size_t k = 0; k < n; ++k) {
-        v[k]->h.ctr = (Ctr*)(d + o_ctr) + k;
-        hd[k] = v[k]->h;
-        hc[k] = Ctr{};
-        hc[k].found = kEmpty;
+        Inst& I = *v[k];
+        I.h.ctr = (Ctr*)(d + o_ctr) + k;
+        I.h.mark = last_pass(I) ? 1 : 0;
+        I.h.found_below = last_pass(I) ? ~0ull : pass_prefix(I, I.pass);
+        hd[k] = I.h;
+        if (I.pass <= 1) {          // first pass of the level: fresh counters
+            hc[k] = Ctr{};
+            hc[k].found = kEmpty;
+        } else {                    // later passes continue the level's epoch
+            hc[k] = I.ctr;
+        }
     }
     if (!tasks.empty()) std::memcpy(host.data() + o_task, tasks.data(), tasks.size() * sizeof(Task));
     if (!bsegs.empty()) std::memcpy(host.data() + o_bseg, bsegs.data(), bsegs.size() * sizeof(Seg));
@@ -542,12 +602,14 @@ void Engine::epoch(std::vector<Inst*>& v) {
     hipEvent_t ev = nullptr;
     if (waves) {
         time_begin(&ev);
-        int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * 8);
+        int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * tun_.expand_blocks_per_cu);
         CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, grid, st_));
         time_end(ev, 0, v);
         for (auto* I : v) I->res->expand_launches++;
     }
-    if (!ids.empty()) {
+    bool any_dedup = false;
+    for (auto* I : v) any_dedup |= !I->check_only;
+    if (any_dedup) {
         std::vector<Inst*> dd;
         for (auto* I : v)
             if (!I->check_only) dd.push_back(I);
@@ -558,13 +620,13 @@ void Engine::epoch(std::vector<Inst*>& v) {
         if (!single) CK(launch_fresh_prefix(D, (Seg*)(d + o_fseg), (int)n, tot, st_));
         uint64_t fcap = 0;
         for (auto* I : dd) fcap += I->h.fresh_cap;
-        const int rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>((fcap + 255) / 256, (uint64_t)cus_ * 16));
+        const int rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>((fcap + 255) / 256, (uint64_t)cus_ * tun_.resolve_blocks_per_cu));
         CK(launch_resolve(D, (const Seg*)(d + o_fseg), (int)n, tot, single ? 1 : 0, rgrid, st_));
         if (btot) {
             int grid = (int)std::min<uint64_t>((btot + 3) / 4, (uint64_t)cus_ * 8);
             CK(launch_count(D, (const Seg*)(d + o_bseg), (int)bsegs.size(), btot, grid, st_));
         }
-        CK(launch_scan(D, (const uint32_t*)(d + o_ids), (int)ids.size(), st_));
+        if (!ids.empty()) CK(launch_scan(D, (const uint32_t*)(d + o_ids), (int)ids.size(), st_));
         time_end(ev, 1, dd);
     }
     std::vector<Ctr> back(n);
@@ -585,7 +647,7 @@ void Engine::store(std::vector<Inst*>& v) {
     }
     if (!tot) return;
     hipEvent_t ev = nullptr;
-    const int grid = (int)std::min<uint64_t>((tot + 3) / 4, (uint64_t)cus_ * 8);
+    const int grid = (int)std::min<uint64_t>((tot + 3) / 4, (uint64_t)cus_ * tun_.store_blocks_per_cu);
     if (n == 1 && epoch_single_ == v[0]) {
         // the epoch left this instance's descriptor at d_step_[0]: no upload
         const Inst& I = *v[0];
@@ -611,6 +673,7 @@ void Engine::store(std::vector<Inst*>& v) {
 void Engine::begin(const vfb_options& opt, bool batch) {
     opt_ = opt;
     batch_ = batch;
+    tun_ = current_tuning();
 }
 
 void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
@@ -697,37 +760,61 @@ void Engine::step(std::vector<Inst*>& act) {
             return;
         }
 
-        // ---- 2-3. epoch; re-run for instances whose set or list overflowed ----
+        // ---- 2-3. epochs: the passes of each level; a level whose set or
+        // list overflowed is grown and built again from its first pass ----
+        for (auto* I : step) {
+            const size_t nsplit = (I->units[I->level + 1].size() - 4) / 3;
+            I->passes = (tun_.pass_min && I->N >= tun_.pass_min && nsplit >= 1) ? 3 : 1;
+            I->pass = I->passes > 1 ? 1 : 0;
+            I->evaluated = 0;
+        }
         std::vector<Inst*> pend = step;
-        for (int attempt = 0; !pend.empty(); ++attempt) {
-            if (attempt > 12) throw DeviceError("epoch did not converge");
+        for (int round = 0; !pend.empty(); ++round) {
+            if (round > 64) throw DeviceError("epoch did not converge");
             pend.erase(std::remove_if(pend.begin(), pend.end(), [](Inst* I) { return I->done; }), pend.end());
             if (pend.empty()) break;
             epoch(pend);
             std::vector<Inst*> again;
             for (auto* Ip : pend) {
                 Inst& I = *Ip;
-                if (I.done || !I.ctr.overflow) continue;
-                I.res->retries++;
-                try {
-                    if (I.ctr.overflow & 2) {
-                        uint64_t fcap = std::min<uint64_t>(
-                            I.N, std::max<uint64_t>(I.fresh_cap * 4, I.ctr.fresh_count + 4096));
-                        ensure(I, I.fresh_ord, fcap * 8);
-                        ensure(I, I.fresh_slot, fcap * 4);
-                        I.fresh_cap = fcap;
-                        I.h.fresh_ord = (unsigned long long*)I.fresh_ord.p;
-                        I.h.fresh_slot = (uint32_t*)I.fresh_slot.p;
-                        I.h.fresh_cap = fcap;
+                if (I.done) continue;
+                if (I.ctr.overflow) {
+                    I.res->retries++;
+                    try {
+                        if (I.ctr.overflow & 2) {
+                            uint64_t fcap = std::min<uint64_t>(
+                                I.N, std::max<uint64_t>(I.fresh_cap * 4, I.ctr.fresh_count + 4096));
+                            ensure(I, I.fresh_ord, fcap * 8);
+                            ensure(I, I.fresh_slot, fcap * 4);
+                            I.fresh_cap = fcap;
+                            I.h.fresh_ord = (unsigned long long*)I.fresh_ord.p;
+                            I.h.fresh_slot = (uint32_t*)I.fresh_slot.p;
+                            I.h.fresh_cap = fcap;
+                        }
+                        uint64_t keys = (uint64_t)(I.cap * 0.7);
+                        ensure_table(I, (I.ctr.overflow & 5) ? keys * 4 : keys, true);
+                        fill(I.winbits.p, I.h.nwords, 0);
+                    } catch (CapacityError& e) {
+                        if (on_capacity(I, e)) continue;
+                        I.check_only = true;   // cannot grow: check the level only
+                        build_units(I);
                     }
-                    uint64_t keys = (uint64_t)(I.cap * 0.7);
-                    ensure_table(I, (I.ctr.overflow & 5) ? keys * 4 : keys, true);
-                    fill(I.winbits.p, I.h.nwords, 0);
+                    I.pass = I.passes > 1 ? 1 : 0;
+                    I.evaluated = 0;
                     again.push_back(&I);
-                } catch (CapacityError& e) {
-                    if (on_capacity(I, e)) continue;
-                    I.check_only = true;   // cannot grow: check the level only
-                    build_units(I);
+                    continue;
+                }
+                uint64_t c = 0, b = 0;
+                pass_work(I, I.pass, &c, &b);
+                I.evaluated += c;
+                I.res->bytes_expand += b;
+                if (I.pass == 1 || I.pass == 2) {
+                    // a check-only level reports every match (the lowest so
+                    // far); it is settled once below the evaluated prefix
+                    const bool hit = I.ctr.found != kEmpty &&
+                                     (!I.check_only || I.ctr.found < pass_prefix(I, I.pass));
+                    if (hit && I.check_only) continue;
+                    I.pass = hit ? 4 : I.pass + 1;
                     again.push_back(&I);
                 }
             }
@@ -741,7 +828,7 @@ void Engine::step(std::vector<Inst*>& act) {
             if (I.done) continue;
             vfb_result& R = *I.res;
             const int s = I.level + 1;
-            R.level_candidates[s] = I.N;
+            R.level_candidates[s] = I.evaluated;
             if (opt_.verbose)
                 std::fprintf(stderr, "Cost %-2d | %s | candidates: %-14llu | fresh: %-12llu | distinct: %-12llu | table: %llu\n",
                              s, I.check_only ? "check" : "dedup", (unsigned long long)I.N,
@@ -764,10 +851,6 @@ void Engine::step(std::vector<Inst*>& act) {
             }
             R.level_winners[s] = I.ctr.winners;
             R.fresh_total += I.ctr.fresh_count;
-            for (const Unit& u : I.units[s]) {
-                const uint64_t a = (u.op == VFB_OP_AND || u.op == VFB_OP_OR) ? 2 : 1;
-                R.bytes_expand += u.count * (a * 8 * (uint64_t)I.h.T + 16);
-            }
             if (I.ctr.found != kEmpty) {
                 R.candidates[s] = I.ctr.found + 1;
                 R.distinct[s] = I.ctr.found_rank;
@@ -840,6 +923,15 @@ void Engine::step(std::vector<Inst*>& act) {
 
 static std::mutex g_mu;
 static std::unique_ptr<Engine> g_engine;
+
+// process-wide tuning; read by engines at begin(), under g_mu
+static vfb_tuning g_tuning = [] {
+    vfb_tuning t;
+    vfb_tuning_default(&t);
+    return t;
+}();
+
+vfb_tuning current_tuning() { return g_tuning; }
 
 std::mutex& engine_mutex() { return g_mu; }
 
@@ -942,6 +1034,31 @@ extern "C" vfb_status vfb_solve_batch(const vfb_instance* const* insts, size_t n
     if (!insts || !out) return VFB_INVALID_INPUT;
     if (n == 0) return VFB_OK;
     return solve_many(insts, n, opt, out);
+}
+
+extern "C" void vfb_tuning_default(vfb_tuning* t) {
+    if (!t) return;
+    t->pass_min = 1ull << 22;
+    t->wide_max_waves = kWideMaxWaves;
+    t->expand_blocks_per_cu = 8;
+    t->resolve_blocks_per_cu = 16;
+    t->store_blocks_per_cu = 8;
+}
+
+extern "C" void vfb_get_tuning(vfb_tuning* t) {
+    if (!t) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    *t = g_tuning;
+}
+
+extern "C" vfb_status vfb_set_tuning(const vfb_tuning* t) {
+    if (!t) return VFB_INVALID_INPUT;
+    auto ok = [](uint32_t b) { return b >= 1 && b <= 64; };
+    if (!ok(t->expand_blocks_per_cu) || !ok(t->resolve_blocks_per_cu) || !ok(t->store_blocks_per_cu))
+        return VFB_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tuning = *t;
+    return VFB_OK;
 }
 
 extern "C" vfb_status vfb_device_info(char* name, size_t cap, int* cus) {

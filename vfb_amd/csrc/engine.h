@@ -141,6 +141,11 @@ struct Inst {
     uint64_t N = 0;             // candidates of the level being built
     std::vector<Task> tasks;
     Ctr ctr{};
+    // A large level is built in three passes in canonical order -- ~ X F G,
+    // then & |, then U -- and stops after the pass that settles a match.
+    int passes = 1;             // 1: one fused pass, 3: split
+    int pass = 0;               // next pass: 0 fused, 1..3 split, 4 winner marking only
+    uint64_t evaluated = 0;     // candidates of this level evaluated so far
 };
 
 class Engine {
@@ -181,6 +186,7 @@ class Engine {
     std::vector<Timed> timed_;
     std::vector<Inst*> live_;     // started, not finished: eviction candidates
     Inst* epoch_single_ = nullptr; // the one instance whose descriptor is at d_step_[0]
+    vfb_tuning tun_{};            // launch / pass parameters, copied at begin()
 
     void sync();
     void h2d(void* dst, const void* src, size_t bytes);
@@ -211,6 +217,7 @@ class Engine {
 // and the mutex that serialises its use.  engine_acquire() creates the engine,
 // or re-creates it with a larger arena when want_bytes exceeds the current one.
 std::mutex& engine_mutex();
+vfb_tuning current_tuning();   // caller holds engine_mutex()
 Engine& engine_acquire(size_t want_bytes);
 void engine_drop();
 bool engine_pinned();   // true while a server holds requests in the engine

@@ -67,6 +67,8 @@ struct InstDesc {
     int level;        // level being built
     int nunits;
     int check_only;   // onTheFly level: target check only, no keys, no storage
+    int mark;         // resolve: 1 sets the winner bitmap (last pass), 0 only reports a match
+    int pad0;
     uint64_t lenmask[kMaxT];
     uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
     int16_t koff[kMaxT];        // first key bit of the trace's field / samples
@@ -77,7 +79,7 @@ struct InstDesc {
     uint64_t table_mask;
     uint64_t claim_limit;       // claims allowed this epoch before the set must grow
     uint64_t epoch_base;        // global ordinal of the level's first candidate
-    uint64_t ord_lo, ord_hi;    // level-local ordinal range of this epoch
+    uint64_t found_below;       // resolve: only matches below this level-local ordinal are reported
     unsigned long long* fresh_ord;
     uint32_t* fresh_slot;
     uint64_t fresh_cap;
@@ -95,10 +97,12 @@ struct InstDesc {
 
 // Fused work item: all operators that read the same operand rows.
 //   kind 0: rows of level lvl_l -> ~, X, F, G     (base[0..3])
-//   kind 1: pairs (level lvl_l x level lvl_r) -> &, |, lUr, rUl  (base[0..2])
+//   kind 1: pairs (level lvl_l x level lvl_r) -> &, |, lUr, rUl  (base[0..2]);
+//           part 0 all four, part 1 & and | only, part 2 lUr and rUl only
+enum { PART_ALL = 0, PART_ANDOR = 1, PART_UNTIL = 2 };
 struct Task {
     uint32_t inst;
-    uint8_t kind, lvl_l, lvl_r, pad;
+    uint8_t kind, lvl_l, lvl_r, part;
     uint32_t nl, nr;
     uint32_t nchunk;            // ceil(nr / candidates per wavefront)
     uint32_t pad1;

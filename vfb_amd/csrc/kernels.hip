@@ -291,16 +291,6 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
     }
 }
 
-__device__ __forceinline__ uint32_t epoch_enable(const InstDesc& d, const uint64_t* ord, bool active) {
-    uint32_t en = 0;
-    if (active) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (ord[q] >= d.ord_lo && ord[q] < d.ord_hi) en |= 1u << q;
-    }
-    return en;
-}
-
 // onTheFly last round (processOnTheFly, LTL_BS.cu:331-349): no key, no
 // storage; the lowest matching ordinal wins
 __device__ __forceinline__ void check_candidates(const InstDesc& d, const uint64_t* ord, uint32_t mismatch,
@@ -437,11 +427,12 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
         }
     }
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, epoch_enable(d, ord, active), lane);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? 0xfu : 0u, lane);
 }
 
-// &, |, l U r, r U l of one pair (left row uniform per wavefront)
-template <int KEY, bool CHECK, int G>
+// &, |, l U r, r U l of one pair (left row uniform per wavefront); PART
+// selects all four, & and | only, or the two U orientations only
+template <int KEY, bool CHECK, int G, int PART>
 __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
     constexpr int W = 64 / G;
     const int sub = lane % G;
@@ -470,18 +461,25 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
             const uint64_t a = av[j], b = bv[j];
             const uint64_t want = t < d.P ? 1ull : 0ull;
             if constexpr (CHECK) {
-                mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
+                if constexpr (PART == PART_ALL)
+                    mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
+                else if constexpr (PART == PART_ANDOR)
+                    mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, 0ull, 0ull, want);
+                else
+                    mm |= bit0_mismatch(0ull, 0ull, until_bit0(a, b), until_bit0(b, a), want);
                 continue;
             }
-            const uint64_t c0 = a & b;
-            const uint64_t c1 = a | b;
-            const uint64_t c2 = until_op(a, b);
-            const uint64_t c3 = until_op(b, a);
-            mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
-                  ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
-            if constexpr (!CHECK) {
+            if constexpr (PART != PART_UNTIL) {
+                const uint64_t c0 = a & b;
+                const uint64_t c1 = a | b;
+                mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1);
                 key_acc<KEY>(k[0], c0, d, t);
                 key_acc<KEY>(k[1], c1, d, t);
+            }
+            if constexpr (PART != PART_ANDOR) {
+                const uint64_t c2 = until_op(a, b);
+                const uint64_t c3 = until_op(b, a);
+                mm |= ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
                 key_acc<KEY>(k[2], c2, d, t);
                 key_acc<KEY>(k[3], c3, d, t);
             }
@@ -490,7 +488,15 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     const uint64_t pair = l * tk.nr + r;
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, epoch_enable(d, ord, active), lane);
+    constexpr uint32_t sel = PART == PART_ALL ? 0xfu : (PART == PART_ANDOR ? 0x3u : 0xcu);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? sel : 0u, lane);
+}
+
+template <int KEY, bool CHECK, int G>
+__device__ __forceinline__ void expand_binary_part(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    if (tk.part == PART_ANDOR) expand_binary<KEY, CHECK, G, PART_ANDOR>(d, tk, wl, lane);
+    else if (tk.part == PART_UNTIL) expand_binary<KEY, CHECK, G, PART_UNTIL>(d, tk, wl, lane);
+    else expand_binary<KEY, CHECK, G, PART_ALL>(d, tk, wl, lane);
 }
 
 // G = lanes per candidate: 1 for large levels (throughput), kWideLanes for
@@ -513,15 +519,15 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
         const uint64_t wl = w - tk.wave_begin;
         if (d.check_only) {
             if (tk.kind == 0) expand_unary<KEY_MIX, true, G>(d, tk, wl, lane);
-            else expand_binary<KEY_MIX, true, G>(d, tk, wl, lane);
+            else expand_binary_part<KEY_MIX, true, G>(d, tk, wl, lane);
         } else if (tk.kind == 0) {
             if (d.key_kind == KEY_MIX) expand_unary<KEY_MIX, false, G>(d, tk, wl, lane);
             else if (d.key_kind == KEY_FIELD) expand_unary<KEY_FIELD, false, G>(d, tk, wl, lane);
             else expand_unary<KEY_STRIDE, false, G>(d, tk, wl, lane);
         } else {
-            if (d.key_kind == KEY_MIX) expand_binary<KEY_MIX, false, G>(d, tk, wl, lane);
-            else if (d.key_kind == KEY_FIELD) expand_binary<KEY_FIELD, false, G>(d, tk, wl, lane);
-            else expand_binary<KEY_STRIDE, false, G>(d, tk, wl, lane);
+            if (d.key_kind == KEY_MIX) expand_binary_part<KEY_MIX, false, G>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_binary_part<KEY_FIELD, false, G>(d, tk, wl, lane);
+            else expand_binary_part<KEY_STRIDE, false, G>(d, tk, wl, lane);
         }
     }
 }
@@ -608,12 +614,14 @@ __global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D,
         const InstDesc& d = D[sg.inst];
         const uint64_t e = g - sg.begin;
         const uint64_t of = d.fresh_ord[e];
+        // a pass before the last only looks for a settled match
+        if (!d.mark && !(of & kMatchBit)) continue;
         const uint32_t slot = d.fresh_slot[e];
         const uint64_t o = of & ~kMatchBit;
         if (d.table[slot].ord == o) {
             const uint64_t local = o - d.epoch_base;
-            atomicOr(&d.winbits[local >> 6], 1ull << (local & 63));
-            if (of & kMatchBit) atomicMin(&d.ctr->found, (unsigned long long)local);
+            if (d.mark) atomicOr(&d.winbits[local >> 6], 1ull << (local & 63));
+            if ((of & kMatchBit) && local < d.found_below) atomicMin(&d.ctr->found, (unsigned long long)local);
         }
     }
 }
