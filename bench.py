@@ -37,24 +37,43 @@ def pct(xs, q):
     return xs[f] + (xs[c] - xs[f]) * (k - f)
 
 
-def cpu_baseline(insts, max_size, budget_s):
-    """Oracle (single thread) on a bounded sample of the same workload."""
+def profiled_traffic():
+    """HBM bytes per k_expand launch of this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the copy-kernel
+    calibration (profiles/k_expand_config2_main.md); None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_expand_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def cpu_baseline(insts, max_size, budget_s, gpu_results=None):
+    """Multi-core CPU build of the same enumeration (oracle/cpu_omp.c,
+    OpenMP over this host's cores) on the same instances, timed around each
+    solve alone; stops after budget_s seconds of CPU work.  Its results are
+    compared with the GPU's."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    done, distinct, t_total = 0, 0, 0.0
-    for inst in insts:
+    done, distinct, t_total, same = 0, 0, 0.0, True
+    for k, inst in enumerate(insts):
         names, lens, atoms, P = inst.arrays()
-        t0 = time.perf_counter()
-        r = O.enumerate_cs(O.Instance(names, lens, atoms, P), max_size, 3)
-        t_total += time.perf_counter() - t0
+        r, dt = O.enumerate_omp(O.Instance(names, lens, atoms, P), max_size, 3)
+        t_total += dt
         distinct += sum(r["distinct"])
         done += 1
+        if gpu_results is not None:
+            g = gpu_results[k]
+            same &= all(r[f] == g[f] for f in ("status", "min_size", "formula", "distinct", "candidates"))
         if t_total > budget_s:
             break
-    return {"value": distinct / t_total if t_total else None, "unit": "distinct CS/s", "cores": 1,
-            "kind": "port",
+    return {"value": distinct / t_total if t_total else None, "unit": "distinct CS/s",
+            "cores": O.omp_threads(), "kind": "port",
             "sample": f"config-2 instances 0..{done - 1} at max_formula_size {max_size} "
-                      f"({distinct} distinct CS in {t_total:.1f} s)"}
+                      f"({distinct} distinct CS in {t_total:.2f} s, OpenMP build of the same "
+                      f"enumeration, {O.omp_threads()} threads)",
+            "bit_exact_vs_gpu": same}
 
 
 def batched_mix(V, n, max_size):
@@ -168,7 +187,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-active", type=int, default=128)
     ap.add_argument("--limits", default="4,10", help="config-3 instances to time (empty: skip)")
-    ap.add_argument("--only", choices=["limits", "mix"], default=None,
+    ap.add_argument("--only", choices=["main", "limits", "mix"], default=None,
                     help="run one section alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -221,6 +240,9 @@ def main():
     for r in plain:
         statuses[r["status"]] = statuses.get(r["status"], 0) + 1
 
+    if args.only == "main":
+        print(json.dumps({"distinct_per_s": distinct / T, "ms_per_step": 1e3 * T / args.steps}))
+        return
     # batched: the same instances stepped together; must equal the plain runs
     V.solve_batch(insts, max_size=args.max_size)  # warm
     t0 = time.perf_counter()
@@ -259,7 +281,7 @@ def main():
         "statuses": {V.status_string(k): v for k, v in statuses.items()},
         "roofline": {"bound": "hbm", "kernel": "k_expand", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": None,
+                     "traffic": profiled_traffic(),
                      "launches": launches, "ms_per_launch": ms_exp / launches if launches else None,
                      "alg_bytes_per_launch": bytes_exp / launches if launches else None},
         "device": dev,
@@ -275,7 +297,8 @@ def main():
         out["served"] = served(V, args.serve_rate, args.serve_seconds, 16, args.serve_active, 256)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(insts, args.cpu_max_size, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(insts, args.cpu_max_size, args.cpu_seconds,
+                                           plain if args.cpu_max_size == args.max_size else None)
     print(json.dumps(out))
 
 

@@ -72,6 +72,15 @@ int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
                   uint64_t max_rows, const char* const* names, ora_result* out);
 int ora_bruteforce(const ora_instance* in, int max_size,
                    const char* const* names, ora_result* out);
+/* multi-core CPU build of ora_enumerate (cpu_omp.c, OpenMP): same results
+   (status, min_size, formula, distinct, candidates, found_ordinal) for
+   key modes 1-3 and no row cap; levels with at least pass_min candidates
+   (0: none) are built in three passes that stop at a settled match.  It is
+   the CPU baseline of the benchmark, pinned against ora_enumerate. */
+int ora_enumerate_omp(const ora_instance* in, int max_size, int key_mode, uint64_t pass_min,
+                      const char* const* names, ora_result* out);
+int ora_omp_threads(void);
+
 /* closed-form count of syntactic formulas of each size, ltl_nocache.cu:248-287 */
 void ora_num_formulas(int A, int max_size, uint64_t* out /* [(max_size+1)*8] */);
 

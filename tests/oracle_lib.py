@@ -31,7 +31,24 @@ class OraResult(C.Structure):
                 ("formula", C.c_char * 8192)]
 
 
+LIB_OMP = os.path.join(ROOT, "oracle", "_build", "libcpu_omp.so")
+
 _lib = None
+_omp = None
+
+
+def omp_lib():
+    """The multi-core CPU build of the enumeration (oracle/cpu_omp.c)."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(LIB_OMP):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _omp = C.CDLL(LIB_OMP)
+        _omp.ora_enumerate_omp.restype = C.c_int
+        _omp.ora_enumerate_omp.argtypes = [C.POINTER(OraInstance), C.c_int, C.c_int, C.c_uint64,
+                                           C.POINTER(C.c_char_p), C.POINTER(OraResult)]
+        _omp.ora_omp_threads.restype = C.c_int
+    return _omp
 
 
 def lib():
@@ -121,6 +138,21 @@ def enumerate_cs(inst, max_size, key_mode=3, max_rows=0):
     r = OraResult()
     lib().ora_enumerate(C.byref(ci), max_size, key_mode, max_rows, names, C.byref(r))
     return _result(r)
+
+
+def enumerate_omp(inst, max_size, key_mode=3, pass_min=1 << 22):
+    """Multi-core build; returns (result, seconds of the solve alone)."""
+    import time
+    ci, keep, names = inst._c()
+    r = OraResult()
+    L = omp_lib()
+    t0 = time.perf_counter()
+    L.ora_enumerate_omp(C.byref(ci), max_size, key_mode, pass_min, names, C.byref(r))
+    return _result(r), time.perf_counter() - t0
+
+
+def omp_threads():
+    return omp_lib().ora_omp_threads()
 
 
 def bruteforce(inst, max_size):

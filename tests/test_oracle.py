@@ -246,3 +246,33 @@ def test_onthefly_semantics():
     n = [0] + full["distinct"][1:4] + [7]
     want = 4 * n[4] + 4 * sum(n[i] * n[4 - i] for i in range(1, 5) if 2 * i <= 4)
     assert r["candidates"][5] == want
+
+
+def _gen_instance(seed, A, T, lmax, p=0.5):
+    rng = random.Random(seed)
+    lens = [rng.randint(1, lmax) for _ in range(T)]
+    atoms = [[sum((rng.random() < p) << k for k in range(lens[t])) for t in range(T)] for _ in range(A)]
+    return O.Instance([f"p{a}" for a in range(A)], lens, atoms, rng.randint(1, T - 1))
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_omp_build_equals_sequential_oracle(seed):
+    """The multi-core CPU baseline (cpu_omp.c) gives the sequential oracle's
+    results, with and without the three-pass split of large levels."""
+    rng = random.Random(9000 + seed)
+    inst = _gen_instance(seed, rng.randint(1, 4), rng.randint(2, 20), 64 if seed % 3 == 0 else 10)
+    key_mode = 1 + seed % 3
+    want = O.enumerate_cs(inst, 7, key_mode)
+    for pass_min in (0, 1, 1 << 10):
+        got, _ = O.enumerate_omp(inst, 7, key_mode, pass_min)
+        for f in ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "top_op"):
+            assert got[f] == want[f], (seed, pass_min, f, got[f], want[f])
+
+
+def test_omp_build_on_example_and_exact_keys():
+    inst = O.Instance.from_json(os.path.join(GOLDEN, "ExampleLTLF.json"))
+    want = O.enumerate_cs(inst, 10, 3)
+    got, _ = O.enumerate_omp(inst, 10, 3, 1)
+    assert (got["formula"], got["distinct"], got["candidates"]) == (want["formula"], want["distinct"],
+                                                                     want["candidates"])
+    assert O.omp_threads() >= 1
