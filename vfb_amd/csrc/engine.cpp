@@ -193,6 +193,7 @@ void Engine::release_all(Inst& I) {
     dfree(I, I.fresh_ord);
     dfree(I, I.fresh_slot);
     dfree(I, I.winbits);
+    dfree(I, I.winbyte);
     dfree(I, I.blkcount);
     dfree(I, I.blkprefix);
     I.fresh_cap = I.nwords_cap = I.nblk_cap = 0;
@@ -426,6 +427,7 @@ void Engine::prepare_scratch(Inst& I) {
     }
     if (nwords > I.nwords_cap) {
         ensure(I, I.winbits, std::max<uint64_t>(nwords, 1) * 8);
+        ensure(I, I.winbyte, std::max<uint64_t>(nwords, 1) * 64);
         I.nwords_cap = nwords;
     }
     if (nblk > I.nblk_cap) {
@@ -437,6 +439,7 @@ void Engine::prepare_scratch(Inst& I) {
     h.fresh_slot = (uint32_t*)I.fresh_slot.p;
     h.fresh_cap = I.fresh_cap;
     h.winbits = (unsigned long long*)I.winbits.p;
+    h.winbyte = (unsigned char*)I.winbyte.p;
     h.nwords = nwords;
     h.blkcount = (uint32_t*)I.blkcount.p;
     h.blkprefix = (unsigned long long*)I.blkprefix.p;
@@ -444,7 +447,7 @@ void Engine::prepare_scratch(Inst& I) {
     const uint64_t est_new =
         std::min<uint64_t>(N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.win_ratio * 1.3))));
     ensure_table(I, I.committed + est_new, false);
-    fill(I.winbits.p, nwords, 0);
+    fill(I.winbyte.p, nwords * 8, 0);
 }
 
 // (re)builds the hash set for want_keys keys at load <= 0.7; keeps only keys
@@ -793,7 +796,7 @@ void Engine::step(std::vector<Inst*>& act) {
                         }
                         uint64_t keys = (uint64_t)(I.cap * 0.7);
                         ensure_table(I, (I.ctr.overflow & 5) ? keys * 4 : keys, true);
-                        fill(I.winbits.p, I.h.nwords, 0);
+                        fill(I.winbyte.p, I.h.nwords * 8, 0);
                     } catch (CapacityError& e) {
                         if (on_capacity(I, e)) continue;
                         I.check_only = true;   // cannot grow: check the level only

@@ -135,7 +135,7 @@ struct Inst {
     std::vector<std::vector<Unit>> units;
     Buf table;
     uint64_t cap = 0, committed = 0;
-    Buf fresh_ord, fresh_slot, winbits, blkcount, blkprefix;
+    Buf fresh_ord, fresh_slot, winbyte, winbits, blkcount, blkprefix;
     uint64_t fresh_cap = 0, nwords_cap = 0, nblk_cap = 0;
     double win_ratio = 1.0, fresh_ratio = 1.0;
     uint64_t N = 0;             // candidates of the level being built

@@ -83,7 +83,8 @@ struct InstDesc {
     unsigned long long* fresh_ord;
     uint32_t* fresh_slot;
     uint64_t fresh_cap;
-    unsigned long long* winbits;
+    unsigned char* winbyte;      // one byte per candidate, set by the resolve (plain stores)
+    unsigned long long* winbits; // the same, packed to bits by the count / scan kernels
     uint64_t nwords;
     uint32_t* blkcount;
     unsigned long long* blkprefix;

@@ -555,15 +555,26 @@ __global__ void k_insert_atoms(const InstDesc* __restrict__ D, uint32_t inst, ui
 }
 
 // Moves the keys of completed epochs (ord < keep_below) into a fresh table.
+// The keys are distinct, so a key takes the first slot whose hi it claims
+// from EMPTY, alone (one CAS); lo and ord are plain stores, read only by
+// later kernels.  Lookups stay valid: they skip slots whose (hi, lo) differ.
 __global__ void k_rehash(const Slot* __restrict__ old, uint64_t old_cap, Slot* tab, uint64_t mask,
                          uint64_t keep_below, unsigned int* fail) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < old_cap; i += stride) {
         const Slot s = old[i];
         if (s.hi == kEmpty || s.lo == kEmpty || s.ord >= keep_below) continue;
-        uint32_t slot;
-        int claimed = 0;
-        if (probe_insert(tab, mask, s.hi, s.lo, s.ord, 0, &slot, &claimed) == PR_FAIL) atomicOr(fail, 1u);
+        uint64_t j = slot_hash(s.hi, s.lo) & mask;
+        int n = 0;
+        for (; n < kProbeLimit; ++n, j = (j + 1) & mask) {
+            if (ld_any(&tab[j].hi) != kEmpty) continue;
+            if (atomicCAS(&tab[j].hi, kEmpty, s.hi) == kEmpty) {
+                tab[j].lo = s.lo;
+                tab[j].ord = s.ord;
+                break;
+            }
+        }
+        if (n == kProbeLimit) atomicOr(fail, 1u);
     }
 }
 
@@ -620,14 +631,40 @@ __global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D,
         const uint64_t o = of & ~kMatchBit;
         if (d.table[slot].ord == o) {
             const uint64_t local = o - d.epoch_base;
-            if (d.mark) atomicOr(&d.winbits[local >> 6], 1ull << (local & 63));
+            // each ordinal appears at most once in the fresh list: a plain
+            // byte store, packed to bits by the count / scan kernels
+            if (d.mark) d.winbyte[local] = 1;
             if ((of & kMatchBit) && local < d.found_below) atomicMin(&d.ctr->found, (unsigned long long)local);
         }
     }
 }
 
-// popcount of each chunk (64 bitmap words = 4096 ordinals) of every
-// instance's winner bitmap; one wavefront per chunk
+// bit k of the result = byte k of x, for bytes that are 0 or 1
+__device__ __forceinline__ uint64_t gather8(uint64_t x) {
+    x &= 0x0101010101010101ull;
+    x = (x | (x >> 7)) & 0x0003000300030003ull;
+    x = (x | (x >> 14)) & 0x0000000F0000000Full;
+    return (x | (x >> 28)) & 0xFFull;
+}
+
+// packs the 64 winner bytes of bitmap word w into the word; returns its popcount
+__device__ __forceinline__ uint32_t pack_word(const InstDesc& d, uint64_t w) {
+    if (w >= d.nwords) return 0u;
+    const uint4* src = reinterpret_cast<const uint4*>(d.winbyte + w * 64);
+    uint64_t bits = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = src[q];
+        const uint64_t a = ((uint64_t)v.y << 32) | v.x, b = ((uint64_t)v.w << 32) | v.z;
+        bits |= gather8(a) << (16 * q);
+        bits |= gather8(b) << (16 * q + 8);
+    }
+    d.winbits[w] = bits;
+    return (uint32_t)__popcll(bits);
+}
+
+// packs and counts each chunk (64 bitmap words = 4096 ordinals) of every
+// instance's winner map; one wavefront per chunk
 __global__ __launch_bounds__(256) void k_count(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
                                                int nseg, uint64_t total_chunks) {
     const int lane = threadIdx.x & 63;
@@ -638,8 +675,7 @@ __global__ __launch_bounds__(256) void k_count(const InstDesc* __restrict__ D, c
         const Seg& sg = segs[seg_find(segs, nseg, g)];
         const InstDesc& d = D[sg.inst];
         const uint64_t c = g - sg.begin;
-        const uint64_t w = c * kChunkWords + lane;
-        uint32_t v = w < d.nwords ? (uint32_t)__popcll(d.winbits[w]) : 0u;
+        uint32_t v = pack_word(d, c * kChunkWords + lane);
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (lane == 0) d.blkcount[c] = v;
     }
@@ -665,8 +701,7 @@ __global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, co
     if (d.nblk <= (uint64_t)kSmallChunks) {
         // small bitmap: count the chunks here (one wavefront per chunk)
         for (uint64_t c = w; c < d.nblk; c += 4) {
-            const uint64_t word = c * kChunkWords + lane;
-            uint32_t v = word < d.nwords ? (uint32_t)__popcll(d.winbits[word]) : 0u;
+            uint32_t v = pack_word(d, c * kChunkWords + lane);
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
             if (lane == 0) d.blkcount[c] = v;
         }
