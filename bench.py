@@ -212,7 +212,8 @@ def main():
     def step():
         res = []
         for inst in insts:
-            res.append(V.solve(inst, max_size=args.max_size, timing=True))
+            # timing=2: HIP events around the expand kernel only (the roofline's)
+            res.append(V.solve(inst, max_size=args.max_size, timing=2))
         return res
 
     for w in range(args.warmup):

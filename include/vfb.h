@@ -82,7 +82,8 @@ typedef struct {
     int max_size;          /* 1..32 (LTLI maxCost, LTL_BS.cu:1071) */
     int relaxed_type;      /* key when total trace length > 126: 1, 2 or 3 (LTL_BS.cu:39-92) */
     int verbose;           /* 1: per-level progress lines on stderr */
-    int timing;            /* 1: time each kernel class with HIP events */
+    int timing;            /* 1: time each kernel class with HIP events; 2: only the
+                              combine+key+dedup kernel (ms_expand; two events per level) */
     uint64_t hbm_budget;   /* bytes of device memory per solve; 0 = the process arena
                               (85% of the HBM free at first use) */
     uint64_t max_rows;     /* cap on stored CS rows, the reference's LTLCacheCapacity

@@ -212,7 +212,8 @@ def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0):
     lib().vfb_options_default(C.byref(o))
     o.max_size = max_size
     o.relaxed_type = relaxed_type
-    o.timing = 1 if timing else 0
+    # True / 1: every kernel class; 2: the expand kernel only (fewer events)
+    o.timing = int(timing) if not isinstance(timing, bool) else (1 if timing else 0)
     o.verbose = 1 if verbose else 0
     o.hbm_budget = int(budget)
     o.max_rows = int(max_rows)

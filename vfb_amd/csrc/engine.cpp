@@ -108,8 +108,9 @@ char* Engine::step_buf(size_t bytes) {
     return d_step_;
 }
 
-void Engine::time_begin(hipEvent_t* a) {
-    if (!opt_.timing) return;
+void Engine::time_begin(hipEvent_t* a, int kind) {
+    *a = nullptr;
+    if (!opt_.timing || (opt_.timing == 2 && kind != 0)) return;
     if (ev_pool_.empty()) {
         hipEvent_t e;
         CK(hipEventCreate(&e));
@@ -121,7 +122,7 @@ void Engine::time_begin(hipEvent_t* a) {
 }
 
 void Engine::time_end(hipEvent_t a, int kind, const std::vector<Inst*>& who) {
-    if (!opt_.timing) return;
+    if (!a) return;
     hipEvent_t b;
     if (ev_pool_.empty()) CK(hipEventCreate(&b));
     else { b = ev_pool_.back(); ev_pool_.pop_back(); }
@@ -428,6 +429,8 @@ void Engine::prepare_scratch(Inst& I) {
     if (nwords > I.nwords_cap) {
         ensure(I, I.winbits, std::max<uint64_t>(nwords, 1) * 8);
         ensure(I, I.winbyte, std::max<uint64_t>(nwords, 1) * 64);
+        // cleared once here; the count / scan kernels clear what they pack
+        fill(I.winbyte.p, std::max<uint64_t>(nwords, 1) * 8, 0);
         I.nwords_cap = nwords;
     }
     if (nblk > I.nblk_cap) {
@@ -447,7 +450,6 @@ void Engine::prepare_scratch(Inst& I) {
     const uint64_t est_new =
         std::min<uint64_t>(N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.win_ratio * 1.3))));
     ensure_table(I, I.committed + est_new, false);
-    fill(I.winbyte.p, nwords * 8, 0);
 }
 
 // (re)builds the hash set for want_keys keys at load <= 0.7; keeps only keys
@@ -604,7 +606,7 @@ void Engine::epoch(std::vector<Inst*>& v) {
 
     hipEvent_t ev = nullptr;
     if (waves) {
-        time_begin(&ev);
+        time_begin(&ev, 0);
         int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * tun_.expand_blocks_per_cu);
         CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, grid, st_));
         time_end(ev, 0, v);
@@ -616,20 +618,28 @@ void Engine::epoch(std::vector<Inst*>& v) {
         std::vector<Inst*> dd;
         for (auto* I : v)
             if (!I->check_only) dd.push_back(I);
-        time_begin(&ev);
+        time_begin(&ev, 1);
         unsigned long long* tot = (unsigned long long*)(d + o_tot);
         // one instance: k_resolve reads its fresh count itself (segment 0 at 0)
         const bool single = n == 1;
-        if (!single) CK(launch_fresh_prefix(D, (Seg*)(d + o_fseg), (int)n, tot, st_));
-        uint64_t fcap = 0;
-        for (auto* I : dd) fcap += I->h.fresh_cap;
-        const int rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>((fcap + 255) / 256, (uint64_t)cus_ * tun_.resolve_blocks_per_cu));
-        CK(launch_resolve(D, (const Seg*)(d + o_fseg), (int)n, tot, single ? 1 : 0, rgrid, st_));
-        if (btot) {
-            int grid = (int)std::min<uint64_t>((btot + 3) / 4, (uint64_t)cus_ * 8);
-            CK(launch_count(D, (const Seg*)(d + o_bseg), (int)bsegs.size(), btot, grid, st_));
+        // one small instance at its last pass: resolve and scan in one workgroup
+        const Inst& I0 = *v[0];
+        if (single && !I0.check_only && last_pass(I0) && I0.h.nblk <= (uint64_t)kSmallChunks &&
+            I0.h.fresh_cap <= kSmallFresh) {
+            CK(launch_resolve_scan1(D, st_));
+        } else {
+            if (!single) CK(launch_fresh_prefix(D, (Seg*)(d + o_fseg), (int)n, tot, st_));
+            uint64_t fcap = 0;
+            for (auto* I : dd) fcap += I->h.fresh_cap;
+            const int rgrid = (int)std::max<uint64_t>(
+                1, std::min<uint64_t>((fcap + 255) / 256, (uint64_t)cus_ * tun_.resolve_blocks_per_cu));
+            CK(launch_resolve(D, (const Seg*)(d + o_fseg), (int)n, tot, single ? 1 : 0, rgrid, st_));
+            if (btot) {
+                int grid = (int)std::min<uint64_t>((btot + 3) / 4, (uint64_t)cus_ * 8);
+                CK(launch_count(D, (const Seg*)(d + o_bseg), (int)bsegs.size(), btot, grid, st_));
+            }
+            if (!ids.empty()) CK(launch_scan(D, (const uint32_t*)(d + o_ids), (int)ids.size(), st_));
         }
-        if (!ids.empty()) CK(launch_scan(D, (const uint32_t*)(d + o_ids), (int)ids.size(), st_));
         time_end(ev, 1, dd);
     }
     std::vector<Ctr> back(n);
@@ -655,7 +665,7 @@ void Engine::store(std::vector<Inst*>& v) {
         // the epoch left this instance's descriptor at d_step_[0]: no upload
         const Inst& I = *v[0];
         const StorePatch p{I.h.out_rows, I.h.out_par, I.h.out_n};
-        time_begin(&ev);
+        time_begin(&ev, 2);
         CK(launch_store((const InstDesc*)d_step_, nullptr, 0, tot, p, grid, st_));
         time_end(ev, 2, v);
         return;
@@ -668,7 +678,7 @@ void Engine::store(std::vector<Inst*>& v) {
     std::memcpy(host.data() + o_seg, segs.data(), segs.size() * sizeof(Seg));
     h2d(d, host.data(), bytes);
     epoch_single_ = nullptr;
-    time_begin(&ev);
+    time_begin(&ev, 2);
     CK(launch_store((const InstDesc*)d, (const Seg*)(d + o_seg), (int)segs.size(), tot, StorePatch{}, grid, st_));
     time_end(ev, 2, v);
 }
@@ -796,7 +806,6 @@ void Engine::step(std::vector<Inst*>& act) {
                         }
                         uint64_t keys = (uint64_t)(I.cap * 0.7);
                         ensure_table(I, (I.ctr.overflow & 5) ? keys * 4 : keys, true);
-                        fill(I.winbyte.p, I.h.nwords * 8, 0);
                     } catch (CapacityError& e) {
                         if (on_capacity(I, e)) continue;
                         I.check_only = true;   // cannot grow: check the level only

@@ -28,6 +28,7 @@ hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const un
 hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_chunks, int grid,
                         hipStream_t st);
 hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStream_t st);
+hipError_t launch_resolve_scan1(const InstDesc* D, hipStream_t st);
 hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch,
                         int grid, hipStream_t st);
 
@@ -206,7 +207,7 @@ class Engine {
     std::string node_string(Inst& I, int level, uint64_t row);
     std::string cand_string(Inst& I, int level, uint64_t o, int* top_op);
     void fill(void* p, uint64_t words, uint64_t v);
-    void time_begin(hipEvent_t* a);
+    void time_begin(hipEvent_t* a, int kind);
     void time_end(hipEvent_t a, int kind, const std::vector<Inst*>& who);
     void collect_times();
     void epoch(std::vector<Inst*>& v);

@@ -34,6 +34,7 @@ constexpr int kWideLanes = 8;         // lanes per candidate in k_expand's small
 constexpr uint64_t kWideMaxWaves = 2048; // levels below this many wavefronts use it
 constexpr int kTraceGroup = 8;        // traces whose operand loads k_expand issues together
 constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are counted in k_scan
+constexpr uint64_t kSmallFresh = 16384; // fresh lists up to this long are resolved in the scan's workgroup
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 

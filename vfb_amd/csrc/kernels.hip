@@ -609,6 +609,24 @@ __global__ void k_fresh_prefix(const InstDesc* __restrict__ D, Seg* segs, int n,
     *total = acc;
 }
 
+// Winner test of fresh-list entry e: the candidate still holds its key's
+// lowest ordinal.  The last pass of a level marks it in the byte map; every
+// pass reports a matching winner below found_below.
+__device__ __forceinline__ void resolve_entry(const InstDesc& d, uint64_t e) {
+    const uint64_t of = d.fresh_ord[e];
+    // a pass before the last only looks for a settled match
+    if (!d.mark && !(of & kMatchBit)) return;
+    const uint32_t slot = d.fresh_slot[e];
+    const uint64_t o = of & ~kMatchBit;
+    if (d.table[slot].ord == o) {
+        const uint64_t local = o - d.epoch_base;
+        // each ordinal appears at most once in the fresh list: a plain byte
+        // store, packed to bits (and cleared) by the count / scan kernels
+        if (d.mark) d.winbyte[local] = 1;
+        if ((of & kMatchBit) && local < d.found_below) atomicMin(&d.ctr->found, (unsigned long long)local);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
                                                  int nseg, const unsigned long long* __restrict__ total_p,
                                                  int single) {
@@ -622,20 +640,7 @@ __global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D,
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
         const Seg& sg = segs[seg_find(segs, nseg, g)];
-        const InstDesc& d = D[sg.inst];
-        const uint64_t e = g - sg.begin;
-        const uint64_t of = d.fresh_ord[e];
-        // a pass before the last only looks for a settled match
-        if (!d.mark && !(of & kMatchBit)) continue;
-        const uint32_t slot = d.fresh_slot[e];
-        const uint64_t o = of & ~kMatchBit;
-        if (d.table[slot].ord == o) {
-            const uint64_t local = o - d.epoch_base;
-            // each ordinal appears at most once in the fresh list: a plain
-            // byte store, packed to bits by the count / scan kernels
-            if (d.mark) d.winbyte[local] = 1;
-            if ((of & kMatchBit) && local < d.found_below) atomicMin(&d.ctr->found, (unsigned long long)local);
-        }
+        resolve_entry(D[sg.inst], g - sg.begin);
     }
 }
 
@@ -647,10 +652,11 @@ __device__ __forceinline__ uint64_t gather8(uint64_t x) {
     return (x | (x >> 28)) & 0xFFull;
 }
 
-// packs the 64 winner bytes of bitmap word w into the word; returns its popcount
+// packs the 64 winner bytes of bitmap word w into the word and clears them;
+// returns the word's popcount
 __device__ __forceinline__ uint32_t pack_word(const InstDesc& d, uint64_t w) {
     if (w >= d.nwords) return 0u;
-    const uint4* src = reinterpret_cast<const uint4*>(d.winbyte + w * 64);
+    uint4* src = reinterpret_cast<uint4*>(d.winbyte + w * 64);
     uint64_t bits = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -659,6 +665,9 @@ __device__ __forceinline__ uint32_t pack_word(const InstDesc& d, uint64_t w) {
         bits |= gather8(a) << (16 * q);
         bits |= gather8(b) << (16 * q + 8);
     }
+    // the map starts every level cleared
+#pragma unroll
+    for (int q = 0; q < 4; ++q) src[q] = make_uint4(0u, 0u, 0u, 0u);
     d.winbits[w] = bits;
     return (uint32_t)__popcll(bits);
 }
@@ -690,12 +699,12 @@ __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
     return v;
 }
 
-// one workgroup per instance: exclusive scan of the chunk counts, the level's
-// survivor total, and the rank of the returned formula among the survivors
-__global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, const uint32_t* __restrict__ insts) {
+// One workgroup (256 threads) per instance: exclusive scan of the chunk
+// counts, the level's survivor total, and the rank of the returned formula
+// among the survivors.
+__device__ void scan_one(const InstDesc& d) {
     __shared__ uint64_t wsum[4];
     __shared__ uint64_t carry;
-    const InstDesc& d = D[insts[blockIdx.x]];
     if (threadIdx.x == 0) carry = 0;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (d.nblk <= (uint64_t)kSmallChunks) {
@@ -721,7 +730,8 @@ __global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, co
         __syncthreads();
     }
     if (threadIdx.x < 64) {
-        const uint64_t f = d.ctr->found;
+        // set by atomics earlier in this kernel (fused form) or before it
+        const uint64_t f = __hip_atomic_load(&d.ctr->found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t rank = 0;
         if (f != kEmpty) {
             const uint64_t fw = f >> 6, fc = fw / kChunkWords;
@@ -736,6 +746,22 @@ __global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, co
             d.ctr->found_rank = rank;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_scan(const InstDesc* __restrict__ D, const uint32_t* __restrict__ insts) {
+    scan_one(D[insts[blockIdx.x]]);
+}
+
+// Resolve and scan of one small level of one instance in a single workgroup
+// (one launch instead of two): the fresh list is resolved by the block, then
+// scanned after a barrier.
+__global__ __launch_bounds__(256) void k_resolve_scan1(const InstDesc* __restrict__ D) {
+    const InstDesc& d = D[0];
+    const uint64_t c = d.ctr->fresh_count;
+    const uint64_t total = c < d.fresh_cap ? c : d.fresh_cap;
+    for (uint64_t e = threadIdx.x; e < total; e += blockDim.x) resolve_entry(d, e);
+    __syncthreads();
+    scan_one(d);
 }
 
 // operands of the candidate with level-local ordinal o
@@ -893,6 +919,11 @@ hipError_t launch_resolve(const InstDesc* D, const Seg* segs, int nseg, const un
 hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_chunks, int grid,
                         hipStream_t st) {
     hipLaunchKernelGGL(k_count, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_chunks);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_scan1(const InstDesc* D, hipStream_t st) {
+    hipLaunchKernelGGL(k_resolve_scan1, dim3(1), dim3(256), 0, st, D);
     return hipGetLastError();
 }
 
