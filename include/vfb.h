@@ -122,6 +122,9 @@ typedef struct {
                               stopped (status VFB_CAPACITY if nothing matched); 0: none */
     int pad;
     uint64_t otf_stored;   /* rows of otf_level that were stored */
+    double ms_probe;       /* HIP-event time of the target-check probes of large levels
+                              (vfb_tuning.pass_min); not part of ms_expand */
+    uint64_t probe_candidates; /* candidates those probes checked */
 } vfb_result;
 
 void vfb_options_default(vfb_options* opt);
@@ -177,10 +180,12 @@ void vfb_server_destroy(vfb_server* srv);
    reference counterpart (the reference hard-codes 1024-thread blocks and one
    launch per temp-buffer chunk, LTL_BS.cu:383-446, 1226-1493). */
 typedef struct {
-    uint64_t pass_min;             /* levels with at least this many candidates are built
-                                      in three passes in canonical order (~ X F G, & |, U)
-                                      and stop after the pass that settles a match;
-                                      0 = always one fused pass */
+    uint64_t pass_min;             /* levels with at least this many candidates are first
+                                      probed (target check only, no keys); a level holding
+                                      a match is then built one canonical operator block at
+                                      a time (~, &, |, X F G, U) and stops at the block that
+                                      settles it, any other level in one fused pass;
+                                      0 = never probe */
     uint32_t wide_max_waves;       /* an expansion of fewer wavefronts gives each
                                       candidate 8 lanes (0 = never) */
     uint32_t expand_blocks_per_cu; /* 256-thread workgroups per CU, expand launch, 1..64 */

@@ -1,10 +1,11 @@
 """GPU parity under every kind of tuning setting: launch shapes and the pass
 split change speed only, never a result.
 
-pass_min=1 builds every level with a binary operator in three passes
-(~ X F G, then & |, then U) that stop at the pass settling a match; the
-results must still equal the oracle's bit for bit, including the per-level
-counts and the returned formula.
+pass_min=1 probes every level with a binary operator (target check only)
+and builds a level that holds a match one canonical operator block at a time
+(~, &, |, X F G, U), stopping at the block that settles it; the results must
+still equal the oracle's bit for bit, including the per-level counts and the
+returned formula.
 """
 import random
 
@@ -85,8 +86,9 @@ def test_split_batch_equals_plain():
 
 
 def test_split_saves_work_at_found_level():
-    """With the split, a level settled by its first or second pass evaluates
-    fewer candidates than the fused pass; the answer is unchanged."""
+    """With the probe and operator blocks, a level settled by its first block
+    evaluates fewer candidates than the fused pass; the answer is unchanged,
+    and the levels before it are evaluated in full either way."""
     inst = V.Instance.generate(2, 3)          # its formula's root is ~ (first pass)
     old = V.set_tuning(pass_min=0)
     try:
@@ -99,3 +101,23 @@ def test_split_saves_work_at_found_level():
     assert split["formula"] == fused["formula"] and split["distinct"] == fused["distinct"]
     assert split["level_candidates"][s] < fused["level_candidates"][s]
     assert split["level_candidates"][:s] == fused["level_candidates"][:s]
+
+
+@pytest.mark.parametrize("rt", [1, 2])
+def test_probe_with_colliding_relaxed_keys(rt):
+    """Relaxed keys of types 1 and 2 sample bits, so distinct CS often share a
+    key: the lowest matching candidate found by the probe may then lose its
+    key to a lower ordinal, and the rest of the level has to be built.  The
+    results still equal the oracle's."""
+    old = V.set_tuning(pass_min=1)
+    try:
+        for k in range(8):
+            inst = V.Instance.generate(2, k)
+            names, lens, atoms, P = inst.arrays()
+            print(f"[parity] probe relaxed type {rt} cfg2 #{k}", flush=True)
+            want = O.enumerate_cs(O.Instance(names, lens, atoms, P), 8, rt)
+            got = V.solve(inst, max_size=8, relaxed_type=rt)
+            for f in FIELDS[:6]:
+                assert got[f] == want[f], (rt, k, f, got[f], want[f])
+    finally:
+        V.set_tuning(**old)

@@ -40,7 +40,8 @@ class Result(C.Structure):
                 ("expand_launches", C.c_uint64), ("table_slots", C.c_uint64),
                 ("retries", C.c_uint64), ("bytes_expand", C.c_uint64),
                 ("bytes_store", C.c_uint64), ("fresh_total", C.c_uint64),
-                ("otf_level", C.c_int), ("pad", C.c_int), ("otf_stored", C.c_uint64)]
+                ("otf_level", C.c_int), ("pad", C.c_int), ("otf_stored", C.c_uint64),
+                ("ms_probe", C.c_double), ("probe_candidates", C.c_uint64)]
 
 
 class Tuning(C.Structure):
@@ -237,6 +238,7 @@ def _todict(r):
         "retries": r.retries, "bytes_expand": r.bytes_expand,
         "bytes_store": r.bytes_store, "fresh_total": r.fresh_total,
         "otf_level": r.otf_level, "otf_stored": r.otf_stored,
+        "ms_probe": r.ms_probe, "probe_candidates": r.probe_candidates,
     }
 
 

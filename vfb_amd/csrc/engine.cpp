@@ -136,7 +136,10 @@ void Engine::collect_times() {
         float ms = 0.f;
         CK(hipEventElapsedTime(&ms, t.a, t.b));
         for (Inst* I : t.who) {
-            double* acc = t.kind == 0 ? &I->res->ms_expand : t.kind == 1 ? &I->res->ms_resolve : &I->res->ms_store;
+            double* acc = t.kind == 0   ? &I->res->ms_expand
+                          : t.kind == 1 ? &I->res->ms_resolve
+                          : t.kind == 2 ? &I->res->ms_store
+                                        : &I->res->ms_probe;
             *acc += ms;
         }
         ev_pool_.push_back(t.a);
@@ -374,41 +377,55 @@ void Engine::build_units(Inst& I) {
     h.mark = 1;
 }
 
-// Passes of a split level, in canonical order: 1 = ~ X F G (unary task),
-// 2 = & | (binary tasks), 3 = U (binary tasks).  After pass p every candidate
-// below pass_prefix(p) has been evaluated, so a match there is settled: no
-// later candidate can take its key or precede it.
-static uint64_t pass_prefix(const Inst& I, int pass) {
+// Passes of a level.  P_FUSED: every operator in one pass.  P_PROBE: target
+// check only (bit 0 of every candidate, no keys, no set), which finds m0, the
+// lowest matching candidate.  A level without one cannot hold the answer and
+// is built by P_FUSED.  Otherwise P_FIRST evaluates the canonical operator
+// blocks up to m0's block (the probe's match is then settled if it is a
+// winner, as it is unless a relaxed key collides), and P_REST the others.
+enum Pass { P_FUSED = 0, P_PROBE = 1, P_FIRST = 2, P_REST = 3 };
+
+static size_t nsplit_of(const Inst& I) { return (I.units[I.level + 1].size() - 4) / 3; }
+
+// first pass for a match at m0: task masks and the end of the evaluated
+// prefix; false when m0 lies in the last block (U), for which P_FUSED is used
+static bool first_pass_for(const Inst& I, uint64_t m0, uint8_t* us, uint8_t* bs, uint64_t* prefix) {
     const auto& U = I.units[I.level + 1];
-    const size_t nsplit = (U.size() - 4) / 3;
-    if (pass == 1) return U[1].base;                  // end of ~
-    if (pass == 2) return U[2 * nsplit + 4].base;     // end of G
-    return I.N;
+    const size_t ns = nsplit_of(I);
+    const uint64_t b_and = U[1].base, b_or = U[ns + 1].base, b_x = U[2 * ns + 1].base, b_u = U[2 * ns + 4].base;
+    if (m0 < b_and) { *us = SEL_NOT; *bs = 0; *prefix = b_and; }
+    else if (m0 < b_or) { *us = SEL_NOT; *bs = SEL_AND; *prefix = b_or; }
+    else if (m0 < b_x) { *us = SEL_NOT; *bs = SEL_ANDOR; *prefix = b_x; }
+    else if (m0 < b_u) { *us = SEL_ALL; *bs = SEL_ANDOR; *prefix = b_u; }
+    else return false;
+    return true;
 }
 
-// whether unit q of the level belongs to pass p (0: fused, all units)
-static bool unit_in_pass(size_t q, size_t nsplit, int pass) {
-    if (pass == 0) return true;
-    const bool unary = q == 0 || (q >= 2 * nsplit + 1 && q <= 2 * nsplit + 3);
-    const bool andor = q >= 1 && q <= 2 * nsplit;
-    if (pass == 1) return unary;
-    if (pass == 2) return andor;
-    if (pass == 3) return !unary && !andor;
-    return false;
+// whether unit q of the level is computed with unary / binary task masks us / bs
+static bool unit_selected(size_t q, size_t ns, uint8_t us, uint8_t bs) {
+    if (q == 0) return us & 1;                        // ~
+    if (q <= ns) return bs & 1;                       // &
+    if (q <= 2 * ns) return bs & 2;                   // |
+    if (q <= 2 * ns + 3) return us & (1u << (q - 2 * ns));   // X F G
+    return bs & 0xc;                                  // U, both orientations
 }
+
+// no keys and no set: an onTheFly level, or the probe of a level
+static bool checks_only(const Inst& I) { return I.check_only || I.pass == P_PROBE; }
 
 // candidates and algorithmic bytes (a*8T + 16 per candidate, no key slot in a
-// check-only level) of pass p
-static void pass_work(const Inst& I, int pass, uint64_t* cand, uint64_t* bytes) {
+// check-only level or a probe) of pass p
+static void pass_work(const Inst& I, uint64_t* cand, uint64_t* bytes) {
     const auto& U = I.units[I.level + 1];
-    const size_t nsplit = (U.size() - 4) / 3;
+    const size_t ns = nsplit_of(I);
+    const bool all = I.pass == P_FUSED || I.pass == P_PROBE;
     *cand = *bytes = 0;
     for (size_t q = 0; q < U.size(); ++q) {
-        if (!unit_in_pass(q, nsplit, pass)) continue;
+        if (!all && !unit_selected(q, ns, I.usel, I.bsel)) continue;
         const Unit& u = U[q];
         const uint64_t a = (u.op == VFB_OP_AND || u.op == VFB_OP_OR) ? 2 : 1;
         *cand += u.count;
-        *bytes += u.count * (a * 8 * (uint64_t)I.h.T + (I.check_only ? 0 : 16));
+        *bytes += u.count * (a * 8 * (uint64_t)I.h.T + (checks_only(I) ? 0 : 16));
     }
 }
 
@@ -536,13 +553,9 @@ void Engine::epoch(std::vector<Inst*>& v) {
     // small epochs use several lanes per candidate (shorter serial trace loop)
     // tasks of each instance's current pass
     auto in_pass = [](const Inst& I, Task& t) {
-        switch (I.pass) {
-            case 0: t.part = PART_ALL; return true;
-            case 1: return t.kind == 0;
-            case 2: t.part = PART_ANDOR; return t.kind == 1;
-            case 3: t.part = PART_UNTIL; return t.kind == 1;
-            default: return false;
-        }
+        if (I.pass == P_FUSED || I.pass == P_PROBE) t.part = SEL_ALL;
+        else t.part = t.kind == 0 ? I.usel : I.bsel;
+        return t.part != 0;
     };
     uint64_t waves1 = 0;
     for (size_t k = 0; k < n; ++k)
@@ -561,14 +574,11 @@ void Engine::epoch(std::vector<Inst*>& v) {
             waves += t.kind == 0 ? (t.nl + per_wave - 1) / per_wave : (uint64_t)t.nl * t.nchunk;
             tasks.push_back(t);
         }
-    // the last pass of a level (0, 3 or 4) marks winners and scans; passes 1
-    // and 2 only report a match below their settled prefix
-    auto last_pass = [](const Inst& I) { return I.pass == 0 || I.pass >= 3; };
     std::vector<uint32_t> ids;
     std::vector<Seg> bsegs;
     uint64_t btot = 0;
     for (size_t k = 0; k < n; ++k) {
-        if (v[k]->check_only || !last_pass(*v[k])) continue;
+        if (checks_only(*v[k])) continue;
         ids.push_back((uint32_t)k);
         // small bitmaps are counted inside k_scan
         if (v[k]->h.nblk > (uint64_t)kSmallChunks) { bsegs.push_back(Seg{(uint32_t)k, 0, btot}); btot += v[k]->h.nblk; }
@@ -587,13 +597,14 @@ void Engine::epoch(std::vector<Inst*>& v) {
     for (size_t k = 0; k < n; ++k) {
         Inst& I = *v[k];
         I.h.ctr = (Ctr*)(d + o_ctr) + k;
-        I.h.mark = last_pass(I) ? 1 : 0;
-        I.h.found_below = last_pass(I) ? ~0ull : pass_prefix(I, I.pass);
+        I.h.mark = 1;
+        I.h.found_below = I.pass == P_FIRST ? I.first_prefix : ~0ull;
         hd[k] = I.h;
-        if (I.pass <= 1) {          // first pass of the level: fresh counters
-            hc[k] = Ctr{};
+        hd[k].check_only = checks_only(I) ? 1 : 0;
+        if (I.pass != P_REST) {
+            hc[k] = Ctr{};          // first pass of the level's epoch: fresh counters
             hc[k].found = kEmpty;
-        } else {                    // later passes continue the level's epoch
+        } else {                    // later passes continue the epoch
             hc[k] = I.ctr;
         }
     }
@@ -605,26 +616,29 @@ void Engine::epoch(std::vector<Inst*>& v) {
     const InstDesc* D = (const InstDesc*)d;
 
     hipEvent_t ev = nullptr;
+    bool all_probe = true;
+    for (auto* I : v) all_probe &= I->pass == P_PROBE;
+    const int ekind = all_probe ? 3 : 0;    // probes are timed apart from the dedup expansion
     if (waves) {
-        time_begin(&ev, 0);
+        time_begin(&ev, ekind);
         int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * tun_.expand_blocks_per_cu);
         CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, grid, st_));
-        time_end(ev, 0, v);
+        time_end(ev, ekind, v);
         for (auto* I : v) I->res->expand_launches++;
     }
     bool any_dedup = false;
-    for (auto* I : v) any_dedup |= !I->check_only;
+    for (auto* I : v) any_dedup |= !checks_only(*I);
     if (any_dedup) {
         std::vector<Inst*> dd;
         for (auto* I : v)
-            if (!I->check_only) dd.push_back(I);
+            if (!checks_only(*I)) dd.push_back(I);
         time_begin(&ev, 1);
         unsigned long long* tot = (unsigned long long*)(d + o_tot);
         // one instance: k_resolve reads its fresh count itself (segment 0 at 0)
         const bool single = n == 1;
         // one small instance at its last pass: resolve and scan in one workgroup
         const Inst& I0 = *v[0];
-        if (single && !I0.check_only && last_pass(I0) && I0.h.nblk <= (uint64_t)kSmallChunks &&
+        if (single && !checks_only(I0) && I0.h.nblk <= (uint64_t)kSmallChunks &&
             I0.h.fresh_cap <= kSmallFresh) {
             CK(launch_resolve_scan1(D, st_));
         } else {
@@ -776,9 +790,10 @@ void Engine::step(std::vector<Inst*>& act) {
         // ---- 2-3. epochs: the passes of each level; a level whose set or
         // list overflowed is grown and built again from its first pass ----
         for (auto* I : step) {
-            const size_t nsplit = (I->units[I->level + 1].size() - 4) / 3;
-            I->passes = (tun_.pass_min && I->N >= tun_.pass_min && nsplit >= 1) ? 3 : 1;
-            I->pass = I->passes > 1 ? 1 : 0;
+            // large deduplicating levels are probed first
+            const bool probe = !I->check_only && tun_.pass_min && I->N >= tun_.pass_min && nsplit_of(*I) >= 1;
+            I->pass = probe ? P_PROBE : P_FUSED;
+            I->usel = I->bsel = SEL_ALL;
             I->evaluated = 0;
         }
         std::vector<Inst*> pend = step;
@@ -806,27 +821,49 @@ void Engine::step(std::vector<Inst*>& act) {
                         }
                         uint64_t keys = (uint64_t)(I.cap * 0.7);
                         ensure_table(I, (I.ctr.overflow & 5) ? keys * 4 : keys, true);
+                        // the level is built again from its first dedup pass
+                        if (I.pass == P_REST) {
+                            I.pass = P_FIRST;
+                            I.usel = I.first_usel;
+                            I.bsel = I.first_bsel;
+                        }
                     } catch (CapacityError& e) {
                         if (on_capacity(I, e)) continue;
                         I.check_only = true;   // cannot grow: check the level only
                         build_units(I);
+                        I.pass = P_FUSED;
                     }
-                    I.pass = I.passes > 1 ? 1 : 0;
                     I.evaluated = 0;
                     again.push_back(&I);
                     continue;
                 }
                 uint64_t c = 0, b = 0;
-                pass_work(I, I.pass, &c, &b);
+                pass_work(I, &c, &b);
+                if (I.pass == P_PROBE) {
+                    // no match anywhere in the level: it cannot hold the
+                    // answer, one fused pass; else the blocks up to the match
+                    I.res->probe_candidates += c;
+                    uint8_t us = SEL_ALL, bs = SEL_ALL;
+                    uint64_t pre = 0;
+                    if (I.ctr.found != kEmpty && first_pass_for(I, I.ctr.found, &us, &bs, &pre)) {
+                        I.pass = P_FIRST;
+                        I.usel = I.first_usel = us;
+                        I.bsel = I.first_bsel = bs;
+                        I.first_prefix = pre;
+                    } else {
+                        I.pass = P_FUSED;
+                    }
+                    again.push_back(&I);
+                    continue;
+                }
                 I.evaluated += c;
                 I.res->bytes_expand += b;
-                if (I.pass == 1 || I.pass == 2) {
-                    // a check-only level reports every match (the lowest so
-                    // far); it is settled once below the evaluated prefix
-                    const bool hit = I.ctr.found != kEmpty &&
-                                     (!I.check_only || I.ctr.found < pass_prefix(I, I.pass));
-                    if (hit && I.check_only) continue;
-                    I.pass = hit ? 4 : I.pass + 1;
+                if (I.pass == P_FIRST && I.ctr.found == kEmpty) {
+                    // the probe's match was no winner (its relaxed key was
+                    // taken by a lower ordinal): the rest of the level
+                    I.pass = P_REST;
+                    I.usel = (uint8_t)(SEL_ALL & ~I.usel);
+                    I.bsel = (uint8_t)(SEL_ALL & ~I.bsel);
                     again.push_back(&I);
                 }
             }
@@ -1050,7 +1087,7 @@ extern "C" vfb_status vfb_solve_batch(const vfb_instance* const* insts, size_t n
 
 extern "C" void vfb_tuning_default(vfb_tuning* t) {
     if (!t) return;
-    t->pass_min = 1ull << 22;
+    t->pass_min = 1ull << 24;
     t->wide_max_waves = kWideMaxWaves;
     t->expand_blocks_per_cu = 8;
     t->resolve_blocks_per_cu = 16;

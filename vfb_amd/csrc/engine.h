@@ -142,10 +142,14 @@ struct Inst {
     uint64_t N = 0;             // candidates of the level being built
     std::vector<Task> tasks;
     Ctr ctr{};
-    // A large level is built in three passes in canonical order -- ~ X F G,
-    // then & |, then U -- and stops after the pass that settles a match.
-    int passes = 1;             // 1: one fused pass, 3: split
-    int pass = 0;               // next pass: 0 fused, 1..3 split, 4 winner marking only
+    // A large level is first probed (target check only, no keys).  Without a
+    // match it is built in one fused pass; with one it is built operator
+    // block by operator block in canonical order and stops at the block that
+    // settles the match.
+    int pass = 0;               // next pass of the level (enum Pass in engine.cpp)
+    uint8_t usel = 0xf, bsel = 0xf;     // operators of the pass: unary / binary task masks
+    uint8_t first_usel = 0xf, first_bsel = 0xf;
+    uint64_t first_prefix = 0;  // every candidate below it is evaluated by the first pass
     uint64_t evaluated = 0;     // candidates of this level evaluated so far
 };
 

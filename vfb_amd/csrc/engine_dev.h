@@ -99,9 +99,14 @@ struct InstDesc {
 
 // Fused work item: all operators that read the same operand rows.
 //   kind 0: rows of level lvl_l -> ~, X, F, G     (base[0..3])
-//   kind 1: pairs (level lvl_l x level lvl_r) -> &, |, lUr, rUl  (base[0..2]);
-//           part 0 all four, part 1 & and | only, part 2 lUr and rUl only
-enum { PART_ALL = 0, PART_ANDOR = 1, PART_UNTIL = 2 };
+//   kind 1: pairs (level lvl_l x level lvl_r) -> &, |, lUr, rUl  (base[0..2])
+// `part` is the mask of the task's four candidates that are computed: bit q
+// selects base[q]'s operator (~ X F G, or & | lUr rUl).  The masks in use:
+enum {
+    SEL_ALL = 0xf,
+    SEL_NOT = 0x1, SEL_XFG = 0xe,                               // unary
+    SEL_AND = 0x1, SEL_ANDOR = 0x3, SEL_ORU = 0xe, SEL_U = 0xc  // binary
+};
 struct Task {
     uint32_t inst;
     uint8_t kind, lvl_l, lvl_r, part;

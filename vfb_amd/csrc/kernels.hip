@@ -382,9 +382,10 @@ __device__ __forceinline__ void finish_group(const InstDesc& d, Key* k, const ui
     }
 }
 
-// ~, X, F, G of one row.  G lanes per row; lane `sub` folds traces
-// sub, sub+G, ... in groups whose loads are issued before their first use.
-template <int KEY, bool CHECK, int G>
+// ~, X, F, G of one row (SEL: mask of the four computed).  G lanes per row;
+// lane `sub` folds traces sub, sub+G, ... in groups whose loads are issued
+// before their first use.
+template <int KEY, bool CHECK, int G, int SEL>
 __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
     constexpr int W = 64 / G;
     const int sub = lane % G;
@@ -413,13 +414,16 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
                 continue;
             }
             const uint64_t c0 = ~a & m;                      // ~
-            const uint64_t c1 = a >> 1;                      // X
-            const uint64_t c2 = suffix_or(a);                // F
-            const uint64_t c3 = a & ~suffix_or(c0);          // G
-            mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1) |
-                  ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
-            if constexpr (!CHECK) {
+            if constexpr ((SEL & 1) != 0) {
+                mm |= (uint32_t)((c0 ^ want) & 1ull);
                 key_acc<KEY>(k[0], c0, d, t);
+            }
+            if constexpr ((SEL & 0xe) != 0) {
+                const uint64_t c1 = a >> 1;                  // X
+                const uint64_t c2 = suffix_or(a);            // F
+                const uint64_t c3 = a & ~suffix_or(c0);      // G
+                mm |= ((uint32_t)((c1 ^ want) & 1ull) << 1) | ((uint32_t)((c2 ^ want) & 1ull) << 2) |
+                      ((uint32_t)((c3 ^ want) & 1ull) << 3);
                 key_acc<KEY>(k[1], c1, d, t);
                 key_acc<KEY>(k[2], c2, d, t);
                 key_acc<KEY>(k[3], c3, d, t);
@@ -427,12 +431,19 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
         }
     }
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? 0xfu : 0u, lane);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? (uint32_t)SEL : 0u, lane);
 }
 
-// &, |, l U r, r U l of one pair (left row uniform per wavefront); PART
-// selects all four, & and | only, or the two U orientations only
-template <int KEY, bool CHECK, int G, int PART>
+template <int KEY, int G>
+__device__ __forceinline__ void expand_unary_part(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    if (tk.part == SEL_NOT) expand_unary<KEY, false, G, SEL_NOT>(d, tk, wl, lane);
+    else if (tk.part == SEL_XFG) expand_unary<KEY, false, G, SEL_XFG>(d, tk, wl, lane);
+    else expand_unary<KEY, false, G, SEL_ALL>(d, tk, wl, lane);
+}
+
+// &, |, l U r, r U l of one pair (left row uniform per wavefront); SEL is
+// the mask of the four computed
+template <int KEY, bool CHECK, int G, int SEL>
 __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
     constexpr int W = 64 / G;
     const int sub = lane % G;
@@ -461,22 +472,21 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
             const uint64_t a = av[j], b = bv[j];
             const uint64_t want = t < d.P ? 1ull : 0ull;
             if constexpr (CHECK) {
-                if constexpr (PART == PART_ALL)
-                    mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
-                else if constexpr (PART == PART_ANDOR)
-                    mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, 0ull, 0ull, want);
-                else
-                    mm |= bit0_mismatch(0ull, 0ull, until_bit0(a, b), until_bit0(b, a), want);
+                // the probe and check-only levels evaluate every operator
+                mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
                 continue;
             }
-            if constexpr (PART != PART_UNTIL) {
+            if constexpr ((SEL & 1) != 0) {
                 const uint64_t c0 = a & b;
-                const uint64_t c1 = a | b;
-                mm |= (uint32_t)((c0 ^ want) & 1ull) | ((uint32_t)((c1 ^ want) & 1ull) << 1);
+                mm |= (uint32_t)((c0 ^ want) & 1ull);
                 key_acc<KEY>(k[0], c0, d, t);
+            }
+            if constexpr ((SEL & 2) != 0) {
+                const uint64_t c1 = a | b;
+                mm |= (uint32_t)((c1 ^ want) & 1ull) << 1;
                 key_acc<KEY>(k[1], c1, d, t);
             }
-            if constexpr (PART != PART_ANDOR) {
+            if constexpr ((SEL & 0xc) != 0) {
                 const uint64_t c2 = until_op(a, b);
                 const uint64_t c3 = until_op(b, a);
                 mm |= ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
@@ -488,15 +498,18 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     const uint64_t pair = l * tk.nr + r;
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
-    constexpr uint32_t sel = PART == PART_ALL ? 0xfu : (PART == PART_ANDOR ? 0x3u : 0xcu);
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? sel : 0u, lane);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? (uint32_t)SEL : 0u, lane);
 }
 
-template <int KEY, bool CHECK, int G>
+template <int KEY, int G>
 __device__ __forceinline__ void expand_binary_part(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
-    if (tk.part == PART_ANDOR) expand_binary<KEY, CHECK, G, PART_ANDOR>(d, tk, wl, lane);
-    else if (tk.part == PART_UNTIL) expand_binary<KEY, CHECK, G, PART_UNTIL>(d, tk, wl, lane);
-    else expand_binary<KEY, CHECK, G, PART_ALL>(d, tk, wl, lane);
+    switch (tk.part) {
+        case SEL_AND: expand_binary<KEY, false, G, SEL_AND>(d, tk, wl, lane); break;
+        case SEL_ANDOR: expand_binary<KEY, false, G, SEL_ANDOR>(d, tk, wl, lane); break;
+        case SEL_ORU: expand_binary<KEY, false, G, SEL_ORU>(d, tk, wl, lane); break;
+        case SEL_U: expand_binary<KEY, false, G, SEL_U>(d, tk, wl, lane); break;
+        default: expand_binary<KEY, false, G, SEL_ALL>(d, tk, wl, lane); break;
+    }
 }
 
 // G = lanes per candidate: 1 for large levels (throughput), kWideLanes for
@@ -518,16 +531,17 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
         const InstDesc& d = D[tk.inst];
         const uint64_t wl = w - tk.wave_begin;
         if (d.check_only) {
-            if (tk.kind == 0) expand_unary<KEY_MIX, true, G>(d, tk, wl, lane);
-            else expand_binary_part<KEY_MIX, true, G>(d, tk, wl, lane);
+            // onTheFly level or the probe of a level: target check only
+            if (tk.kind == 0) expand_unary<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
+            else expand_binary<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
         } else if (tk.kind == 0) {
-            if (d.key_kind == KEY_MIX) expand_unary<KEY_MIX, false, G>(d, tk, wl, lane);
-            else if (d.key_kind == KEY_FIELD) expand_unary<KEY_FIELD, false, G>(d, tk, wl, lane);
-            else expand_unary<KEY_STRIDE, false, G>(d, tk, wl, lane);
+            if (d.key_kind == KEY_MIX) expand_unary_part<KEY_MIX, G>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_unary_part<KEY_FIELD, G>(d, tk, wl, lane);
+            else expand_unary_part<KEY_STRIDE, G>(d, tk, wl, lane);
         } else {
-            if (d.key_kind == KEY_MIX) expand_binary_part<KEY_MIX, false, G>(d, tk, wl, lane);
-            else if (d.key_kind == KEY_FIELD) expand_binary_part<KEY_FIELD, false, G>(d, tk, wl, lane);
-            else expand_binary_part<KEY_STRIDE, false, G>(d, tk, wl, lane);
+            if (d.key_kind == KEY_MIX) expand_binary_part<KEY_MIX, G>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_binary_part<KEY_FIELD, G>(d, tk, wl, lane);
+            else expand_binary_part<KEY_STRIDE, G>(d, tk, wl, lane);
         }
     }
 }
