@@ -76,9 +76,18 @@ def cpu_baseline(insts, max_size, budget_s, gpu_results=None):
             "bit_exact_vs_gpu": same}
 
 
-def batched_mix(V, n, max_size):
-    """Config 4: n uneven instances, batched vs sequential plain solves."""
+def batched_mix(V, n, max_size, autotune=True):
+    """Config 4: n uneven instances, batched vs sequential plain solves.
+
+    The library's autotuner first picks launch / pass parameters on a sample
+    of the mix (instances 0..47, batched); both legs then run with them.  Also
+    reported: the same comparison without the instances that keep the GPU
+    busy alone (plain solve over 10 ms), where batching has idle GPU to fill."""
     insts = [V.Instance.generate(4, k) for k in range(n)]
+    tuned = None
+    if autotune:
+        sample = insts[:min(48, n)]
+        tuned = V.autotune(sample, max_size=max_size, batched=True, reps=2)
     V.solve(insts[0], max_size=max_size)
     t0 = time.perf_counter()
     plain = [V.solve(i, max_size=max_size) for i in insts]
@@ -89,11 +98,22 @@ def batched_mix(V, n, max_size):
     fields = ("status", "min_size", "formula", "distinct", "candidates", "level_winners", "otf_level", "otf_stored")
     exact = all(a[f] == b[f] for a, b in zip(plain, batch) for f in fields)
     lat = [r["ms_total"] for r in plain]
+    small = [k for k in range(n) if plain[k]["ms_total"] < 10.0]
+    t_small_plain = sum(plain[k]["ms_total"] for k in small) / 1e3
+    t0 = time.perf_counter()
+    sb = V.solve_batch([insts[k] for k in small], max_size=max_size)
+    t_small_batch = time.perf_counter() - t0
+    exact_small = all(plain[k][f] == b[f] for k, b in zip(small, sb) for f in fields)
     return {"instances": n, "max_formula_size": max_size,
             "instances_per_s_plain": n / t_plain, "instances_per_s_batched": n / t_batch,
             "batched_vs_plain": t_plain / t_batch, "bit_exact": exact,
             "plain_latency_ms_p50": pct(lat, 0.5), "plain_latency_ms_p99": pct(lat, 0.99),
-            "distinct_cs": sum(sum(r["level_winners"]) for r in plain)}
+            "distinct_cs": sum(sum(r["level_winners"]) for r in plain),
+            "gpu_saturating_instances": n - len(small),
+            "gpu_saturating_share_of_plain_time": 1.0 - t_small_plain / t_plain,
+            "others": {"instances": len(small), "batched_vs_plain": t_small_plain / t_small_batch,
+                       "bit_exact": exact_small},
+            "autotuned": tuned}
 
 
 def served(V, rate, seconds, max_size, max_active, check):

@@ -49,6 +49,7 @@ Engine::Engine(size_t arena_bytes) {
 Engine::~Engine() {
     (void)hipStreamSynchronize(st_);
     if (d_step_) (void)hipFree(d_step_);
+    if (d_ops_) (void)hipFree(d_ops_);
     if (h_flag_) (void)hipHostFree(h_flag_);
     if (d_flag_) (void)hipFree(d_flag_);
     if (pin_) (void)hipHostFree(pin_);
@@ -192,6 +193,7 @@ void Engine::ensure(Inst& I, Buf& b, size_t bytes) {
 }
 
 void Engine::release_all(Inst& I) {
+    purge_ops(I);
     for (auto& b : I.rows) dfree(I, b);
     dfree(I, I.table);
     dfree(I, I.fresh_ord);
@@ -204,9 +206,77 @@ void Engine::release_all(Inst& I) {
     I.cap = 0;
 }
 
-void Engine::fill(void* p, uint64_t words, uint64_t v) {
+void Engine::fill(Inst& I, void* p, uint64_t words, uint64_t v) {
+    if (batch_) {
+        fills_.push_back(FillOp{&I, (unsigned long long*)p, SIZE_MAX, words, v});
+        return;
+    }
     int grid = (int)std::min<uint64_t>((words + 255) / 256, (uint64_t)cus_ * 8);
     CK(launch_fill((unsigned long long*)p, words, v, std::max(grid, 1), st_));
+}
+
+void Engine::upload(Inst& I, void* dst, const uint64_t* src, uint64_t words) {
+    if (!batch_) {
+        h2d(dst, src, words * 8);
+        return;
+    }
+    fills_.push_back(FillOp{&I, (unsigned long long*)dst, staged_.size(), words, 0});
+    staged_.insert(staged_.end(), src, src + words);
+}
+
+// drops the queued operations of an instance whose buffers are released
+void Engine::purge_ops(const Inst& I) {
+    fills_.erase(std::remove_if(fills_.begin(), fills_.end(), [&](const FillOp& f) { return f.I == &I; }),
+                 fills_.end());
+    rehashes_.erase(std::remove_if(rehashes_.begin(), rehashes_.end(), [&](const RehashOp& r) { return r.I == &I; }),
+                    rehashes_.end());
+}
+
+// one upload of the segment tables (and staged data), one fill / copy launch,
+// one rehash launch; fills first, since rehash targets are fresh tables
+void Engine::flush_ops() {
+    if (fills_.empty() && rehashes_.empty()) return;
+    const size_t nf = fills_.size(), nr = rehashes_.size();
+    const size_t o_rs = align256(nf * sizeof(OpSeg));
+    const size_t o_st = o_rs + align256(nr * sizeof(RehashSeg));
+    const size_t bytes = o_st + staged_.size() * 8;
+    if (bytes > ops_cap_) {
+        sync();
+        if (d_ops_) CK(hipFree(d_ops_));
+        ops_cap_ = std::max(bytes, ops_cap_ * 2);
+        CK(hipMalloc((void**)&d_ops_, ops_cap_));
+    }
+    std::vector<char> host(bytes, 0);
+    OpSeg* fs = (OpSeg*)host.data();
+    uint64_t ftot = 0;
+    for (size_t k = 0; k < nf; ++k) {
+        const FillOp& f = fills_[k];
+        const unsigned long long* src =
+            f.staged == SIZE_MAX ? nullptr : (const unsigned long long*)(d_ops_ + o_st) + f.staged;
+        fs[k] = OpSeg{f.dst, src, f.words, f.value, ftot};
+        ftot += f.words;
+    }
+    RehashSeg* rs = (RehashSeg*)(host.data() + o_rs);
+    uint64_t rtot = 0;
+    for (size_t k = 0; k < nr; ++k) {
+        const RehashOp& r = rehashes_[k];
+        rs[k] = RehashSeg{r.old, r.tab, r.mask, r.keep, rtot};
+        rtot += r.old_cap;
+    }
+    if (!staged_.empty()) std::memcpy(host.data() + o_st, staged_.data(), staged_.size() * 8);
+    h2d(d_ops_, host.data(), bytes);
+    if (ftot) {
+        const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((ftot + 255) / 256, (uint64_t)cus_ * 8));
+        CK(launch_fill_multi((const OpSeg*)d_ops_, (int)nf, ftot, grid, st_));
+    }
+    if (rtot) {
+        const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((rtot + 255) / 256, (uint64_t)cus_ * 8));
+        CK(launch_rehash_multi((const RehashSeg*)(d_ops_ + o_rs), (int)nr, rtot, d_flag_, grid, st_));
+        flag_dirty_ = true;
+    }
+    fills_.clear();
+    rehashes_.clear();
+    staged_.clear();
 }
 
 // key parameters per trace (LTL_BS.cu:30-110), computed once per instance
@@ -297,7 +367,7 @@ void Engine::setup(Inst& I) {
     for (int a = 0; a < A; ++a)
         for (int t = 0; t < T; ++t) col[(size_t)t * A + a] = in.atoms[(size_t)a * T + t] & len_mask(in.lens[t]);
     dalloc(I, I.rows[1], col.size() * 8);
-    h2d(I.rows[1].p, col.data(), col.size() * 8);
+    upload(I, I.rows[1].p, col.data(), col.size());
     I.nrows[1] = (uint32_t)A;
     I.stored = (uint64_t)A;
     I.level = 1;
@@ -447,7 +517,7 @@ void Engine::prepare_scratch(Inst& I) {
         ensure(I, I.winbits, std::max<uint64_t>(nwords, 1) * 8);
         ensure(I, I.winbyte, std::max<uint64_t>(nwords, 1) * 64);
         // cleared once here; the count / scan kernels clear what they pack
-        fill(I.winbyte.p, std::max<uint64_t>(nwords, 1) * 8, 0);
+        fill(I, I.winbyte.p, std::max<uint64_t>(nwords, 1) * 8, 0);
         I.nwords_cap = nwords;
     }
     if (nblk > I.nblk_cap) {
@@ -473,8 +543,10 @@ void Engine::prepare_scratch(Inst& I) {
 // of completed epochs (ord < epoch_base)
 void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     uint64_t need = next_pow2(std::max<uint64_t>(4096, (uint64_t)(want_keys / 0.7) + 1));
-    // headroom while the set is small, so that small instances rarely rehash
-    if (need < (1ull << 24)) need = std::min<uint64_t>(need * 4, 1ull << 24);
+    // headroom while the set is small, so that a small plain solve rarely
+    // rehashes (a launch each); batched rehashes and clears share launches,
+    // where headroom would only add bytes to clear
+    if (!batch_ && need < (1ull << 24)) need = std::min<uint64_t>(need * 4, 1ull << 24);
     if (need > (1ull << 32)) need = 1ull << 32;
     if (I.table.p && need <= I.cap && !force_rebuild) {
         I.h.claim_limit = I.cap * 7 / 10 > I.committed ? I.cap * 7 / 10 - I.committed : 0;
@@ -483,12 +555,16 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     if (force_rebuild && need <= I.cap) need = I.cap;
     Buf nt;
     dalloc(I, nt, need * sizeof(Slot));
-    fill(nt.p, need * 4, kEmpty);
+    fill(I, nt.p, need * 4, kEmpty);
     if (I.table.p) {
         // the probe-limit flag is read at the next stream sync (check_flag)
-        int grid = (int)std::min<uint64_t>((I.cap + 255) / 256, (uint64_t)cus_ * 8);
-        CK(launch_rehash((const Slot*)I.table.p, I.cap, (Slot*)nt.p, need - 1, I.gbase, d_flag_, grid, st_));
-        flag_dirty_ = true;
+        if (batch_) {
+            rehashes_.push_back(RehashOp{&I, (const Slot*)I.table.p, I.cap, (Slot*)nt.p, need - 1, I.gbase});
+        } else {
+            int grid = (int)std::min<uint64_t>((I.cap + 255) / 256, (uint64_t)cus_ * 8);
+            CK(launch_rehash((const Slot*)I.table.p, I.cap, (Slot*)nt.p, need - 1, I.gbase, d_flag_, grid, st_));
+            flag_dirty_ = true;
+        }
         release_after_sync(I, I.table);
     }
     I.table = nt;
@@ -548,6 +624,7 @@ bool Engine::on_capacity(Inst& I, const CapacityError& e) {
 }
 
 void Engine::epoch(std::vector<Inst*>& v) {
+    flush_ops();
     const size_t n = v.size();
     // packed upload: descriptors | counters | tasks | fresh segments | total | chunk segments | ids
     // small epochs use several lanes per candidate (shorter serial trace loop)
@@ -738,21 +815,26 @@ void Engine::start(std::vector<Inst*>& all, std::vector<Inst*>& out) {
     // atoms into the hash sets (hashSetsInitialisation, LTL_BS.cu:1217)
     act.erase(std::remove_if(act.begin(), act.end(), [](Inst* I) { return I->done; }), act.end());
     if (!act.empty()) {
+        flush_ops();    // atom rows uploaded, tables cleared
         const size_t n = act.size();
         const size_t o_ctr = align256(n * sizeof(InstDesc));
-        const size_t bytes = o_ctr + align256(n * sizeof(Ctr));
+        const size_t o_seg = o_ctr + align256(n * sizeof(Ctr));
+        const size_t bytes = o_seg + align256(n * sizeof(Seg));
         char* d = step_buf(bytes);
         std::vector<char> host(bytes, 0);
+        uint64_t total = 0;
         for (size_t k = 0; k < n; ++k) {
             Inst& I = *act[k];
             I.h.lvl[1] = (const unsigned long long*)I.rows[1].p;
             I.h.nrows[1] = I.nrows[1];
             I.h.ctr = (Ctr*)(d + o_ctr) + k;
             ((InstDesc*)host.data())[k] = I.h;
+            ((Seg*)(host.data() + o_seg))[k] = Seg{(uint32_t)k, 0, total};
+            total += I.nrows[1];
         }
         h2d(d, host.data(), bytes);
-        for (size_t k = 0; k < n; ++k)
-            CK(launch_insert_atoms((const InstDesc*)d, (uint32_t)k, act[k]->nrows[1], d_flag_, st_));
+        // every instance's atoms in one launch
+        CK(launch_insert_atoms_multi((const InstDesc*)d, (const Seg*)(d + o_seg), (int)n, total, d_flag_, st_));
         flag_dirty_ = true;
         for (size_t k = 0; k < n; ++k) {
             act[k]->committed = act[k]->nrows[1];   // at most A distinct atom keys

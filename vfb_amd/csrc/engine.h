@@ -29,6 +29,11 @@ hipError_t launch_count(const InstDesc* D, const Seg* segs, int nseg, uint64_t t
                         hipStream_t st);
 hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStream_t st);
 hipError_t launch_resolve_scan1(const InstDesc* D, hipStream_t st);
+hipError_t launch_insert_atoms_multi(const InstDesc* D, const Seg* segs, int nseg, uint64_t total, unsigned int* fail,
+                                     hipStream_t st);
+hipError_t launch_fill_multi(const OpSeg* segs, int n, uint64_t total, int grid, hipStream_t st);
+hipError_t launch_rehash_multi(const RehashSeg* segs, int n, uint64_t total, unsigned int* fail, int grid,
+                               hipStream_t st);
 hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch,
                         int grid, hipStream_t st);
 
@@ -180,6 +185,15 @@ class Engine {
     unsigned int* h_flag_ = nullptr;   // its pinned host copy
     bool flag_dirty_ = false;
     std::vector<void*> pending_free_;
+    // batch mode: fills, uploads and rehashes of many instances are queued and
+    // flushed as one launch each before the next expansion
+    struct FillOp { Inst* I; unsigned long long* dst; size_t staged; uint64_t words; uint64_t value; };
+    struct RehashOp { Inst* I; const Slot* old; uint64_t old_cap; Slot* tab; uint64_t mask; uint64_t keep; };
+    std::vector<FillOp> fills_;
+    std::vector<RehashOp> rehashes_;
+    std::vector<unsigned long long> staged_;  // upload data of copy ops (staged != SIZE_MAX)
+    char* d_ops_ = nullptr;
+    size_t ops_cap_ = 0;
     char* pin_ = nullptr;         // pinned staging for uploads, recycled at sync points
     size_t pin_cap_ = 0, pin_used_ = 0;
     std::vector<hipEvent_t> ev_pool_;
@@ -210,7 +224,10 @@ class Engine {
     bool on_capacity(Inst& I, const CapacityError& e);
     std::string node_string(Inst& I, int level, uint64_t row);
     std::string cand_string(Inst& I, int level, uint64_t o, int* top_op);
-    void fill(void* p, uint64_t words, uint64_t v);
+    void fill(Inst& I, void* p, uint64_t words, uint64_t v);
+    void upload(Inst& I, void* dst, const uint64_t* src, uint64_t words);
+    void flush_ops();
+    void purge_ops(const Inst& I);
     void time_begin(hipEvent_t* a, int kind);
     void time_end(hipEvent_t a, int kind, const std::vector<Inst*>& who);
     void collect_times();

@@ -117,6 +117,26 @@ struct Task {
     uint64_t wave_begin;        // prefix over tasks, in wavefronts
 };
 
+// One segment of a batched fill / copy: words [begin, begin + words) of the
+// flat index space set dst[i] = src ? src[i] : value.
+struct OpSeg {
+    unsigned long long* dst;
+    const unsigned long long* src;
+    uint64_t words;
+    uint64_t value;
+    uint64_t begin;
+};
+
+// One segment of a batched rehash: slots [begin, begin + old_cap) of the flat
+// index space are the old table's, moved into tab if ord < keep_below.
+struct RehashSeg {
+    const Slot* old;
+    Slot* tab;
+    uint64_t mask;
+    uint64_t keep_below;
+    uint64_t begin;
+};
+
 // Output of a level's store, passed by value when one instance is stored
 // (its descriptor is then the one already on the device from the epoch).
 struct StorePatch {

@@ -546,6 +546,17 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
     }
 }
 
+// index of the segment holding flat index g (segments sorted by begin)
+template <class S>
+__device__ __forceinline__ int seg_of(const S* segs, int n, uint64_t g) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].begin <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Inserts the rows of level 1 (the atoms) with ordinals 0..A-1 so that later
 // levels deduplicate against them (hashSetsInitialisation, LTL_BS.cu:136-160).
 template <int KEY>
@@ -559,6 +570,20 @@ __device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned i
     if (probe_insert(d.table, d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed) == PR_FAIL) atomicOr(fail, 1u);
 }
 
+// the atoms of many instances in one launch (segment k = instance k's atoms)
+__global__ void k_insert_atoms_multi(const InstDesc* __restrict__ D, const Seg* __restrict__ segs, int nseg,
+                                     uint64_t total, unsigned int* fail) {
+    const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const Seg& sg = segs[seg_of(segs, nseg, g)];
+    const InstDesc& d = D[sg.inst];
+    const uint64_t x = g - sg.begin;
+    const uint32_t n = d.nrows[1];
+    if (d.key_kind == KEY_MIX) insert_row<KEY_MIX>(d, x, n, fail);
+    else if (d.key_kind == KEY_FIELD) insert_row<KEY_FIELD>(d, x, n, fail);
+    else insert_row<KEY_STRIDE>(d, x, n, fail);
+}
+
 __global__ void k_insert_atoms(const InstDesc* __restrict__ D, uint32_t inst, uint32_t n, unsigned int* fail) {
     const uint64_t x = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (x >= n) return;
@@ -568,27 +593,48 @@ __global__ void k_insert_atoms(const InstDesc* __restrict__ D, uint32_t inst, ui
     else insert_row<KEY_STRIDE>(d, x, n, fail);
 }
 
-// Moves the keys of completed epochs (ord < keep_below) into a fresh table.
+// Moves one key of a completed epoch (ord < keep_below) into a fresh table.
 // The keys are distinct, so a key takes the first slot whose hi it claims
 // from EMPTY, alone (one CAS); lo and ord are plain stores, read only by
 // later kernels.  Lookups stay valid: they skip slots whose (hi, lo) differ.
+__device__ __forceinline__ void rehash_slot(const Slot& s, Slot* tab, uint64_t mask, uint64_t keep_below,
+                                            unsigned int* fail) {
+    if (s.hi == kEmpty || s.lo == kEmpty || s.ord >= keep_below) return;
+    uint64_t j = slot_hash(s.hi, s.lo) & mask;
+    for (int n = 0; n < kProbeLimit; ++n, j = (j + 1) & mask) {
+        if (ld_any(&tab[j].hi) != kEmpty) continue;
+        if (atomicCAS(&tab[j].hi, kEmpty, s.hi) == kEmpty) {
+            tab[j].lo = s.lo;
+            tab[j].ord = s.ord;
+            return;
+        }
+    }
+    atomicOr(fail, 1u);
+}
+
 __global__ void k_rehash(const Slot* __restrict__ old, uint64_t old_cap, Slot* tab, uint64_t mask,
                          uint64_t keep_below, unsigned int* fail) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < old_cap; i += stride) {
-        const Slot s = old[i];
-        if (s.hi == kEmpty || s.lo == kEmpty || s.ord >= keep_below) continue;
-        uint64_t j = slot_hash(s.hi, s.lo) & mask;
-        int n = 0;
-        for (; n < kProbeLimit; ++n, j = (j + 1) & mask) {
-            if (ld_any(&tab[j].hi) != kEmpty) continue;
-            if (atomicCAS(&tab[j].hi, kEmpty, s.hi) == kEmpty) {
-                tab[j].lo = s.lo;
-                tab[j].ord = s.ord;
-                break;
-            }
-        }
-        if (n == kProbeLimit) atomicOr(fail, 1u);
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < old_cap; i += stride)
+        rehash_slot(old[i], tab, mask, keep_below, fail);
+}
+
+// the rehashes of many instances' tables in one launch
+__global__ void k_rehash_multi(const RehashSeg* __restrict__ segs, int n, uint64_t total, unsigned int* fail) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
+        const RehashSeg& sg = segs[seg_of(segs, n, g)];
+        rehash_slot(sg.old[g - sg.begin], sg.tab, sg.mask, sg.keep_below, fail);
+    }
+}
+
+// fills and copies of many buffers in one launch
+__global__ void k_fill_multi(const OpSeg* __restrict__ segs, int n, uint64_t total) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
+        const OpSeg& sg = segs[seg_of(segs, n, g)];
+        const uint64_t i = g - sg.begin;
+        sg.dst[i] = sg.src ? sg.src[i] : sg.value;
     }
 }
 
@@ -899,6 +945,24 @@ hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint6
         hipLaunchKernelGGL(k_expand<kWideLanes>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
     else
         hipLaunchKernelGGL(k_expand<1>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_atoms_multi(const InstDesc* D, const Seg* segs, int nseg, uint64_t total, unsigned int* fail,
+                                     hipStream_t st) {
+    hipLaunchKernelGGL(k_insert_atoms_multi, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, st, D, segs, nseg, total,
+                       fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_multi(const OpSeg* segs, int n, uint64_t total, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_multi, dim3(grid), dim3(256), 0, st, segs, n, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_rehash_multi(const RehashSeg* segs, int n, uint64_t total, unsigned int* fail, int grid,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_rehash_multi, dim3(grid), dim3(256), 0, st, segs, n, total, fail);
     return hipGetLastError();
 }
 
