@@ -88,7 +88,17 @@ typedef struct {
                               (85% of the HBM free at first use) */
     uint64_t max_rows;     /* cap on stored CS rows, the reference's LTLCacheCapacity
                               (LTL_BS.cu:1182); 0 = limited by hbm_budget only */
+    int search;            /* VFB_SEARCH_CS (default): the CS enumeration of LTL_BS;
+                              VFB_SEARCH_BRUTE: ltl_nocache's syntactic brute force
+                              (formulas in its numbering order, each evaluated from
+                              scratch, ltl_nocache.cu:78-287, 319-446): the same
+                              formula text as ltl_nocache; candidates[s] counts the
+                              formulas checked, distinct[] stays 0 */
+    int reserved;
 } vfb_options;
+
+#define VFB_SEARCH_CS 0
+#define VFB_SEARCH_BRUTE 1
 
 typedef struct {
     vfb_status status;

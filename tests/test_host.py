@@ -166,3 +166,18 @@ def test_tuning_file_roundtrip(tmp_path):
     path.write_text("expand_blocks_per_cu 0\n")
     with pytest.raises(V.VfbError):
         V.load_tuning(path)
+
+
+def test_cli_brute_search_env_accepted():
+    """VFB_SEARCH=brute selects ltl_nocache's search; without a GPU the CLI
+    reports the device error on stderr (no CPU fallback) and exits 0."""
+    env = dict(os.environ, VFB_SEARCH="brute", VFB_QUIET="1", HIP_VISIBLE_DEVICES="")
+    p = subprocess.run([V.CLI_PATH, os.path.join(GOLDEN, "ExampleLTLF.json"), "4"], capture_output=True,
+                       text=True, env=env, timeout=120)
+    assert p.returncode == 0
+    assert "device error" in p.stderr and p.stdout == ""
+
+
+def test_options_have_search_field():
+    o = V._opts(10, 3, False, 0, False, 0, V.SEARCH_BRUTE)
+    assert o.search == V.SEARCH_BRUTE and o.max_size == 10

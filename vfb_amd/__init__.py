@@ -23,7 +23,11 @@ OP_NAMES = ["~", "&", "|", "X", "F", "G", "U", "atom"]
 
 class Options(C.Structure):
     _fields_ = [("max_size", C.c_int), ("relaxed_type", C.c_int), ("verbose", C.c_int),
-                ("timing", C.c_int), ("hbm_budget", C.c_uint64), ("max_rows", C.c_uint64)]
+                ("timing", C.c_int), ("hbm_budget", C.c_uint64), ("max_rows", C.c_uint64),
+                ("search", C.c_int), ("reserved", C.c_int)]
+
+
+SEARCH_CS, SEARCH_BRUTE = 0, 1
 
 
 class Result(C.Structure):
@@ -208,9 +212,10 @@ class Instance:
             raise VfbError(f"write_json: {status_string(st)}")
 
 
-def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0):
+def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0, search=SEARCH_CS):
     o = Options()
     lib().vfb_options_default(C.byref(o))
+    o.search = search
     o.max_size = max_size
     o.relaxed_type = relaxed_type
     # True / 1: every kernel class; 2: the expand kernel only (fewer events)
@@ -242,9 +247,11 @@ def _todict(r):
     }
 
 
-def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbose=False, max_rows=0):
-    """Plain single-instance solve (the drop-in for LTLI)."""
-    o = _opts(max_size, relaxed_type, timing, budget, verbose, max_rows)
+def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbose=False, max_rows=0,
+          search=SEARCH_CS):
+    """Plain single-instance solve (the drop-in for LTLI; search=SEARCH_BRUTE
+    runs ltl_nocache's syntactic brute force instead)."""
+    o = _opts(max_size, relaxed_type, timing, budget, verbose, max_rows, search)
     r = Result()
     lib().vfb_solve(inst.handle, C.byref(o), C.byref(r))
     try:

@@ -35,6 +35,8 @@ int main(int argc, char** argv) {
     opt.max_size = max_size;
     opt.verbose = std::getenv("VFB_QUIET") ? 0 : 1;
     if (const char* rt = std::getenv("VFB_RELAXED_TYPE")) opt.relaxed_type = std::atoi(rt);
+    // VFB_SEARCH=brute: ltl_nocache's own syntactic search (its formula, its counters)
+    if (const char* se = std::getenv("VFB_SEARCH")) opt.search = std::strcmp(se, "brute") == 0 ? VFB_SEARCH_BRUTE : VFB_SEARCH_CS;
     vfb_result res;
     vfb_status st = vfb_solve(inst, &opt, &res);
     if (st == VFB_INVALID_INPUT || st == VFB_DEVICE_ERROR) {

@@ -1112,6 +1112,26 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
         run.push_back(&st[k]);
     }
     std::lock_guard<std::mutex> lk(g_mu);
+    if (opt.search == VFB_SEARCH_BRUTE) {
+        // ltl_nocache's syntactic search, one instance after another
+        vfb_status worst = VFB_OK;
+        for (auto* I : run) {
+            const vfb_status s = brute_solve(*I->in, I->max_size, opt.verbose, I->res);
+            if (s == VFB_DEVICE_ERROR) {
+                std::free(I->res->formula);
+                I->res->status = VFB_DEVICE_ERROR;
+                I->res->formula = strdup("see_the_error");
+                worst = VFB_DEVICE_ERROR;
+            }
+        }
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (size_t k = 0; k < n; ++k) {
+            out[k].ms_total = ms;
+            if (!out[k].formula) out[k].formula = strdup("Not found !");
+        }
+        if (worst == VFB_DEVICE_ERROR) return VFB_DEVICE_ERROR;
+        return n == 1 ? out[0].status : VFB_OK;
+    }
     try {
         Engine& eng = engine_acquire(opt.hbm_budget);
         uint64_t per = opt.hbm_budget ? std::min<uint64_t>(opt.hbm_budget, eng.arena_size()) : eng.arena_size();

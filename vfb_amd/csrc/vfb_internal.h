@@ -24,6 +24,9 @@ namespace vfb {
 // Validates limits; returns VFB_OK or VFB_INVALID_INPUT with a message.
 vfb_status validate(const vfb_instance& in, std::string* why);
 
+// ltl_nocache's syntactic brute-force search on the GPU (brute.hip)
+vfb_status brute_solve(const vfb_instance& in, int max_size, int verbose, vfb_result* R);
+
 inline uint64_t len_mask(int len) {
     return len >= 64 ? ~uint64_t(0) : ((uint64_t(1) << len) - 1);
 }
