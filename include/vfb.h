@@ -94,7 +94,12 @@ typedef struct {
                               scratch, ltl_nocache.cu:78-287, 319-446): the same
                               formula text as ltl_nocache; candidates[s] counts the
                               formulas checked, distinct[] stays 0 */
-    int reserved;
+    int bs_max_size;       /* > max_size: when nothing matches up to max_size (status
+                              VFB_NOT_FOUND, no row cap reached) and T <= 63, the BS stage
+                              of LTL_BS (LTL_BS.cu:808-1067, 1508-1525) searches Boolean
+                              combinations (~ & |) of the time-0 vectors of every stored
+                              CS up to this size (unit costs); 0 = off.  Only with
+                              VFB_SEARCH_CS, and not in the server */
 } vfb_options;
 
 #define VFB_SEARCH_CS 0
@@ -130,7 +135,9 @@ typedef struct {
     int otf_level;         /* level whose rows were truncated (onTheFly, LTL_BS.cu:516-522);
                               the next level was then checked only and the search
                               stopped (status VFB_CAPACITY if nothing matched); 0: none */
-    int pad;
+    int bs_stage;          /* 1: the formula was found by the BS stage (vfb_options.bs_max_size);
+                              distinct[] / candidates[] then continue past max_size with
+                              its levels */
     uint64_t otf_stored;   /* rows of otf_level that were stored */
     double ms_probe;       /* HIP-event time of the target-check probes of large levels
                               (vfb_tuning.pass_min); not part of ms_expand */

@@ -384,8 +384,141 @@ static int candidate(enum_ctx* c, const uint64_t* cs, int op, int l, int r)
     return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* BS (Boolean-set) stage: LTL_BS.cu:1508-1525 (transfer of bit 0 of every  */
+/* stored CS), 112-133 (dedup of those vectors), 162-181 (~ & | on one    */
+/* word), 351-381 and 870-1067 (levels by cost, target = positives mask). */
+/* Unit costs.  Q6 (| priced as &) and Q7 (one-block init) are not        */
+/* copied; level boundaries are taken after the dedup (the reference      */
+/* keeps the pre-dedup LTL start points); the lowest ordinal wins.        */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    uint64_t* val;
+    uint8_t* kind;      /* 0 leaf (a = stored CS row), 1 ~, 2 &, 3 | */
+    int64_t* a;
+    int64_t* b;
+    size_t n, cap;
+} bsnodes;
+
+static int bs_push(bsnodes* v, uint64_t val, int kind, int64_t a, int64_t b)
+{
+    if (v->n == v->cap) {
+        size_t nc = v->cap ? v->cap * 2 : 1024;
+        uint64_t* x = (uint64_t*)realloc(v->val, nc * 8);
+        if (!x) return 0;
+        v->val = x;
+        uint8_t* k = (uint8_t*)realloc(v->kind, nc);
+        if (!k) return 0;
+        v->kind = k;
+        int64_t* y = (int64_t*)realloc(v->a, nc * 8);
+        if (!y) return 0;
+        v->a = y;
+        y = (int64_t*)realloc(v->b, nc * 8);
+        if (!y) return 0;
+        v->b = y;
+        v->cap = nc;
+    }
+    v->val[v->n] = val; v->kind[v->n] = (uint8_t)kind; v->a[v->n] = a; v->b[v->n] = b;
+    v->n++;
+    return 1;
+}
+
+static void bs_text(const bsnodes* v, const rowstore* st, int64_t k, const char* const* names, sbuf* b)
+{
+    switch (v->kind[k]) {
+    case 0: row_to_string(st, (int)v->a[k], names, b); return;
+    case 1: sb_put(b, "~("); bs_text(v, st, v->a[k], names, b); sb_put(b, ")"); return;
+    default:
+        sb_put(b, "(");
+        bs_text(v, st, v->a[k], names, b);
+        sb_put(b, v->kind[k] == 2 ? ") & (" : ") | (");
+        bs_text(v, st, v->b[k], names, b);
+        sb_put(b, ")");
+    }
+}
+
+/* BS levels max_size+1 .. bs_max over the rows of levels 1..max_size
+   (start[l] = first row of level l); returns 1 if a formula was found */
+static int bs_stage(const ora_instance* in, const rowstore* st, const size_t* start, int max_size, int bs_max,
+                    const char* const* names, ora_result* out)
+{
+    const int T = in->T, P = in->P;
+    const uint64_t tmask = T >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << T) - 1);
+    const uint64_t pmask = ((uint64_t)1 << P) - 1;
+    bsnodes v;
+    memset(&v, 0, sizeof v);
+    keyset seen;
+    if (!keyset_init(&seen, 1024)) return 0;
+    size_t bstart[ORA_MAX_SIZE + 2];
+    int found = 0;
+    /* the transferred vectors, first occurrence kept, level by level */
+    for (int l = 1; l <= max_size; ++l) {
+        bstart[l] = v.n;
+        for (size_t r = start[l]; r < start[l + 1]; ++r) {
+            uint64_t w = 0;
+            for (int t = 0; t < T; ++t) w |= (st->cs[r * T + t] & 1) << t;
+            if (keyset_insert(&seen, 0, w) == 1) bs_push(&v, w, 0, (int64_t)r, 0);
+        }
+    }
+    bstart[max_size + 1] = v.n;
+    for (int s = max_size + 1; s <= bs_max && !found; ++s) {
+        uint64_t ord = 0;
+        for (int kind = 1; kind <= 3 && !found; ++kind) {
+            const int nsplit = kind == 1 ? 1 : (s - 1) / 2;
+            for (int i = 1; i <= nsplit && !found; ++i) {
+                const int ll = kind == 1 ? s - 1 : i, lr = s - 1 - i;
+                const size_t l0 = bstart[ll], l1 = bstart[ll + 1];
+                const size_t r0 = kind == 1 ? 0 : bstart[lr], r1 = kind == 1 ? 1 : bstart[lr + 1];
+                for (size_t x = l0; x < l1 && !found; ++x)
+                    for (size_t y = r0; y < r1 && !found; ++y) {
+                        const uint64_t w = kind == 1 ? (~v.val[x] & tmask)
+                                         : kind == 2 ? (v.val[x] & v.val[y]) : (v.val[x] | v.val[y]);
+                        out->candidates[s]++;
+                        const uint64_t o = ord++;
+                        if (keyset_insert(&seen, 0, w) != 1) continue;
+                        if (w == pmask) {   /* BSInsertInCache, LTL_BS.cu:290-291 */
+                            bs_push(&v, w, kind, (int64_t)x, kind == 1 ? 0 : (int64_t)y);
+                            sbuf b = { out->formula, 0, sizeof out->formula, 0 };
+                            out->formula[0] = 0;
+                            bs_text(&v, st, (int64_t)v.n - 1, names, &b);
+                            out->min_size = s;
+                            out->top_op = kind == 1 ? ORA_NOT : kind == 2 ? ORA_AND : ORA_OR;
+                            out->found_ordinal = o;
+                            out->bs_stage = 1;
+                            found = 1;
+                            break;
+                        }
+                        out->distinct[s]++;
+                        bs_push(&v, w, kind, (int64_t)x, kind == 1 ? 0 : (int64_t)y);
+                    }
+            }
+        }
+        bstart[s + 1] = v.n;
+        if (!found) out->levels_done = s;
+    }
+    keyset_free(&seen);
+    free(v.val); free(v.kind); free(v.a); free(v.b);
+    return found;
+}
+
+static int enumerate_core(const ora_instance* in, int max_size, int key_mode, uint64_t max_rows, int bs_max,
+                          const char* const* names, ora_result* out);
+
 int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
                   uint64_t max_rows, const char* const* names, ora_result* out)
+{
+    return enumerate_core(in, max_size, key_mode, max_rows, 0, names, out);
+}
+
+int ora_enumerate_bs(const ora_instance* in, int max_size, int key_mode, int bs_max_size,
+                     const char* const* names, ora_result* out)
+{
+    return enumerate_core(in, max_size, key_mode, 0, bs_max_size, names, out);
+}
+
+static int enumerate_core(const ora_instance* in, int max_size, int key_mode, uint64_t max_rows, int bs_max,
+                          const char* const* names, ora_result* out)
 {
     memset(out, 0, sizeof *out);
     if (in->T < 1 || in->T > ORA_MAX_TRACES || in->A < 1 || in->A > ORA_MAX_ATOMS ||
@@ -489,6 +622,10 @@ int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
         }
     }
 
+    /* nothing up to max_size: Boolean combinations of the stored CS (BS) */
+    if (!c.done && !c.otf && bs_max > max_size && bs_max <= ORA_MAX_SIZE && T <= 63 &&
+        bs_stage(in, &c.st, start, max_size, bs_max, names, out))
+        c.done = 1;
     free(cs);
     keyset_free(&c.ks);
     free(c.rs.idx);

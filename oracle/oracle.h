@@ -49,7 +49,7 @@ typedef struct {
     uint64_t candidates[ORA_MAX_SIZE + 1]; /* candidates generated per level (found level: through the match) */
     uint64_t found_ordinal; /* level-local ordinal of the returned formula */
     int otf_level;          /* level whose rows were truncated at max_rows (0: none) */
-    int pad;
+    int bs_stage;           /* 1: the formula was found by the BS stage */
     uint64_t otf_stored;    /* rows of that level that were stored */
     char formula[8192];     /* ltl_nocache.cu:289-317 syntax */
 } ora_result;
@@ -70,6 +70,13 @@ void ora_key(const uint64_t* cs, const int* len, int T, int relaxed_type,
    LTL_BS.cu:1229-1233, 1490-1491) with status 3 if nothing matched. */
 int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
                   uint64_t max_rows, const char* const* names, ora_result* out);
+/* ora_enumerate without a row cap, then, if nothing matched up to max_size
+   and T <= 63, the BS stage over every stored CS up to bs_max_size
+   (LTL_BS.cu:808-1067, 1508-1525; unit costs; Q6/Q7 not copied; level
+   boundaries after the dedup; the lowest ordinal wins).  BS levels are
+   reported in distinct[] / candidates[] at sizes max_size+1 .. */
+int ora_enumerate_bs(const ora_instance* in, int max_size, int key_mode, int bs_max_size,
+                     const char* const* names, ora_result* out);
 int ora_bruteforce(const ora_instance* in, int max_size,
                    const char* const* names, ora_result* out);
 /* multi-core CPU build of ora_enumerate (cpu_omp.c, OpenMP): same results

@@ -27,7 +27,7 @@ class OraResult(C.Structure):
                 ("distinct", C.c_uint64 * (MAX_SIZE + 1)),
                 ("candidates", C.c_uint64 * (MAX_SIZE + 1)),
                 ("found_ordinal", C.c_uint64),
-                ("otf_level", C.c_int), ("pad", C.c_int), ("otf_stored", C.c_uint64),
+                ("otf_level", C.c_int), ("bs_stage", C.c_int), ("otf_stored", C.c_uint64),
                 ("formula", C.c_char * 8192)]
 
 
@@ -69,6 +69,9 @@ def lib():
         _lib.ora_enumerate.restype = C.c_int
         _lib.ora_enumerate.argtypes = [C.POINTER(OraInstance), C.c_int, C.c_int, C.c_uint64,
                                        C.POINTER(C.c_char_p), C.POINTER(OraResult)]
+        _lib.ora_enumerate_bs.restype = C.c_int
+        _lib.ora_enumerate_bs.argtypes = [C.POINTER(OraInstance), C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_char_p), C.POINTER(OraResult)]
         _lib.ora_bruteforce.restype = C.c_int
         _lib.ora_bruteforce.argtypes = [C.POINTER(OraInstance), C.c_int,
                                         C.POINTER(C.c_char_p), C.POINTER(OraResult)]
@@ -128,7 +131,7 @@ def _result(r):
         "levels_done": r.levels_done,
         "distinct": list(r.distinct), "candidates": list(r.candidates),
         "found_ordinal": r.found_ordinal,
-        "otf_level": r.otf_level, "otf_stored": r.otf_stored,
+        "otf_level": r.otf_level, "otf_stored": r.otf_stored, "bs_stage": r.bs_stage,
         "formula": r.formula.decode(),
     }
 
@@ -153,6 +156,14 @@ def enumerate_omp(inst, max_size, key_mode=3, pass_min=1 << 22):
 
 def omp_threads():
     return omp_lib().ora_omp_threads()
+
+
+def enumerate_bs(inst, max_size, bs_max_size, key_mode=3):
+    """CS enumeration, then the BS stage up to bs_max_size if nothing matched."""
+    ci, keep, names = inst._c()
+    r = OraResult()
+    lib().ora_enumerate_bs(C.byref(ci), max_size, key_mode, bs_max_size, names, C.byref(r))
+    return _result(r)
 
 
 def bruteforce(inst, max_size):

@@ -276,3 +276,97 @@ def test_omp_build_on_example_and_exact_keys():
     assert (got["formula"], got["distinct"], got["candidates"]) == (want["formula"], want["distinct"],
                                                                      want["candidates"])
     assert O.omp_threads() >= 1
+
+
+# ---- BS stage ---------------------------------------------------------------
+
+def _all_trees(A, size, memo):
+    """Every syntax tree of the grammar {atoms, ~ X F G, & | U} of a size."""
+    if size in memo:
+        return memo[size]
+    out = []
+    if size == 1:
+        out = [("atom", a) for a in range(A)]
+    else:
+        for c in _all_trees(A, size - 1, memo):
+            out += [(op, c) for op in ("~", "X", "F", "G")]
+        for i in range(1, size - 1):
+            for l in _all_trees(A, i, memo):
+                for r in _all_trees(A, size - 1 - i, memo):
+                    out += [(op, l, r) for op in ("&", "|", "U")]
+    memo[size] = out
+    return out
+
+
+def _bs_min_cost(inst, max_size, bs_max):
+    """Independent minimal cost of a Boolean combination (~ & |, unit costs)
+    of formulas of size <= max_size whose time-0 vector over the traces is
+    the positive mask: leaves from textbook semantics, then a fixpoint over
+    the 2^T vectors."""
+    from ltl_semantics import eval_tree
+    T, P, A = inst.T, inst.P, inst.A
+    full = (1 << T) - 1
+    best = {}
+    memo = {}
+    for s in range(1, max_size + 1):
+        for tree in _all_trees(A, s, memo):
+            v = 0
+            for t in range(T):
+                if eval_tree(tree, [inst.atoms[a][t] for a in range(A)], inst.lens[t])[0]:
+                    v |= 1 << t
+            best.setdefault(v, s)
+    changed = True
+    while changed:
+        changed = False
+        items = list(best.items())
+        for v, c in items:
+            nv, nc = ~v & full, c + 1
+            if nc <= bs_max and best.get(nv, 99) > nc:
+                best[nv] = nc
+                changed = True
+        for v, c in items:
+            for w, d in items:
+                for nv in (v & w, v | w):
+                    if c + d + 1 <= bs_max and best.get(nv, 99) > c + d + 1:
+                        best[nv] = c + d + 1
+                        changed = True
+    return best.get((1 << P) - 1)
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_bs_stage_min_cost_and_formula(seed):
+    """The BS stage's size equals an independent fixpoint over the time-0
+    vectors, and its formula (an LTL formula: ~ & | over LTL subformulas)
+    separates the traces with exactly that size."""
+    rng = random.Random(7100 + seed)
+    A = rng.randint(1, 2)
+    T = rng.randint(3, 6)
+    lens = [rng.randint(1, 4) for _ in range(T)]
+    atoms = [[rng.randrange(1 << lens[t]) for t in range(T)] for _ in range(A)]
+    inst = O.Instance([f"p{a}" for a in range(A)], lens, atoms, rng.randint(1, T - 1))
+    max_size, bs_max = 2, 7
+    plain = O.enumerate_cs(inst, max_size, 3)
+    got = O.enumerate_bs(inst, max_size, bs_max, 3)
+    want = _bs_min_cost(inst, max_size, bs_max)
+    if plain["status"] == 0:
+        assert got["formula"] == plain["formula"] and not got["bs_stage"]
+        return
+    if want is None:
+        assert got["status"] == 1 and got["formula"] == "Not found !"
+        return
+    assert got["status"] == 0 and got["bs_stage"] == 1
+    assert got["min_size"] == want, (seed, got["min_size"], want, got["formula"])
+    assert separates(got["formula"], inst.names, inst.lens, inst.atoms, inst.P)
+    assert size(parse(got["formula"], inst.names)) == want
+    # levels up to max_size are the CS stage's own
+    assert got["distinct"][:max_size + 1] == plain["distinct"][:max_size + 1]
+
+
+def test_bs_stage_off_when_traces_exceed_63():
+    rng = random.Random(5)
+    T = 64
+    lens = [3] * T
+    atoms = [[rng.randrange(8) for _ in range(T)]]
+    inst = O.Instance(["p0"], lens, atoms, 32)
+    got = O.enumerate_bs(inst, 2, 6, 3)
+    assert got["bs_stage"] == 0

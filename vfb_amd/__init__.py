@@ -24,7 +24,7 @@ OP_NAMES = ["~", "&", "|", "X", "F", "G", "U", "atom"]
 class Options(C.Structure):
     _fields_ = [("max_size", C.c_int), ("relaxed_type", C.c_int), ("verbose", C.c_int),
                 ("timing", C.c_int), ("hbm_budget", C.c_uint64), ("max_rows", C.c_uint64),
-                ("search", C.c_int), ("reserved", C.c_int)]
+                ("search", C.c_int), ("bs_max_size", C.c_int)]
 
 
 SEARCH_CS, SEARCH_BRUTE = 0, 1
@@ -44,7 +44,7 @@ class Result(C.Structure):
                 ("expand_launches", C.c_uint64), ("table_slots", C.c_uint64),
                 ("retries", C.c_uint64), ("bytes_expand", C.c_uint64),
                 ("bytes_store", C.c_uint64), ("fresh_total", C.c_uint64),
-                ("otf_level", C.c_int), ("pad", C.c_int), ("otf_stored", C.c_uint64),
+                ("otf_level", C.c_int), ("bs_stage", C.c_int), ("otf_stored", C.c_uint64),
                 ("ms_probe", C.c_double), ("probe_candidates", C.c_uint64)]
 
 
@@ -212,10 +212,11 @@ class Instance:
             raise VfbError(f"write_json: {status_string(st)}")
 
 
-def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0, search=SEARCH_CS):
+def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0, search=SEARCH_CS, bs_max_size=0):
     o = Options()
     lib().vfb_options_default(C.byref(o))
     o.search = search
+    o.bs_max_size = bs_max_size
     o.max_size = max_size
     o.relaxed_type = relaxed_type
     # True / 1: every kernel class; 2: the expand kernel only (fewer events)
@@ -242,16 +243,17 @@ def _todict(r):
         "expand_launches": r.expand_launches, "table_slots": r.table_slots,
         "retries": r.retries, "bytes_expand": r.bytes_expand,
         "bytes_store": r.bytes_store, "fresh_total": r.fresh_total,
-        "otf_level": r.otf_level, "otf_stored": r.otf_stored,
+        "otf_level": r.otf_level, "otf_stored": r.otf_stored, "bs_stage": r.bs_stage,
         "ms_probe": r.ms_probe, "probe_candidates": r.probe_candidates,
     }
 
 
 def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbose=False, max_rows=0,
-          search=SEARCH_CS):
+          search=SEARCH_CS, bs_max_size=0):
     """Plain single-instance solve (the drop-in for LTLI; search=SEARCH_BRUTE
-    runs ltl_nocache's syntactic brute force instead)."""
-    o = _opts(max_size, relaxed_type, timing, budget, verbose, max_rows, search)
+    runs ltl_nocache's syntactic brute force instead; bs_max_size enables the
+    BS stage after an unsuccessful search)."""
+    o = _opts(max_size, relaxed_type, timing, budget, verbose, max_rows, search, bs_max_size)
     r = Result()
     lib().vfb_solve(inst.handle, C.byref(o), C.byref(r))
     try:
@@ -260,10 +262,10 @@ def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbo
         lib().vfb_result_free(C.byref(r))
 
 
-def solve_batch(insts, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, max_rows=0):
+def solve_batch(insts, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, max_rows=0, bs_max_size=0):
     """Many instances stepped together; each result equals solve() alone."""
     n = len(insts)
-    o = _opts(max_size, relaxed_type, timing, budget, False, max_rows)
+    o = _opts(max_size, relaxed_type, timing, budget, False, max_rows, SEARCH_CS, bs_max_size)
     hs = (C.c_void_p * n)(*[i.handle.value for i in insts])
     rs = (Result * n)()
     lib().vfb_solve_batch(hs, n, C.byref(o), rs)
@@ -343,8 +345,9 @@ class Server:
     as slots and HBM budget free up, and running requests share launches."""
 
     def __init__(self, max_size=MAX_SIZE, relaxed_type=3, budget=0, max_active=256, request_budget=0,
-                 max_rows=0):
-        o = _opts(max_size, relaxed_type, False, request_budget, False, max_rows)
+                 max_rows=0, bs_max_size=0):
+        # the library refuses bs_max_size != 0 (the server steps CS levels only)
+        o = _opts(max_size, relaxed_type, False, request_budget, False, max_rows, SEARCH_CS, bs_max_size)
         h = C.c_void_p()
         st = lib().vfb_server_create(C.byref(o), int(budget), int(max_active), C.byref(h))
         if st != OK:

@@ -19,6 +19,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <unordered_set>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -396,10 +397,12 @@ void Engine::build_units(Inst& I) {
     add(VFB_OP_NOT, s - 1, 0);
     for (int i = 1; 2 * i <= s - 1; ++i) add(VFB_OP_AND, i, s - 1 - i);
     for (int i = 1; 2 * i <= s - 1; ++i) add(VFB_OP_OR, i, s - 1 - i);
-    add(VFB_OP_NEXT, s - 1, 0);
-    add(VFB_OP_FINALLY, s - 1, 0);
-    add(VFB_OP_GLOBALLY, s - 1, 0);
-    for (int i = 1; 2 * i <= s - 1; ++i) add(VFB_OP_UNTIL, i, s - 1 - i);
+    if (!I.bs_mode) {                  // the BS stage combines with ~ & | only
+        add(VFB_OP_NEXT, s - 1, 0);
+        add(VFB_OP_FINALLY, s - 1, 0);
+        add(VFB_OP_GLOBALLY, s - 1, 0);
+        for (int i = 1; 2 * i <= s - 1; ++i) add(VFB_OP_UNTIL, i, s - 1 - i);
+    }
     I.N = base;
 
     I.tasks.clear();
@@ -412,9 +415,11 @@ void Engine::build_units(Inst& I) {
         t.nr = 1;
         t.nchunk = 1;
         t.base[0] = U[0].base;                   // ~
-        t.base[1] = U[1 + 2 * nsplit].base;      // X
-        t.base[2] = U[2 + 2 * nsplit].base;      // F
-        t.base[3] = U[3 + 2 * nsplit].base;      // G
+        if (!I.bs_mode) {
+            t.base[1] = U[1 + 2 * nsplit].base;  // X
+            t.base[2] = U[2 + 2 * nsplit].base;  // F
+            t.base[3] = U[3 + 2 * nsplit].base;  // G
+        }
         I.tasks.push_back(t);
     }
     for (int i = 1; i <= nsplit; ++i) {
@@ -429,7 +434,7 @@ void Engine::build_units(Inst& I) {
         t.nchunk = (t.nr + 63) / 64;
         t.base[0] = U[i].base;                   // & split i
         t.base[1] = U[nsplit + i].base;          // | split i
-        t.base[2] = U[3 + 2 * nsplit + i].base;  // U split i
+        if (!I.bs_mode) t.base[2] = U[3 + 2 * nsplit + i].base;  // U split i
         I.tasks.push_back(t);
     }
 
@@ -576,10 +581,12 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
 }
 
 std::string Engine::node_string(Inst& I, int level, uint64_t row) {
-    if (level == 1) return I.in->names[row];
+    if (level == 1 && !I.bs_mode) return I.in->names[row];
     uint64_t o = 0;
     const uint64_t* par = (const uint64_t*)I.rows[level].p + (uint64_t)I.h.T * I.nrows[level];
     CK(hipMemcpy(&o, par + row, 8, hipMemcpyDeviceToHost));
+    // a BS leaf is the CS row it was taken from
+    if (I.bs_mode && level <= I.ltl_levels) return node_string(*I.ltl, level, o);
     return cand_string(I, level, o, nullptr);
 }
 
@@ -608,6 +615,7 @@ void Engine::finish(Inst& I, vfb_status st) {
     I.res->status = st;
     if (!I.res->formula) I.res->formula = strdup("Not found !");
     release_all(I);
+    if (I.ltl) release_all(*I.ltl);   // a BS stage's leaves are its CS instance's rows
 }
 
 void Engine::defer(Inst& I) {
@@ -615,6 +623,98 @@ void Engine::defer(Inst& I) {
     release_all(I);
     I.done = true;
     I.deferred = true;
+    if (I.ltl) {                       // the whole solve is redone, CS stage first
+        release_all(*I.ltl);
+        I.ltl->deferred = true;
+    }
+}
+
+// The BS stage of LTL_BS (LTL_BS.cu:808-1067, 1508-1525) for a CS instance
+// whose levels 1..max_size are stored and hold no match: bit 0 of every trace
+// of every stored row becomes one word (k_bs_words); the distinct words, in
+// level-then-row order (the first occurrence kept, LTL_BS.cu:112-133), are
+// the leaf levels of a one-trace instance whose levels max_size+1.. are the
+// ~ & | combinations (unit costs; the | cost of Q6 not copied), deduplicated
+// exactly (key = the word) and matched when the word is the positives mask.
+void Engine::start_bs(Inst& I) {
+    const int L = I.level, T = I.h.T;
+    uint64_t total = 0;
+    for (int l = 1; l <= L; ++l) total += I.nrows[l];
+    Buf words;
+    dalloc(I, words, std::max<uint64_t>(total, 1) * 8);
+    uint64_t off = 0;
+    for (int l = 1; l <= L; ++l) {
+        CK(launch_bs_words((const unsigned long long*)I.rows[l].p, I.nrows[l], T,
+                           (unsigned long long*)words.p + off, st_));
+        off += I.nrows[l];
+    }
+    std::vector<uint64_t> w(total);
+    if (total) CK(hipMemcpyAsync(w.data(), words.p, total * 8, hipMemcpyDeviceToHost, st_));
+    sync();
+    dfree(I, words);
+    // first occurrence of each word, level by level (row order within a level)
+    auto B = std::make_unique<Inst>();
+    B->bs_in = std::make_unique<vfb_instance>();
+    B->bs_in->names = {"bs"};
+    B->bs_in->lens = {T};
+    B->in = B->bs_in.get();
+    B->res = I.res;
+    B->max_size = I.bs_max;
+    B->budget = I.budget;
+    B->prio = I.prio;
+    B->bs_mode = true;
+    B->ltl = &I;
+    B->ltl_levels = L;
+    std::memset(&B->h, 0, sizeof(B->h));
+    key_params(*B->bs_in, 3, B->h);      // one trace of T bits: the exact key is the word
+    B->h.T = 1;
+    B->h.P = 0;
+    B->h.bs_mode = 1;
+    B->h.bs_target = (1ull << I.in->P) - 1;
+    B->rows.assign(kMaxS + 2, Buf{});
+    B->nrows.assign(kMaxS + 2, 0);
+    B->units.assign(kMaxS + 2, {});
+    std::unordered_set<uint64_t> seen;
+    seen.reserve(total * 2 + 16);
+    std::vector<uint64_t> keys;
+    off = 0;
+    live_.push_back(B.get());
+    drop_live(live_, &I);
+    I.done = true;                      // the CS stage hands over; its rows stay
+    I.bs_pending = false;
+    Inst& b = *B;
+    I.bs = std::move(B);
+    try {
+        for (int l = 1; l <= L; ++l) {
+            std::vector<uint64_t> vals, src;
+            for (uint32_t r = 0; r < I.nrows[l]; ++r)
+                if (seen.insert(w[off + r]).second) {
+                    vals.push_back(w[off + r]);
+                    src.push_back(r);
+                }
+            off += I.nrows[l];
+            b.nrows[l] = (uint32_t)vals.size();
+            if (vals.empty()) continue;
+            // one column of words, then the parents: the CS row of each leaf
+            std::vector<uint64_t> blk(vals);
+            blk.insert(blk.end(), src.begin(), src.end());
+            dalloc(b, b.rows[l], blk.size() * 8);
+            upload(b, b.rows[l].p, blk.data(), blk.size());
+            keys.insert(keys.end(), vals.begin(), vals.end());
+        }
+        b.level = L;
+        b.stored = b.gbase = b.committed = keys.size();
+        ensure_table(b, keys.size() + 4096, false);
+        Buf kb;
+        dalloc(b, kb, std::max<size_t>(keys.size(), 1) * 8);
+        upload(b, kb.p, keys.data(), keys.size());
+        flush_ops();
+        CK(launch_insert_keys(b.h.table, b.h.table_mask, (const unsigned long long*)kb.p, keys.size(), d_flag_, st_));
+        flag_dirty_ = true;
+        release_after_sync(b, kb);
+    } catch (CapacityError& e) {
+        if (!on_capacity(b, e)) finish(b, VFB_CAPACITY);
+    }
 }
 
 // true when the instance leaves the batch (arena memory is shared there)
@@ -630,7 +730,8 @@ void Engine::epoch(std::vector<Inst*>& v) {
     // small epochs use several lanes per candidate (shorter serial trace loop)
     // tasks of each instance's current pass
     auto in_pass = [](const Inst& I, Task& t) {
-        if (I.pass == P_FUSED || I.pass == P_PROBE) t.part = SEL_ALL;
+        if (I.bs_mode) t.part = t.kind == 0 ? SEL_NOT : SEL_ANDOR;   // ~ & | only
+        else if (I.pass == P_FUSED || I.pass == P_PROBE) t.part = SEL_ALL;
         else t.part = t.kind == 0 ? I.usel : I.bsel;
         return t.part != 0;
     };
@@ -856,6 +957,7 @@ void Engine::step(std::vector<Inst*>& act) {
                     prepare_scratch(*I);
                 } catch (CapacityError& e) {
                     if (on_capacity(*I, e)) continue;
+                    if (I->bs_mode) { finish(*I, VFB_CAPACITY); continue; }
                     // no memory to deduplicate this level: check it only, then stop
                     I->check_only = true;
                     build_units(*I);
@@ -873,7 +975,8 @@ void Engine::step(std::vector<Inst*>& act) {
         // list overflowed is grown and built again from its first pass ----
         for (auto* I : step) {
             // large deduplicating levels are probed first
-            const bool probe = !I->check_only && tun_.pass_min && I->N >= tun_.pass_min && nsplit_of(*I) >= 1;
+            const bool probe = !I->check_only && !I->bs_mode && tun_.pass_min && I->N >= tun_.pass_min &&
+                               nsplit_of(*I) >= 1;
             I->pass = probe ? P_PROBE : P_FUSED;
             I->usel = I->bsel = SEL_ALL;
             I->evaluated = 0;
@@ -911,6 +1014,7 @@ void Engine::step(std::vector<Inst*>& act) {
                         }
                     } catch (CapacityError& e) {
                         if (on_capacity(I, e)) continue;
+                        if (I.bs_mode) { finish(I, VFB_CAPACITY); continue; }
                         I.check_only = true;   // cannot grow: check the level only
                         build_units(I);
                         I.pass = P_FUSED;
@@ -987,6 +1091,7 @@ void Engine::step(std::vector<Inst*>& act) {
                 R.distinct[s] = I.ctr.found_rank;
                 R.min_size = s;
                 R.found_ordinal = I.ctr.found;
+                R.bs_stage = I.bs_mode ? 1 : 0;
                 std::string f = cand_string(I, s, I.ctr.found, &R.top_op);
                 R.formula = strdup(f.c_str());
                 finish(I, VFB_OK);
@@ -996,8 +1101,15 @@ void Engine::step(std::vector<Inst*>& act) {
             R.distinct[s] = I.ctr.winners;
             R.levels_done = s;
             if (s >= I.max_size) {
-                finish(I, VFB_NOT_FOUND);
-                continue;
+                // nothing up to max_size: the BS stage if asked for (its rows
+                // are this level's too), else the search ends
+                const bool bs = !I.bs_mode && I.bs_max > I.max_size && I.bs_max <= VFB_MAX_SIZE &&
+                                I.h.T <= 63 && R.otf_level == 0;
+                if (!bs) {
+                    finish(I, VFB_NOT_FOUND);
+                    continue;
+                }
+                I.bs_pending = true;
             }
             // rows to keep: all survivors, or as many as max_rows / the budget allow
             const uint64_t W = I.ctr.winners;
@@ -1022,6 +1134,11 @@ void Engine::step(std::vector<Inst*>& act) {
                 }
             }
             if (evicted) continue;
+            if (keep < W && (I.bs_pending || I.bs_mode)) {
+                // the BS stage needs every row; a BS level has no check-only round
+                finish(I, I.bs_mode ? VFB_CAPACITY : VFB_NOT_FOUND);
+                continue;
+            }
             if (keep < W) {
                 I.next_check_only = true;
                 R.otf_level = s;
@@ -1047,8 +1164,18 @@ void Engine::step(std::vector<Inst*>& act) {
         if (!st.empty()) store(st);
     }
     std::vector<Inst*> nxt;
-    for (auto* I : act)
+    for (auto* I : act) {
+        if (!I->done && I->bs_pending) {
+            try {
+                start_bs(*I);
+            } catch (CapacityError& e) {
+                if (!on_capacity(*I, e)) finish(*I, VFB_CAPACITY);
+            }
+            if (I->bs && !I->bs->done) nxt.push_back(I->bs.get());
+            continue;
+        }
         if (!I->done) nxt.push_back(I);
+    }
     act.swap(nxt);
 }
 
@@ -1108,6 +1235,7 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
         st[k].res = &out[k];
         st[k].max_size = opt.max_size;
         st[k].max_rows = opt.max_rows;
+        st[k].bs_max = opt.bs_max_size;
         st[k].prio = k;
         run.push_back(&st[k]);
     }
@@ -1145,6 +1273,7 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
             fresh.res = I->res;
             fresh.max_size = I->max_size;
             fresh.max_rows = I->max_rows;
+            fresh.bs_max = I->bs_max;
             fresh.budget = per;
             reset_result(fresh.res);
             std::vector<Inst*> one{&fresh};

@@ -32,6 +32,9 @@ hipError_t launch_resolve_scan1(const InstDesc* D, hipStream_t st);
 hipError_t launch_insert_atoms_multi(const InstDesc* D, const Seg* segs, int nseg, uint64_t total, unsigned int* fail,
                                      hipStream_t st);
 hipError_t launch_fill_multi(const OpSeg* segs, int n, uint64_t total, int grid, hipStream_t st);
+hipError_t launch_bs_words(const unsigned long long* rows, uint32_t n, int T, unsigned long long* out, hipStream_t st);
+hipError_t launch_insert_keys(Slot* tab, uint64_t mask, const unsigned long long* keys, uint64_t n, unsigned int* fail,
+                              hipStream_t st);
 hipError_t launch_rehash_multi(const RehashSeg* segs, int n, uint64_t total, unsigned int* fail, int grid,
                                hipStream_t st);
 hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch,
@@ -151,6 +154,15 @@ struct Inst {
     // match it is built in one fused pass; with one it is built operator
     // block by operator block in canonical order and stops at the block that
     // settles the match.
+    // BS stage (LTL_BS.cu:808-1067): Boolean combinations of this instance's
+    // stored time-0 vectors, run as a second instance over one-word rows
+    int bs_max = 0;             // vfb_options.bs_max_size
+    bool bs_mode = false;       // this instance is a BS stage
+    bool bs_pending = false;    // the CS search ended without a match: start the BS stage
+    Inst* ltl = nullptr;        // BS stage: the CS instance whose rows are its leaves
+    int ltl_levels = 0;         // BS stage: leaf levels 1..ltl_levels (parents = CS rows)
+    std::unique_ptr<Inst> bs;   // CS instance: its BS stage
+    std::unique_ptr<vfb_instance> bs_in;   // BS stage: its one-trace instance
     int pass = 0;               // next pass of the level (enum Pass in engine.cpp)
     uint8_t usel = 0xf, bsel = 0xf;     // operators of the pass: unary / binary task masks
     uint8_t first_usel = 0xf, first_bsel = 0xf;
@@ -220,6 +232,7 @@ class Engine {
     void prepare_scratch(Inst& I);
     void ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild);
     void finish(Inst& I, vfb_status st);
+    void start_bs(Inst& I);
     void defer(Inst& I);
     bool on_capacity(Inst& I, const CapacityError& e);
     std::string node_string(Inst& I, int level, uint64_t row);

@@ -69,7 +69,7 @@ struct InstDesc {
     int nunits;
     int check_only;   // onTheFly level: target check only, no keys, no storage
     int mark;         // resolve: 1 sets the winner bitmap (last pass), 0 only reports a match
-    int pad0;
+    int bs_mode;      // BS stage: one word per row, a match is the word == bs_target
     uint64_t lenmask[kMaxT];
     uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
     int16_t koff[kMaxT];        // first key bit of the trace's field / samples
@@ -81,6 +81,7 @@ struct InstDesc {
     uint64_t claim_limit;       // claims allowed this epoch before the set must grow
     uint64_t epoch_base;        // global ordinal of the level's first candidate
     uint64_t found_below;       // resolve: only matches below this level-local ordinal are reported
+    uint64_t bs_target;         // BS stage: the positive-traces mask
     unsigned long long* fresh_ord;
     uint32_t* fresh_slot;
     uint64_t fresh_cap;

@@ -76,6 +76,12 @@ struct Key {
     uint64_t hi, lo;
 };
 
+// 1 if candidate word c misses the target of its trace: bit 0 against the
+// trace's label, or (BS stage) the whole word against the positives mask
+__device__ __forceinline__ uint32_t miss(uint64_t c, uint64_t want, uint64_t cmask) {
+    return ((c ^ want) & cmask) != 0ull ? 1u : 0u;
+}
+
 // Folds trace t's CS word into the key (LTL_BS.cu:30-110).  Trace-uniform
 // parameters (offsets, widths, masks) were precomputed on the host.
 template <int KEY>
@@ -407,7 +413,8 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
             if (t >= T) break;
             const uint64_t a = av[j];
             const uint64_t m = d.lenmask[t];
-            const uint64_t want = t < d.P ? 1ull : 0ull;
+            const uint64_t want = d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
+            const uint64_t cmask = d.bs_mode ? ~0ull : 1ull;
             if constexpr (CHECK) {
                 // only bit 0 (time 0) matters: closed forms of the operators there
                 mm |= bit0_mismatch(~a & 1ull, (a >> 1) & 1ull, a != 0, a == m, want);
@@ -415,15 +422,14 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
             }
             const uint64_t c0 = ~a & m;                      // ~
             if constexpr ((SEL & 1) != 0) {
-                mm |= (uint32_t)((c0 ^ want) & 1ull);
+                mm |= miss(c0, want, cmask);
                 key_acc<KEY>(k[0], c0, d, t);
             }
             if constexpr ((SEL & 0xe) != 0) {
                 const uint64_t c1 = a >> 1;                  // X
                 const uint64_t c2 = suffix_or(a);            // F
                 const uint64_t c3 = a & ~suffix_or(c0);      // G
-                mm |= ((uint32_t)((c1 ^ want) & 1ull) << 1) | ((uint32_t)((c2 ^ want) & 1ull) << 2) |
-                      ((uint32_t)((c3 ^ want) & 1ull) << 3);
+                mm |= (miss(c1, want, cmask) << 1) | (miss(c2, want, cmask) << 2) | (miss(c3, want, cmask) << 3);
                 key_acc<KEY>(k[1], c1, d, t);
                 key_acc<KEY>(k[2], c2, d, t);
                 key_acc<KEY>(k[3], c3, d, t);
@@ -470,7 +476,8 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
             const int t = t0 + j * G;
             if (t >= T) break;
             const uint64_t a = av[j], b = bv[j];
-            const uint64_t want = t < d.P ? 1ull : 0ull;
+            const uint64_t want = d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
+            const uint64_t cmask = d.bs_mode ? ~0ull : 1ull;
             if constexpr (CHECK) {
                 // the probe and check-only levels evaluate every operator
                 mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
@@ -478,18 +485,18 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
             }
             if constexpr ((SEL & 1) != 0) {
                 const uint64_t c0 = a & b;
-                mm |= (uint32_t)((c0 ^ want) & 1ull);
+                mm |= miss(c0, want, cmask);
                 key_acc<KEY>(k[0], c0, d, t);
             }
             if constexpr ((SEL & 2) != 0) {
                 const uint64_t c1 = a | b;
-                mm |= (uint32_t)((c1 ^ want) & 1ull) << 1;
+                mm |= miss(c1, want, cmask) << 1;
                 key_acc<KEY>(k[1], c1, d, t);
             }
             if constexpr ((SEL & 0xc) != 0) {
                 const uint64_t c2 = until_op(a, b);
                 const uint64_t c3 = until_op(b, a);
-                mm |= ((uint32_t)((c2 ^ want) & 1ull) << 2) | ((uint32_t)((c3 ^ want) & 1ull) << 3);
+                mm |= (miss(c2, want, cmask) << 2) | (miss(c3, want, cmask) << 3);
                 key_acc<KEY>(k[2], c2, d, t);
                 key_acc<KEY>(k[3], c3, d, t);
             }
@@ -568,6 +575,27 @@ __device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned i
     uint32_t slot;
     int claimed = 0;
     if (probe_insert(d.table, d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed) == PR_FAIL) atomicOr(fail, 1u);
+}
+
+// BS stage: bit 0 of every trace of each stored row, packed into one word
+// (transfer, LTL_BS.cu:808-822); rows are T columns of n words
+__global__ void k_bs_words(const unsigned long long* __restrict__ rows, uint32_t n, int T,
+                           unsigned long long* __restrict__ out) {
+    const uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint64_t w = 0;
+    for (int t = 0; t < T; ++t) w |= (rows[(uint64_t)t * n + r] & 1ull) << t;
+    out[r] = w;
+}
+
+// distinct keys (hi = 0, lo = keys[i]) with ordinals 0..n-1 into a set
+__global__ void k_insert_keys(Slot* tab, uint64_t mask, const unsigned long long* __restrict__ keys, uint64_t n,
+                              unsigned int* fail) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t slot;
+    int claimed = 0;
+    if (probe_insert(tab, mask, 0ull, keys[i], i, 0, &slot, &claimed) == PR_FAIL) atomicOr(fail, 1u);
 }
 
 // the atoms of many instances in one launch (segment k = instance k's atoms)
@@ -952,6 +980,17 @@ hipError_t launch_insert_atoms_multi(const InstDesc* D, const Seg* segs, int nse
                                      hipStream_t st) {
     hipLaunchKernelGGL(k_insert_atoms_multi, dim3((unsigned)((total + 63) / 64)), dim3(64), 0, st, D, segs, nseg, total,
                        fail);
+    return hipGetLastError();
+}
+
+hipError_t launch_bs_words(const unsigned long long* rows, uint32_t n, int T, unsigned long long* out, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_bs_words, dim3((n + 255) / 256), dim3(256), 0, st, rows, n, T, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_keys(Slot* tab, uint64_t mask, const unsigned long long* keys, uint64_t n, unsigned int* fail,
+                              hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_insert_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, tab, mask, keys, n, fail);
     return hipGetLastError();
 }
 
