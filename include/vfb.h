@@ -51,6 +51,12 @@ typedef struct vfb_instance vfb_instance;
    ltl_nocache.cu:35-76).  Positive traces come first. */
 vfb_status vfb_instance_load_json(const char* path, vfb_instance** out);
 
+/* Reads LTL_BS's text trace format (replaces readFile, LTL_BS.cu:1535-1607):
+   positive traces, "---", negative traces, "---", then the alphabet line;
+   a trace is steps separated by ';', a step the 0/1 values of the alphabet's
+   letters separated by ','.  APs are the sorted letters. */
+vfb_status vfb_instance_load_text(const char* path, vfb_instance** out);
+
 /* Builds an instance from bit-packed traces: atoms[a * T + t] bit k is the
    value of AP a at step k of trace t; traces 0..num_pos-1 are positive
    (the layout the reference builds at LTL_BS.cu:1127-1163). */
@@ -100,6 +106,13 @@ typedef struct {
                               combinations (~ & |) of the time-0 vectors of every stored
                               CS up to this size (unit costs); 0 = off.  Only with
                               VFB_SEARCH_CS, and not in the server */
+    uint16_t cost[8];      /* LTL_BS's cost function (LTL_BS.cu:1085-1493, its CLI c1..c8):
+                              costs of p ~ & | X F G U; the levels are then costs and
+                              max_size is maxCost.  All 0 = unit costs (formula size).
+                              Non-unit costs: no probe, no onTheFly round (a capped
+                              level ends the search with VFB_CAPACITY), no BS stage */
+    int neg_type;          /* LTL_BS's NegType: 1 (or 0) ~ of any formula, 2 ~ of atoms only */
+    int pad_opt;
 } vfb_options;
 
 #define VFB_SEARCH_CS 0

@@ -502,23 +502,34 @@ static int bs_stage(const ora_instance* in, const rowstore* st, const size_t* st
     return found;
 }
 
+static const int UNIT_COST[8] = { 1, 1, 1, 1, 1, 1, 1, 1 };
+
 static int enumerate_core(const ora_instance* in, int max_size, int key_mode, uint64_t max_rows, int bs_max,
-                          const char* const* names, ora_result* out);
+                          const int* cost, int neg_type, const char* const* names, ora_result* out);
 
 int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
                   uint64_t max_rows, const char* const* names, ora_result* out)
 {
-    return enumerate_core(in, max_size, key_mode, max_rows, 0, names, out);
+    return enumerate_core(in, max_size, key_mode, max_rows, 0, UNIT_COST, 1, names, out);
 }
 
 int ora_enumerate_bs(const ora_instance* in, int max_size, int key_mode, int bs_max_size,
                      const char* const* names, ora_result* out)
 {
-    return enumerate_core(in, max_size, key_mode, 0, bs_max_size, names, out);
+    return enumerate_core(in, max_size, key_mode, 0, bs_max_size, UNIT_COST, 1, names, out);
+}
+
+int ora_enumerate_costs(const ora_instance* in, int max_cost, int key_mode, const int* cost, int neg_type,
+                        const char* const* names, ora_result* out)
+{
+    for (int k = 0; k < 8; ++k)
+        if (cost[k] < 1) { memset(out, 0, sizeof *out); out->status = 2; return 2; }
+    if (neg_type != 1 && neg_type != 2) { memset(out, 0, sizeof *out); out->status = 2; return 2; }
+    return enumerate_core(in, max_cost, key_mode, 0, 0, cost, neg_type, names, out);
 }
 
 static int enumerate_core(const ora_instance* in, int max_size, int key_mode, uint64_t max_rows, int bs_max,
-                          const char* const* names, ora_result* out)
+                          const int* cost, int neg_type, const char* const* names, ora_result* out)
 {
     memset(out, 0, sizeof *out);
     if (in->T < 1 || in->T > ORA_MAX_TRACES || in->A < 1 || in->A > ORA_MAX_ATOMS ||
@@ -534,20 +545,26 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
 
     uint64_t* cs = (uint64_t*)malloc(T * 8);
     size_t start[ORA_MAX_SIZE + 2];
+    /* level = cost (LTL_BS.cu:1085-1493): p c1, ~ c2, & c3, | c4, X c5, F c6,
+       G c7, U c8; unit costs make it the formula size */
+    const int c1 = cost[0], c2 = cost[1], c3 = cost[2], c4 = cost[3];
+    const int c5 = cost[4], c6 = cost[5], c7 = cost[6], c8 = cost[7];
+    const int unit = c1 == 1 && c2 == 1 && c3 == 1 && c4 == 1 && c5 == 1 && c6 == 1 && c7 == 1 && c8 == 1;
+    if (c1 > max_size) { out->status = 2; free(cs); keyset_free(&c.ks); return 2; }
+    for (int l = 0; l <= c1; ++l) start[l] = 0;
 
-    /* level 1: every AP is stored (LTL_BS.cu:1127-1163) and its key inserted
+    /* level c1: every AP is stored (LTL_BS.cu:1127-1163) and its key inserted
        (hashSetsInitialisation, LTL_BS.cu:136-160, 1217) */
-    c.level = 1;
-    start[1] = 0;
+    c.level = c1;
     for (int a = 0; a < in->A && !c.done; ++a) {
         for (int t = 0; t < T; ++t) cs[t] = in->atoms[a * T + t] & ora_mask(in->len[t]);
-        out->candidates[1]++;
+        out->candidates[c1]++;
         c.local_ord++;
         if (matches_target(cs, T, in->P)) {
             char tmp[160];
             atom_name(a, names, tmp);
             snprintf(out->formula, sizeof out->formula, "%s", tmp);
-            out->min_size = 1; out->top_op = ORA_ATOM; out->found_ordinal = (uint64_t)a;
+            out->min_size = c1; out->top_op = ORA_ATOM; out->found_ordinal = (uint64_t)a;
             c.done = 1;
             break;
         }
@@ -559,29 +576,30 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
             keyset_insert(&c.ks, hi, lo);
         }
         rows_push(&c.st, cs, ORA_ATOM, a, 0);
-        out->distinct[1]++;
+        out->distinct[c1]++;
     }
-    if (!c.done) out->levels_done = 1;
-    start[2] = c.st.rows;
+    if (!c.done) out->levels_done = c1;
+    start[c1 + 1] = c.st.rows;
 
     int last_round = 0;
-    for (int s = 2; s <= max_size && !c.done; ++s) {
+    for (int s = c1 + 1; s <= max_size && !c.done; ++s) {
         if (c.otf) { last_round = 1; c.check_only = 1; }
         c.level = s;
         c.local_ord = 0;
-        const size_t u0 = start[s - 1], u1 = start[s];
         const uint64_t* CS = NULL;
 #define ROW(i) (c.st.cs + (size_t)(i) * T)
-        /* ~ over level s-1 (LTL_BS.cu:1236-1272, NegType 1) */
-        for (size_t x = u0; x < u1 && !c.done; ++x) {
-            CS = ROW(x);
-            for (int t = 0; t < T; ++t) cs[t] = ora_apply(ORA_NOT, CS[t], 0, in->len[t]);
-            candidate(&c, cs, ORA_NOT, (int)x, 0);
-        }
-        /* & then | over splits i <= s-1-i, left-major (LTL_BS.cu:1274-1346) */
-        for (int op = ORA_AND; op <= ORA_OR && !c.done; ++op)
-            for (int i = 1; 2 * i <= s - 1 && !c.done; ++i) {
-                int j = s - 1 - i;
+        /* ~ over level s-c2 (LTL_BS.cu:1236-1272; NegType 2: atoms only) */
+        if (s - c2 >= c1 && (neg_type == 1 || s - c2 == c1))
+            for (size_t x = start[s - c2]; x < start[s - c2 + 1] && !c.done; ++x) {
+                CS = ROW(x);
+                for (int t = 0; t < T; ++t) cs[t] = ora_apply(ORA_NOT, CS[t], 0, in->len[t]);
+                candidate(&c, cs, ORA_NOT, (int)x, 0);
+            }
+        /* & then | over splits c1 <= i, 2i <= s - cost, left-major (LTL_BS.cu:1274-1346) */
+        for (int op = ORA_AND; op <= ORA_OR && !c.done; ++op) {
+            const int co = op == ORA_AND ? c3 : c4;
+            for (int i = c1; 2 * i <= s - co && !c.done; ++i) {
+                int j = s - i - co;
                 for (size_t l = start[i]; l < start[i + 1] && !c.done; ++l)
                     for (size_t r = start[j]; r < start[j + 1] && !c.done; ++r) {
                         const uint64_t* L = ROW(l);
@@ -590,17 +608,21 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
                         candidate(&c, cs, op, (int)l, (int)r);
                     }
             }
-        /* X, F, G over level s-1 (LTL_BS.cu:1348-1451) */
-        for (int op = ORA_NEXT; op <= ORA_GLOBALLY && !c.done; ++op)
-            for (size_t x = u0; x < u1 && !c.done; ++x) {
+        }
+        /* X, F, G over levels s-c5, s-c6, s-c7 (LTL_BS.cu:1348-1451) */
+        for (int op = ORA_NEXT; op <= ORA_GLOBALLY && !c.done; ++op) {
+            const int lv = s - (op == ORA_NEXT ? c5 : op == ORA_FINALLY ? c6 : c7);
+            if (lv < c1) continue;
+            for (size_t x = start[lv]; x < start[lv + 1] && !c.done; ++x) {
                 CS = ROW(x);
                 for (int t = 0; t < T; ++t) cs[t] = ora_apply(op, CS[t], 0, in->len[t]);
                 candidate(&c, cs, op, (int)x, 0);
             }
+        }
         /* U over splits, each pair in both orientations (LTL_BS.cu:1453-1488,
            424-442, with every pair evaluated) */
-        for (int i = 1; 2 * i <= s - 1 && !c.done; ++i) {
-            int j = s - 1 - i;
+        for (int i = c1; 2 * i <= s - c8 && !c.done; ++i) {
+            int j = s - i - c8;
             for (size_t l = start[i]; l < start[i + 1] && !c.done; ++l)
                 for (size_t r = start[j]; r < start[j + 1] && !c.done; ++r) {
                     const uint64_t* L = ROW(l);
@@ -623,7 +645,7 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
     }
 
     /* nothing up to max_size: Boolean combinations of the stored CS (BS) */
-    if (!c.done && !c.otf && bs_max > max_size && bs_max <= ORA_MAX_SIZE && T <= 63 &&
+    if (!c.done && !c.otf && unit && bs_max > max_size && bs_max <= ORA_MAX_SIZE && T <= 63 &&
         bs_stage(in, &c.st, start, max_size, bs_max, names, out))
         c.done = 1;
     free(cs);

@@ -77,6 +77,11 @@ int ora_enumerate(const ora_instance* in, int max_size, int key_mode,
    reported in distinct[] / candidates[] at sizes max_size+1 .. */
 int ora_enumerate_bs(const ora_instance* in, int max_size, int key_mode, int bs_max_size,
                      const char* const* names, ora_result* out);
+/* ora_enumerate with LTL_BS's cost function (LTL_BS.cu:1085-1493): level =
+   cost, cost[8] = p ~ & | X F G U (all >= 1), neg_type 1 (~ of any formula)
+   or 2 (~ of atoms only); no row cap, no BS stage */
+int ora_enumerate_costs(const ora_instance* in, int max_cost, int key_mode, const int* cost, int neg_type,
+                        const char* const* names, ora_result* out);
 int ora_bruteforce(const ora_instance* in, int max_size,
                    const char* const* names, ora_result* out);
 /* multi-core CPU build of ora_enumerate (cpu_omp.c, OpenMP): same results

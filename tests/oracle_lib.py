@@ -72,6 +72,9 @@ def lib():
         _lib.ora_enumerate_bs.restype = C.c_int
         _lib.ora_enumerate_bs.argtypes = [C.POINTER(OraInstance), C.c_int, C.c_int, C.c_int,
                                           C.POINTER(C.c_char_p), C.POINTER(OraResult)]
+        _lib.ora_enumerate_costs.restype = C.c_int
+        _lib.ora_enumerate_costs.argtypes = [C.POINTER(OraInstance), C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int,
+                                             C.POINTER(C.c_char_p), C.POINTER(OraResult)]
         _lib.ora_bruteforce.restype = C.c_int
         _lib.ora_bruteforce.argtypes = [C.POINTER(OraInstance), C.c_int,
                                         C.POINTER(C.c_char_p), C.POINTER(OraResult)]
@@ -163,6 +166,14 @@ def enumerate_bs(inst, max_size, bs_max_size, key_mode=3):
     ci, keep, names = inst._c()
     r = OraResult()
     lib().ora_enumerate_bs(C.byref(ci), max_size, key_mode, bs_max_size, names, C.byref(r))
+    return _result(r)
+
+
+def enumerate_costs(inst, max_cost, costs, neg_type=1, key_mode=3):
+    """CS enumeration with LTL_BS's cost function (p ~ & | X F G U) and NegType."""
+    ci, keep, names = inst._c()
+    r = OraResult()
+    lib().ora_enumerate_costs(C.byref(ci), max_cost, key_mode, (C.c_int * 8)(*costs), neg_type, names, C.byref(r))
     return _result(r)
 
 

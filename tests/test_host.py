@@ -181,3 +181,48 @@ def test_cli_brute_search_env_accepted():
 def test_options_have_search_field():
     o = V._opts(10, 3, False, 0, False, 0, V.SEARCH_BRUTE)
     assert o.search == V.SEARCH_BRUTE and o.max_size == 10
+
+
+def _read_text_py(path):
+    """Independent restatement of readFile (LTL_BS.cu:1535-1607) in Python."""
+    lines = [l.rstrip("\r\n") for l in open(path)]
+    while lines and not lines[-1]:
+        lines.pop()
+    alpha = [c for c in lines[-1] if "a" <= c <= "z"]
+    letters = sorted(set(alpha))
+    body = lines[:-1]
+    cut = body.index("---")
+    pos, rest = body[:cut], body[cut + 1:]
+    neg = rest[:rest.index("---")] if "---" in rest else rest
+    traces = pos + neg
+    lens = [len(tr.split(";")) for tr in traces]
+    atoms = [[0] * len(traces) for _ in letters]
+    for t, tr in enumerate(traces):
+        for k, step in enumerate(tr.split(";")):
+            vals = step.split(",")
+            for j, v in enumerate(vals):
+                if v == "1":
+                    atoms[letters.index(alpha[j])][t] |= 1 << k
+    return letters, lens, atoms, len(pos)
+
+
+def test_text_reader_matches_restatement_and_json():
+    path = os.path.join(GOLDEN, "example_ltlbs.txt")
+    inst = V.Instance.from_text(path)
+    assert inst.arrays() == _read_text_py(path)
+    # the same traces as ExampleLTLF.json, with a0 -> a, a1 -> b
+    names, lens, atoms, P = V.Instance.from_json(os.path.join(GOLDEN, "ExampleLTLF.json")).arrays()
+    assert inst.arrays() == (["a", "b"], lens, atoms, P)
+
+
+def test_ltlbs_cli_argument_checks():
+    path = os.path.join(GOLDEN, "example_ltlbs.txt")
+    out = subprocess.run([V.CLI_PATH, path, "1", "1", "0", "1", "1", "1", "1", "1", "10", "3", "2", "1"],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "Argument number 4" in out.stdout
+    out = subprocess.run([V.CLI_PATH, path] + ["1"] * 8 + ["10", "3", "4", "3"], capture_output=True, text=True,
+                         timeout=60)
+    assert "RlxUnqChkTyp" in out.stdout and "NegType" in out.stdout
+    out = subprocess.run([V.CLI_PATH, path] + ["1"] * 8 + ["40", "3", "2", "1"], capture_output=True, text=True,
+                         timeout=60)
+    assert "at most 32" in out.stdout

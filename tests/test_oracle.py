@@ -370,3 +370,96 @@ def test_bs_stage_off_when_traces_exceed_63():
     inst = O.Instance(["p0"], lens, atoms, 32)
     got = O.enumerate_bs(inst, 2, 6, 3)
     assert got["bs_stage"] == 0
+
+
+# ---- cost function and NegType (LTL_BS's CLI) --------------------------------
+
+def _cost_min(inst, costs, neg_type, max_cost):
+    """Independent minimal cost of a separating formula: a semantic dynamic
+    program over per-position truth vectors (textbook semantics), levels by
+    cost (p c1, ~ c2, & c3, | c4, X c5, F c6, G c7, U c8; NegType 2: ~ of
+    atoms only), each vector kept at its lowest cost."""
+    c1, c2, c3, c4, c5, c6, c7, c8 = costs
+    T, A = inst.T, inst.A
+    code = {"~": 0, "&": 1, "|": 2, "X": 3, "F": 4, "G": 5, "U": 6}
+    seen = set()
+    lv = {}
+
+    def target(v):
+        return all((v[t] & 1) == (1 if t < inst.P else 0) for t in range(T))
+
+    def add(C, v):
+        if v in seen:
+            return False
+        seen.add(v)
+        lv.setdefault(C, []).append(v)
+        return target(v)
+
+    for a in range(A):
+        if add(c1, tuple(inst.atoms[a][t] & ((1 << inst.lens[t]) - 1) for t in range(T))):
+            return c1
+    for C in range(c1 + 1, max_cost + 1):
+        hit = False
+        cand = []
+        if C - c2 >= c1 and (neg_type == 1 or C - c2 == c1):
+            cand += [("~", x, None) for x in lv.get(C - c2, [])]
+        for op, co in (("&", c3), ("|", c4), ("U", c8)):
+            for i in range(c1, C - co - c1 + 1):
+                j = C - i - co
+                for x in lv.get(i, []):
+                    for y in lv.get(j, []):
+                        cand.append((op, x, y))
+        for op, co in (("X", c5), ("F", c6), ("G", c7)):
+            cand += [(op, x, None) for x in lv.get(C - co, [])] if C - co >= c1 else []
+        for op, x, y in cand:
+            v = tuple(pos_apply(code[op], x[t], y[t] if y else 0, inst.lens[t]) for t in range(T))
+            hit |= add(C, v)
+        if hit:
+            return C
+    return None
+
+
+def _formula_cost(tree, costs):
+    c1, c2, c3, c4, c5, c6, c7, c8 = costs
+    table = {"~": c2, "&": c3, "|": c4, "X": c5, "F": c6, "G": c7, "U": c8}
+    if tree[0] == "atom":
+        return c1
+    return table[tree[0]] + sum(_formula_cost(c, costs) for c in tree[1:])
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_costs_min_and_formula(seed):
+    rng = random.Random(6100 + seed)
+    A = rng.randint(1, 2)
+    T = rng.randint(2, 5)
+    lens = [rng.randint(1, 4) for _ in range(T)]
+    atoms = [[rng.randrange(1 << lens[t]) for t in range(T)] for _ in range(A)]
+    inst = O.Instance([f"p{a}" for a in range(A)], lens, atoms, rng.randint(1, T - 1))
+    costs = [rng.randint(1, 3) for _ in range(8)]
+    neg_type = 1 + seed % 2
+    max_cost = 9
+    got = O.enumerate_costs(inst, max_cost, costs, neg_type)
+    want = _cost_min(inst, costs, neg_type, max_cost)
+    if want is None:
+        assert got["status"] == 1, (seed, got["formula"])
+        return
+    assert got["status"] == 0 and got["min_size"] == want, (seed, costs, neg_type, got["min_size"], want)
+    tree = parse(got["formula"], inst.names)
+    assert separates(got["formula"], inst.names, inst.lens, inst.atoms, inst.P)
+    assert _formula_cost(tree, costs) == want
+    if neg_type == 2:
+        def negs_on_atoms(t):
+            return (t[0] != "~" or t[1][0] == "atom") and all(negs_on_atoms(c) for c in t[1:] if isinstance(c, tuple))
+        assert negs_on_atoms(tree), got["formula"]
+
+
+def test_unit_costs_equal_plain():
+    rng = random.Random(1)
+    for k in range(6):
+        T = rng.randint(2, 8)
+        lens = [rng.randint(1, 8) for _ in range(T)]
+        atoms = [[rng.randrange(1 << lens[t]) for t in range(T)] for _ in range(2)]
+        inst = O.Instance(["a", "b"], lens, atoms, rng.randint(1, T - 1))
+        a = O.enumerate_cs(inst, 7, 3)
+        b = O.enumerate_costs(inst, 7, [1] * 8, 1)
+        assert (a["formula"], a["distinct"], a["candidates"]) == (b["formula"], b["distinct"], b["candidates"])

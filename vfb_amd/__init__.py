@@ -24,7 +24,8 @@ OP_NAMES = ["~", "&", "|", "X", "F", "G", "U", "atom"]
 class Options(C.Structure):
     _fields_ = [("max_size", C.c_int), ("relaxed_type", C.c_int), ("verbose", C.c_int),
                 ("timing", C.c_int), ("hbm_budget", C.c_uint64), ("max_rows", C.c_uint64),
-                ("search", C.c_int), ("bs_max_size", C.c_int)]
+                ("search", C.c_int), ("bs_max_size", C.c_int),
+                ("cost", C.c_uint16 * 8), ("neg_type", C.c_int), ("pad_opt", C.c_int)]
 
 
 SEARCH_CS, SEARCH_BRUTE = 0, 1
@@ -59,7 +60,7 @@ TUNING_FIELDS = [f for f, _ in Tuning._fields_]
 
 # every symbol declared in include/vfb.h
 EXPORTS = [
-    "vfb_instance_load_json", "vfb_instance_from_bits", "vfb_instance_generate",
+    "vfb_instance_load_json", "vfb_instance_load_text", "vfb_instance_from_bits", "vfb_instance_generate",
     "vfb_instance_write_json", "vfb_instance_shape", "vfb_instance_copy",
     "vfb_instance_atom_name", "vfb_instance_planted", "vfb_instance_max_size",
     "vfb_instance_free", "vfb_options_default", "vfb_solve", "vfb_solve_batch",
@@ -83,6 +84,7 @@ def lib():
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.vfb_instance_load_json.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.vfb_instance_load_text.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.vfb_instance_from_bits.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int,
                                          C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(vp)]
     L.vfb_instance_generate.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
@@ -164,6 +166,15 @@ class Instance:
         return cls(h.value)
 
     @classmethod
+    def from_text(cls, path):
+        """LTL_BS's text trace format (readFile, LTL_BS.cu:1535-1607)."""
+        h = C.c_void_p()
+        st = lib().vfb_instance_load_text(os.fsencode(path), C.byref(h))
+        if st != OK:
+            raise VfbError(f"load_text({path}): {status_string(st)}")
+        return cls(h.value)
+
+    @classmethod
     def generate(cls, config, index):
         h = C.c_void_p()
         st = lib().vfb_instance_generate(int(config), int(index), C.byref(h))
@@ -212,11 +223,16 @@ class Instance:
             raise VfbError(f"write_json: {status_string(st)}")
 
 
-def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0, search=SEARCH_CS, bs_max_size=0):
+def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0, search=SEARCH_CS, bs_max_size=0,
+          costs=None, neg_type=1):
     o = Options()
     lib().vfb_options_default(C.byref(o))
     o.search = search
     o.bs_max_size = bs_max_size
+    if costs is not None:
+        for k in range(8):
+            o.cost[k] = int(costs[k])
+    o.neg_type = neg_type
     o.max_size = max_size
     o.relaxed_type = relaxed_type
     # True / 1: every kernel class; 2: the expand kernel only (fewer events)
@@ -249,11 +265,12 @@ def _todict(r):
 
 
 def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbose=False, max_rows=0,
-          search=SEARCH_CS, bs_max_size=0):
+          search=SEARCH_CS, bs_max_size=0, costs=None, neg_type=1):
     """Plain single-instance solve (the drop-in for LTLI; search=SEARCH_BRUTE
     runs ltl_nocache's syntactic brute force instead; bs_max_size enables the
-    BS stage after an unsuccessful search)."""
-    o = _opts(max_size, relaxed_type, timing, budget, verbose, max_rows, search, bs_max_size)
+    BS stage after an unsuccessful search; costs = LTL_BS's c1..c8 with
+    max_size as maxCost, neg_type its NegType)."""
+    o = _opts(max_size, relaxed_type, timing, budget, verbose, max_rows, search, bs_max_size, costs, neg_type)
     r = Result()
     lib().vfb_solve(inst.handle, C.byref(o), C.byref(r))
     try:
@@ -262,10 +279,11 @@ def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbo
         lib().vfb_result_free(C.byref(r))
 
 
-def solve_batch(insts, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, max_rows=0, bs_max_size=0):
+def solve_batch(insts, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, max_rows=0, bs_max_size=0,
+                costs=None, neg_type=1):
     """Many instances stepped together; each result equals solve() alone."""
     n = len(insts)
-    o = _opts(max_size, relaxed_type, timing, budget, False, max_rows, SEARCH_CS, bs_max_size)
+    o = _opts(max_size, relaxed_type, timing, budget, False, max_rows, SEARCH_CS, bs_max_size, costs, neg_type)
     hs = (C.c_void_p * n)(*[i.handle.value for i in insts])
     rs = (Result * n)()
     lib().vfb_solve_batch(hs, n, C.byref(o), rs)

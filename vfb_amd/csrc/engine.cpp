@@ -167,7 +167,17 @@ void Engine::dalloc(Inst& I, Buf& b, size_t bytes) {
         }
         if (!victim) {
             if (older_holds) throw CapacityError("device arena exhausted", true);
-            break;   // alone in the arena: the plain solve's limit applies
+            // buffers released since the last sync may free enough
+            if (!pending_free_.empty()) {
+                sync();
+                p = arena_->alloc(rb);
+                continue;
+            }
+            // a batch solves this instance again alone, as the plain solve
+            // would (its arena holds none of the others' holes); a server
+            // lets the oldest request go on under the plain limit
+            if (defer_alone_) throw CapacityError("device arena exhausted", true);
+            break;
         }
         sync();
         defer(*victim);
@@ -332,6 +342,7 @@ static void key_params(const vfb_instance& in, int relaxed_type, InstDesc& h) {
 void Engine::setup(Inst& I) {
     const vfb_instance& in = *I.in;
     const int A = in.A(), T = in.T();
+    const int c1 = I.cost[0];           // the atoms' level (their cost)
     std::memset(&I.h, 0, sizeof(I.h));
     I.h.T = T;
     I.h.P = in.P;
@@ -342,101 +353,127 @@ void Engine::setup(Inst& I) {
     vfb_result& R = *I.res;
     // level 1: the atoms, all stored, checked in order (LTL_BS.cu:1127-1163)
     for (int a = 0; a < A; ++a) {
-        R.candidates[1]++;
+        R.candidates[c1]++;
         bool ok = true;
         for (int t = 0; t < T && ok; ++t) {
             bool b = in.atoms[(size_t)a * T + t] & 1;
             if ((t < in.P) != b) ok = false;
         }
         if (ok) {
-            R.min_size = 1;
+            R.min_size = c1;
             R.top_op = VFB_OP_ATOM;
             R.found_ordinal = (uint64_t)a;
             R.formula = strdup(in.names[a].c_str());
-            R.level_winners[1] = (uint64_t)A;
-            R.level_candidates[1] = (uint64_t)A;
+            R.level_winners[c1] = (uint64_t)A;
+            R.level_candidates[c1] = (uint64_t)A;
             I.done = true;
             R.status = VFB_OK;
             return;
         }
-        R.distinct[1]++;
+        R.distinct[c1]++;
     }
-    R.level_winners[1] = (uint64_t)A;
-    R.level_candidates[1] = (uint64_t)A;
-    R.levels_done = 1;
+    R.level_winners[c1] = (uint64_t)A;
+    R.level_candidates[c1] = (uint64_t)A;
+    R.levels_done = c1;
     std::vector<uint64_t> col((size_t)A * T);
     for (int a = 0; a < A; ++a)
         for (int t = 0; t < T; ++t) col[(size_t)t * A + a] = in.atoms[(size_t)a * T + t] & len_mask(in.lens[t]);
-    dalloc(I, I.rows[1], col.size() * 8);
-    upload(I, I.rows[1].p, col.data(), col.size());
-    I.nrows[1] = (uint32_t)A;
+    dalloc(I, I.rows[c1], col.size() * 8);
+    upload(I, I.rows[c1].p, col.data(), col.size());
+    I.nrows[c1] = (uint32_t)A;
     I.stored = (uint64_t)A;
-    I.level = 1;
+    I.level = c1;
     I.gbase = (uint64_t)A;
 }
 
 // candidate units of the next level in canonical order, and fused tasks
 void Engine::build_units(Inst& I) {
     const int s = I.level + 1;
+    // LTL_BS's cost function (LTL_BS.cu:1236-1488): level = cost; p c1, ~ c2,
+    // & c3, | c4, X c5, F c6, G c7, U c8 (unit costs: level = size)
+    const int* c = I.cost;
+    const int c1 = c[0];
     auto& U = I.units[s];
     U.clear();
     uint64_t base = 0;
-    auto add = [&](int op, int ll, int lr) {
+    auto add = [&](int op, int ll, int lr, bool on) {
         Unit u{};
         u.op = op;
-        u.lvl_l = ll;
-        u.lvl_r = lr;
-        u.nl = I.nrows[ll];
-        u.nr = lr ? I.nrows[lr] : 1;
+        u.lvl_l = on ? ll : 0;
+        u.lvl_r = on ? lr : 0;
+        u.nl = on ? I.nrows[ll] : 0;
+        u.nr = on && lr ? I.nrows[lr] : 1;
         u.base = base;
         uint64_t pairs = (uint64_t)u.nl * (lr ? u.nr : 1);
         u.count = op == VFB_OP_UNTIL ? 2 * pairs : pairs;
         base += u.count;
         U.push_back(u);
+        return U.size() - 1;
     };
-    add(VFB_OP_NOT, s - 1, 0);
-    for (int i = 1; 2 * i <= s - 1; ++i) add(VFB_OP_AND, i, s - 1 - i);
-    for (int i = 1; 2 * i <= s - 1; ++i) add(VFB_OP_OR, i, s - 1 - i);
+    // canonical order: ~, & splits, | splits, X, F, G, U splits; an operator
+    // with no operand level at this cost keeps an empty unit
+    const int lnot = s - c[1];
+    const size_t q_not = add(VFB_OP_NOT, lnot, 0, lnot >= c1 && (I.neg_type != 2 || lnot == c1));
+    std::vector<size_t> q_and, q_or, q_until;
+    for (int i = c1; 2 * i <= s - c[2]; ++i) q_and.push_back(add(VFB_OP_AND, i, s - i - c[2], true));
+    for (int i = c1; 2 * i <= s - c[3]; ++i) q_or.push_back(add(VFB_OP_OR, i, s - i - c[3], true));
+    size_t q_un[4] = {q_not, SIZE_MAX, SIZE_MAX, SIZE_MAX};
     if (!I.bs_mode) {                  // the BS stage combines with ~ & | only
-        add(VFB_OP_NEXT, s - 1, 0);
-        add(VFB_OP_FINALLY, s - 1, 0);
-        add(VFB_OP_GLOBALLY, s - 1, 0);
-        for (int i = 1; 2 * i <= s - 1; ++i) add(VFB_OP_UNTIL, i, s - 1 - i);
+        const int op3[3] = {VFB_OP_NEXT, VFB_OP_FINALLY, VFB_OP_GLOBALLY};
+        for (int k = 0; k < 3; ++k) {
+            const int lv = s - c[4 + k];
+            q_un[1 + k] = add(op3[k], lv, 0, lv >= c1);
+        }
+        for (int i = c1; 2 * i <= s - c[7]; ++i) q_until.push_back(add(VFB_OP_UNTIL, i, s - i - c[7], true));
     }
     I.N = base;
 
+    // fused tasks: the operators that read the same operand rows
     I.tasks.clear();
-    const int nsplit = (s - 1) / 2;
-    if (I.nrows[s - 1]) {
-        Task t{};
-        t.kind = 0;
-        t.lvl_l = (uint8_t)(s - 1);
-        t.nl = I.nrows[s - 1];
-        t.nr = 1;
-        t.nchunk = 1;
-        t.base[0] = U[0].base;                   // ~
-        if (!I.bs_mode) {
-            t.base[1] = U[1 + 2 * nsplit].base;  // X
-            t.base[2] = U[2 + 2 * nsplit].base;  // F
-            t.base[3] = U[3 + 2 * nsplit].base;  // G
+    std::vector<Task> un;
+    for (int k = 0; k < 4; ++k) {      // ~ X F G by operand level
+        if (q_un[k] == SIZE_MAX || !U[q_un[k]].count) continue;
+        const Unit& u = U[q_un[k]];
+        Task* t = nullptr;
+        for (auto& x : un)
+            if (x.lvl_l == u.lvl_l) t = &x;
+        if (!t) {
+            un.push_back(Task{});
+            t = &un.back();
+            t->kind = 0;
+            t->lvl_l = (uint8_t)u.lvl_l;
+            t->nl = u.nl;
+            t->nr = 1;
+            t->nchunk = 1;
         }
-        I.tasks.push_back(t);
+        t->ops |= 1u << k;
+        t->base[k] = u.base;
     }
-    for (int i = 1; i <= nsplit; ++i) {
-        const int j = s - 1 - i;
-        if (!I.nrows[i] || !I.nrows[j]) continue;
-        Task t{};
-        t.kind = 1;
-        t.lvl_l = (uint8_t)i;
-        t.lvl_r = (uint8_t)j;
-        t.nl = I.nrows[i];
-        t.nr = I.nrows[j];
-        t.nchunk = (t.nr + 63) / 64;
-        t.base[0] = U[i].base;                   // & split i
-        t.base[1] = U[nsplit + i].base;          // | split i
-        if (!I.bs_mode) t.base[2] = U[3 + 2 * nsplit + i].base;  // U split i
-        I.tasks.push_back(t);
-    }
+    std::vector<Task> bin;
+    auto bin_add = [&](size_t q, uint32_t bits, int slot) {
+        const Unit& u = U[q];
+        if (!u.count) return;
+        Task* t = nullptr;
+        for (auto& x : bin)
+            if (x.lvl_l == u.lvl_l && x.lvl_r == u.lvl_r) t = &x;
+        if (!t) {
+            bin.push_back(Task{});
+            t = &bin.back();
+            t->kind = 1;
+            t->lvl_l = (uint8_t)u.lvl_l;
+            t->lvl_r = (uint8_t)u.lvl_r;
+            t->nl = u.nl;
+            t->nr = u.nr;
+            t->nchunk = (t->nr + 63) / 64;
+        }
+        t->ops |= bits;
+        t->base[slot] = u.base;
+    };
+    for (size_t q : q_and) bin_add(q, 0x1, 0);
+    for (size_t q : q_or) bin_add(q, 0x2, 1);
+    for (size_t q : q_until) bin_add(q, 0xc, 2);
+    I.tasks.insert(I.tasks.end(), un.begin(), un.end());
+    I.tasks.insert(I.tasks.end(), bin.begin(), bin.end());
 
     InstDesc& h = I.h;
     h.level = s;
@@ -548,10 +585,8 @@ void Engine::prepare_scratch(Inst& I) {
 // of completed epochs (ord < epoch_base)
 void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     uint64_t need = next_pow2(std::max<uint64_t>(4096, (uint64_t)(want_keys / 0.7) + 1));
-    // headroom while the set is small, so that a small plain solve rarely
-    // rehashes (a launch each); batched rehashes and clears share launches,
-    // where headroom would only add bytes to clear
-    if (!batch_ && need < (1ull << 24)) need = std::min<uint64_t>(need * 4, 1ull << 24);
+    // no headroom: plain, batched and served solves size their sets alike, so
+    // their HBM-budget decisions (and hence their results) agree
     if (need > (1ull << 32)) need = 1ull << 32;
     if (I.table.p && need <= I.cap && !force_rebuild) {
         I.h.claim_limit = I.cap * 7 / 10 > I.committed ? I.cap * 7 / 10 - I.committed : 0;
@@ -581,7 +616,7 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
 }
 
 std::string Engine::node_string(Inst& I, int level, uint64_t row) {
-    if (level == 1 && !I.bs_mode) return I.in->names[row];
+    if (level == I.cost[0] && !I.bs_mode) return I.in->names[row];
     uint64_t o = 0;
     const uint64_t* par = (const uint64_t*)I.rows[level].p + (uint64_t)I.h.T * I.nrows[level];
     CK(hipMemcpy(&o, par + row, 8, hipMemcpyDeviceToHost));
@@ -730,9 +765,11 @@ void Engine::epoch(std::vector<Inst*>& v) {
     // small epochs use several lanes per candidate (shorter serial trace loop)
     // tasks of each instance's current pass
     auto in_pass = [](const Inst& I, Task& t) {
-        if (I.bs_mode) t.part = t.kind == 0 ? SEL_NOT : SEL_ANDOR;   // ~ & | only
-        else if (I.pass == P_FUSED || I.pass == P_PROBE) t.part = SEL_ALL;
-        else t.part = t.kind == 0 ? I.usel : I.bsel;
+        uint32_t sel;
+        if (I.bs_mode) sel = t.kind == 0 ? SEL_NOT : SEL_ANDOR;   // ~ & | only
+        else if (I.pass == P_FUSED || I.pass == P_PROBE) sel = SEL_ALL;
+        else sel = t.kind == 0 ? I.usel : I.bsel;
+        t.part = (uint8_t)(sel & t.ops);
         return t.part != 0;
     };
     uint64_t waves1 = 0;
@@ -878,11 +915,13 @@ void Engine::store(std::vector<Inst*>& v) {
 void Engine::begin(const vfb_options& opt, bool batch) {
     opt_ = opt;
     batch_ = batch;
+    defer_alone_ = false;
     tun_ = current_tuning();
 }
 
 void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
     begin(opt, all.size() > 1);
+    defer_alone_ = true;           // vfb_solve_batch: memory-starved members are solved alone
     std::vector<Inst*> act;
     start(all, act);
     while (!act.empty()) step(act);
@@ -926,19 +965,21 @@ void Engine::start(std::vector<Inst*>& all, std::vector<Inst*>& out) {
         uint64_t total = 0;
         for (size_t k = 0; k < n; ++k) {
             Inst& I = *act[k];
-            I.h.lvl[1] = (const unsigned long long*)I.rows[1].p;
-            I.h.nrows[1] = I.nrows[1];
+            const int c1 = I.cost[0];     // the atoms' level
             I.h.ctr = (Ctr*)(d + o_ctr) + k;
-            ((InstDesc*)host.data())[k] = I.h;
+            InstDesc& hd = ((InstDesc*)host.data())[k];
+            hd = I.h;
+            hd.lvl[1] = (const unsigned long long*)I.rows[c1].p;   // the atom kernel reads slot 1
+            hd.nrows[1] = I.nrows[c1];
             ((Seg*)(host.data() + o_seg))[k] = Seg{(uint32_t)k, 0, total};
-            total += I.nrows[1];
+            total += I.nrows[c1];
         }
         h2d(d, host.data(), bytes);
         // every instance's atoms in one launch
         CK(launch_insert_atoms_multi((const InstDesc*)d, (const Seg*)(d + o_seg), (int)n, total, d_flag_, st_));
         flag_dirty_ = true;
         for (size_t k = 0; k < n; ++k) {
-            act[k]->committed = act[k]->nrows[1];   // at most A distinct atom keys
+            act[k]->committed = act[k]->nrows[act[k]->cost[0]];   // at most A distinct atom keys
             if (!act[k]->done) out.push_back(act[k]);
         }
     }
@@ -975,8 +1016,8 @@ void Engine::step(std::vector<Inst*>& act) {
         // list overflowed is grown and built again from its first pass ----
         for (auto* I : step) {
             // large deduplicating levels are probed first
-            const bool probe = !I->check_only && !I->bs_mode && tun_.pass_min && I->N >= tun_.pass_min &&
-                               nsplit_of(*I) >= 1;
+            const bool probe = !I->check_only && !I->bs_mode && I->unit_cost && tun_.pass_min &&
+                               I->N >= tun_.pass_min && nsplit_of(*I) >= 1;
             I->pass = probe ? P_PROBE : P_FUSED;
             I->usel = I->bsel = SEL_ALL;
             I->evaluated = 0;
@@ -1103,7 +1144,7 @@ void Engine::step(std::vector<Inst*>& act) {
             if (s >= I.max_size) {
                 // nothing up to max_size: the BS stage if asked for (its rows
                 // are this level's too), else the search ends
-                const bool bs = !I.bs_mode && I.bs_max > I.max_size && I.bs_max <= VFB_MAX_SIZE &&
+                const bool bs = !I.bs_mode && I.unit_cost && I.bs_max > I.max_size && I.bs_max <= VFB_MAX_SIZE &&
                                 I.h.T <= 63 && R.otf_level == 0;
                 if (!bs) {
                     finish(I, VFB_NOT_FOUND);
@@ -1134,9 +1175,10 @@ void Engine::step(std::vector<Inst*>& act) {
                 }
             }
             if (evicted) continue;
-            if (keep < W && (I.bs_pending || I.bs_mode)) {
-                // the BS stage needs every row; a BS level has no check-only round
-                finish(I, I.bs_mode ? VFB_CAPACITY : VFB_NOT_FOUND);
+            if (keep < W && (I.bs_pending || I.bs_mode || !I.unit_cost)) {
+                // the BS stage needs every row; a BS level, or a level under a
+                // cost function, has no check-only round
+                finish(I, I.bs_pending ? VFB_NOT_FOUND : VFB_CAPACITY);
                 continue;
             }
             if (keep < W) {
@@ -1212,6 +1254,29 @@ static void reset_result(vfb_result* r) {
     r->status = VFB_NOT_FOUND;
 }
 
+// LTL_BS's cost function: all zero (unit costs) or all >= 1 with the atoms'
+// cost within max_size; NegType 0, 1 or 2
+static bool costs_valid(const vfb_options& o) {
+    bool zero = true, pos = true;
+    for (int k = 0; k < 8; ++k) {
+        zero &= o.cost[k] == 0;
+        pos &= o.cost[k] >= 1;
+    }
+    if (!zero && (!pos || o.cost[0] > o.max_size)) return false;
+    return o.neg_type >= 0 && o.neg_type <= 2;
+}
+
+static void set_costs(Inst& I, const vfb_options& o) {
+    bool zero = true;
+    for (int k = 0; k < 8; ++k) zero &= o.cost[k] == 0;
+    I.unit_cost = true;
+    for (int k = 0; k < 8; ++k) {
+        I.cost[k] = zero ? 1 : o.cost[k];
+        I.unit_cost &= I.cost[k] == 1;
+    }
+    I.neg_type = o.neg_type == 2 ? 2 : 1;
+}
+
 static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const vfb_options* opt_in, vfb_result* out) {
     vfb_options opt;
     vfb_options_default(&opt);
@@ -1226,7 +1291,8 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
     std::vector<Inst*> run;
     for (size_t k = 0; k < n; ++k) {
         std::string why;
-        if (!insts[k] || validate(*insts[k], &why) != VFB_OK || opt.max_size < 1 || opt.max_size > VFB_MAX_SIZE) {
+        if (!insts[k] || validate(*insts[k], &why) != VFB_OK || opt.max_size < 1 || opt.max_size > VFB_MAX_SIZE ||
+            !costs_valid(opt)) {
             out[k].status = VFB_INVALID_INPUT;
             out[k].formula = strdup("see_the_error");
             continue;
@@ -1236,6 +1302,7 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
         st[k].max_size = opt.max_size;
         st[k].max_rows = opt.max_rows;
         st[k].bs_max = opt.bs_max_size;
+        set_costs(st[k], opt);
         st[k].prio = k;
         run.push_back(&st[k]);
     }
@@ -1274,6 +1341,7 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
             fresh.max_size = I->max_size;
             fresh.max_rows = I->max_rows;
             fresh.bs_max = I->bs_max;
+            set_costs(fresh, opt);
             fresh.budget = per;
             reset_result(fresh.res);
             std::vector<Inst*> one{&fresh};

@@ -156,6 +156,9 @@ struct Inst {
     // settles the match.
     // BS stage (LTL_BS.cu:808-1067): Boolean combinations of this instance's
     // stored time-0 vectors, run as a second instance over one-word rows
+    int cost[8] = {1, 1, 1, 1, 1, 1, 1, 1};  // p ~ & | X F G U (LTL_BS's c1..c8); level = cost
+    int neg_type = 1;           // LTL_BS's NegType: 2 = ~ of atoms only
+    bool unit_cost = true;
     int bs_max = 0;             // vfb_options.bs_max_size
     bool bs_mode = false;       // this instance is a BS stage
     bool bs_pending = false;    // the CS search ended without a match: start the BS stage
@@ -218,6 +221,7 @@ class Engine {
     std::vector<Inst*> live_;     // started, not finished: eviction candidates
     Inst* epoch_single_ = nullptr; // the one instance whose descriptor is at d_step_[0]
     vfb_tuning tun_{};            // launch / pass parameters, copied at begin()
+    bool defer_alone_ = false;    // batch: an instance that cannot get arena memory is deferred
 
     void sync();
     void h2d(void* dst, const void* src, size_t bytes);

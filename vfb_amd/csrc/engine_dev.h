@@ -113,7 +113,7 @@ struct Task {
     uint8_t kind, lvl_l, lvl_r, part;
     uint32_t nl, nr;
     uint32_t nchunk;            // ceil(nr / candidates per wavefront)
-    uint32_t pad1;
+    uint32_t ops;               // candidates of base[] that exist at this level (cost function)
     uint64_t base[4];
     uint64_t wave_begin;        // prefix over tasks, in wavefronts
 };

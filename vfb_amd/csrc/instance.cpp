@@ -5,6 +5,7 @@
 // fixes the atom order, a trace's length is the length of its value arrays,
 // positive traces come first.  The schema is fixed, so a small reader of our
 // own replaces nlohmann/json.
+#include <algorithm>
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -357,6 +358,86 @@ extern "C" vfb_status vfb_instance_load_json(const char* path, vfb_instance** ou
     if (st != VFB_OK) {
         std::fprintf(stderr, "%s\n", why.c_str());
         return st;
+    }
+    *out = inst.release();
+    return VFB_OK;
+}
+
+// LTL_BS's text trace format (readFile, LTL_BS.cu:1535-1607): one trace per
+// line, positives, a line "---", negatives, a line "---", and last the
+// alphabet line (the letters a-z, in the order of each step's values).  A
+// trace is steps separated by ';', a step the 0/1 values of the letters
+// separated by ','.  APs are ordered as the sorted letters (the reference
+// keeps them in a std::set<char>).
+extern "C" vfb_status vfb_instance_load_text(const char* path, vfb_instance** out) {
+    if (!path || !out) return VFB_INVALID_INPUT;
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        std::fprintf(stderr, "Failed to open the input file.\n");
+        return VFB_IO_ERROR;
+    }
+    std::vector<std::string> lines;
+    for (std::string line; std::getline(f, line);) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        lines.push_back(line);
+    }
+    size_t last = lines.size();
+    while (last > 0 && lines[last - 1].empty()) --last;
+    if (last == 0) return VFB_INVALID_INPUT;
+    std::string alpha;
+    for (char c : lines[last - 1])
+        if (c >= 'a' && c <= 'z') alpha += c;
+    std::string sorted = alpha;
+    std::sort(sorted.begin(), sorted.end());
+    sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
+    if (sorted.empty()) return VFB_INVALID_INPUT;
+    // a trace: per step, the letters whose value is 1 (as readFile builds it)
+    auto parse = [&](const std::string& line) {
+        std::vector<std::string> trace;
+        std::string token;
+        size_t j = 0;
+        for (char c : line) {
+            if (c == ';') {
+                trace.push_back(token);
+                token.clear();
+                j = 0;
+            } else if (c == ',') {
+                continue;
+            } else {
+                if (c == '1' && j < alpha.size()) token += alpha[j];
+                ++j;
+            }
+        }
+        trace.push_back(token);
+        return trace;
+    };
+    std::vector<std::vector<std::string>> pos, neg;
+    size_t i = 0;
+    for (; i < last - 1 && lines[i] != "---"; ++i) pos.push_back(parse(lines[i]));
+    for (++i; i < last - 1 && lines[i] != "---"; ++i) neg.push_back(parse(lines[i]));
+    auto inst = std::make_unique<vfb_instance>();
+    for (char c : sorted) inst->names.push_back(std::string(1, c));
+    const int A = (int)sorted.size();
+    inst->P = (int)pos.size();
+    std::vector<const std::vector<std::string>*> all;
+    for (auto& t : pos) all.push_back(&t);
+    for (auto& t : neg) all.push_back(&t);
+    const int T = (int)all.size();
+    inst->atoms.assign((size_t)A * T, 0);
+    for (int t = 0; t < T; ++t) {
+        const auto& tr = *all[t];
+        inst->lens.push_back((int)tr.size());
+        for (size_t k = 0; k < tr.size() && k < 64; ++k)
+            for (char c : tr[k]) {
+                const int a = (int)(std::lower_bound(sorted.begin(), sorted.end(), c) - sorted.begin());
+                inst->atoms[(size_t)a * T + t] |= 1ull << k;
+            }
+    }
+    std::string why;
+    if (validate(*inst, &why) != VFB_OK) {
+        std::fprintf(stderr, "%s\n", why.c_str());
+        return VFB_INVALID_INPUT;
     }
     *out = inst.release();
     return VFB_OK;

@@ -437,7 +437,8 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
         }
     }
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? (uint32_t)SEL : 0u, lane);
+    // SEL_ALL also serves any other mask (the cost function's): tk.part enables
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part) : 0u, lane);
 }
 
 template <int KEY, int G>
@@ -505,7 +506,7 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     const uint64_t pair = l * tk.nr + r;
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? (uint32_t)SEL : 0u, lane);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part) : 0u, lane);
 }
 
 template <int KEY, int G>

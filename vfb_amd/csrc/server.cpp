@@ -138,7 +138,10 @@ extern "C" vfb_status vfb_server_create(const vfb_options* opt, uint64_t hbm_bud
     if (opt) s->opt = *opt;
     if (s->opt.relaxed_type < 1 || s->opt.relaxed_type > 3) s->opt.relaxed_type = 3;
     if (s->opt.max_size < 1 || s->opt.max_size > VFB_MAX_SIZE) return VFB_INVALID_INPUT;
-    if (s->opt.search != VFB_SEARCH_CS || s->opt.bs_max_size) return VFB_INVALID_INPUT;   // CS levels only
+    // CS levels of unit cost only
+    if (s->opt.search != VFB_SEARCH_CS || s->opt.bs_max_size) return VFB_INVALID_INPUT;
+    for (int k = 0; k < 8; ++k)
+        if (s->opt.cost[k] > 1) return VFB_INVALID_INPUT;
     s->max_active = max_active > 0 ? max_active : 1024;
     {
         std::lock_guard<std::mutex> g(engine_mutex());
