@@ -109,6 +109,7 @@ def batched_mix(V, n, max_size, autotune=True):
             "batched_vs_plain": t_plain / t_batch, "bit_exact": exact,
             "plain_latency_ms_p50": pct(lat, 0.5), "plain_latency_ms_p99": pct(lat, 0.99),
             "distinct_cs": sum(sum(r["level_winners"]) for r in plain),
+            "batch_restarts": sum(r["restarts"] for r in batch), "batch_parked": sum(r["parked"] for r in batch),
             "gpu_saturating_instances": n - len(small),
             "gpu_saturating_share_of_plain_time": 1.0 - t_small_plain / t_plain,
             "others": {"instances": len(small), "batched_vs_plain": t_small_plain / t_small_batch,

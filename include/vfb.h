@@ -155,6 +155,10 @@ typedef struct {
     double ms_probe;       /* HIP-event time of the target-check probes of large levels
                               (vfb_tuning.pass_min); not part of ms_expand */
     uint64_t probe_candidates; /* candidates those probes checked */
+    uint32_t restarts;     /* batch / server: times the solve was evicted for arena memory
+                              and started again from its atoms (0 for vfb_solve) */
+    uint32_t parked;       /* vfb_solve_batch: 1 if it waited for the others at a level of
+                              vfb_tuning.park_min candidates and then went on alone */
 } vfb_result;
 
 void vfb_options_default(vfb_options* opt);
@@ -221,6 +225,11 @@ typedef struct {
     uint32_t expand_blocks_per_cu; /* 256-thread workgroups per CU, expand launch, 1..64 */
     uint32_t resolve_blocks_per_cu;/* same, winner resolution, 1..64 */
     uint32_t store_blocks_per_cu;  /* same, survivor storage, 1..64 */
+    uint64_t park_min;             /* vfb_solve_batch: a member whose next level has at least
+                                      this many candidates waits, its state kept, until the
+                                      others are done, then goes on alone (oldest first), so
+                                      such members neither share the device nor evict one
+                                      another mid-solve; 0 = never park */
 } vfb_tuning;
 
 void vfb_tuning_default(vfb_tuning* t);

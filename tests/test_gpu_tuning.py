@@ -85,6 +85,25 @@ def test_split_batch_equals_plain():
         V.set_tuning(**old)
 
 
+@pytest.mark.parametrize("park_min", [1, 1 << 12, 1 << 16])
+def test_parked_batch_equals_plain(park_min):
+    """Batch members reaching a level of park_min candidates wait, their rows
+    and set kept, and go on alone after the others, oldest first; park_min=1
+    parks every member at its first level, so they all run one by one from
+    there.  Every result equals the plain solve's."""
+    insts = [V.Instance.generate(4, k) for k in range(40)] + [V.Instance.generate(2, k) for k in range(4)]
+    plain = [V.solve(i, max_size=12) for i in insts]
+    old = V.set_tuning(park_min=park_min)
+    try:
+        print(f"[parity] parked batch park_min {park_min}", flush=True)
+        batch = V.solve_batch(insts, max_size=12)
+    finally:
+        V.set_tuning(**old)
+    for k, (a, b) in enumerate(zip(plain, batch)):
+        for f in FIELDS + ("level_winners", "level_candidates"):
+            assert a[f] == b[f], (park_min, k, f, a[f], b[f])
+
+
 def test_split_saves_work_at_found_level():
     """With the probe and operator blocks, a level settled by its first block
     evaluates fewer candidates than the fused pass; the answer is unchanged,

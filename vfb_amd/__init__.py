@@ -46,13 +46,14 @@ class Result(C.Structure):
                 ("retries", C.c_uint64), ("bytes_expand", C.c_uint64),
                 ("bytes_store", C.c_uint64), ("fresh_total", C.c_uint64),
                 ("otf_level", C.c_int), ("bs_stage", C.c_int), ("otf_stored", C.c_uint64),
-                ("ms_probe", C.c_double), ("probe_candidates", C.c_uint64)]
+                ("ms_probe", C.c_double), ("probe_candidates", C.c_uint64),
+                ("restarts", C.c_uint32), ("parked", C.c_uint32)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("pass_min", C.c_uint64), ("wide_max_waves", C.c_uint32),
                 ("expand_blocks_per_cu", C.c_uint32), ("resolve_blocks_per_cu", C.c_uint32),
-                ("store_blocks_per_cu", C.c_uint32)]
+                ("store_blocks_per_cu", C.c_uint32), ("park_min", C.c_uint64)]
 
 
 TUNING_FIELDS = [f for f, _ in Tuning._fields_]
@@ -261,6 +262,7 @@ def _todict(r):
         "bytes_store": r.bytes_store, "fresh_total": r.fresh_total,
         "otf_level": r.otf_level, "otf_stored": r.otf_stored, "bs_stage": r.bs_stage,
         "ms_probe": r.ms_probe, "probe_candidates": r.probe_candidates,
+        "restarts": r.restarts, "parked": r.parked,
     }
 
 

@@ -916,15 +916,32 @@ void Engine::begin(const vfb_options& opt, bool batch) {
     opt_ = opt;
     batch_ = batch;
     defer_alone_ = false;
+    park_ = false;
+    parked_.clear();
     tun_ = current_tuning();
 }
 
 void Engine::run(std::vector<Inst*>& all, const vfb_options& opt) {
     begin(opt, all.size() > 1);
     defer_alone_ = true;           // vfb_solve_batch: memory-starved members are solved alone
+    park_ = batch_ && tun_.park_min > 0;
     std::vector<Inst*> act;
     start(all, act);
-    while (!act.empty()) step(act);
+    for (;;) {
+        while (!act.empty()) step(act);
+        // parked members go on one at a time, oldest first: each fills the
+        // device alone, as its plain solve would
+        parked_.erase(std::remove_if(parked_.begin(), parked_.end(), [](Inst* I) { return I->done; }),
+                      parked_.end());
+        if (parked_.empty()) break;
+        auto it = std::min_element(parked_.begin(), parked_.end(),
+                                   [](const Inst* a, const Inst* b) { return a->prio < b->prio; });
+        Inst* I = *it;
+        parked_.erase(it);
+        I->parked = false;
+        act.push_back(I);
+    }
+    park_ = false;
     sync();
     for (Inst* I : all) drop_live(live_, I);
 }
@@ -989,10 +1006,20 @@ void Engine::step(std::vector<Inst*>& act) {
     {
         // ---- 1. units, tasks, scratch ----
         std::vector<Inst*> step;
+        size_t running = 0;
+        for (auto* I : act) running += I->done ? 0 : 1;
         for (auto* I : act) {
             if (I->done) continue;
             I->check_only = I->next_check_only;
             build_units(*I);
+            if (park_ && running > 1 && !I->bs_mode && I->N >= tun_.park_min) {
+                // a level this large fills the device alone: the member
+                // waits, its rows and set kept, until the others are done
+                I->parked = true;
+                I->res->parked = 1;
+                parked_.push_back(I);
+                continue;
+            }
             if (!I->check_only) {
                 try {
                     prepare_scratch(*I);
@@ -1216,7 +1243,7 @@ void Engine::step(std::vector<Inst*>& act) {
             if (I->bs && !I->bs->done) nxt.push_back(I->bs.get());
             continue;
         }
-        if (!I->done) nxt.push_back(I);
+        if (!I->done && !I->parked) nxt.push_back(I);
     }
     act.swap(nxt);
 }
@@ -1343,7 +1370,9 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
             fresh.bs_max = I->bs_max;
             set_costs(fresh, opt);
             fresh.budget = per;
+            const uint32_t restarts = fresh.res->restarts + 1;
             reset_result(fresh.res);
+            fresh.res->restarts = restarts;
             std::vector<Inst*> one{&fresh};
             eng.run(one, opt);
         }
@@ -1391,6 +1420,7 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     t->expand_blocks_per_cu = 8;
     t->resolve_blocks_per_cu = 16;
     t->store_blocks_per_cu = 8;
+    t->park_min = 1ull << 24;
 }
 
 extern "C" void vfb_get_tuning(vfb_tuning* t) {

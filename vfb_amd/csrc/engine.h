@@ -133,6 +133,7 @@ struct Inst {
     InstDesc h{};
     bool done = false;
     bool deferred = false;      // batch/server: evicted for memory, restarted later
+    bool parked = false;        // batch: waiting, state kept, to go on alone after the others
     uint64_t prio = 0;          // admission order: older instances evict younger ones
     bool check_only = false;    // this level is the onTheFly last round
     bool next_check_only = false;
@@ -222,6 +223,8 @@ class Engine {
     Inst* epoch_single_ = nullptr; // the one instance whose descriptor is at d_step_[0]
     vfb_tuning tun_{};            // launch / pass parameters, copied at begin()
     bool defer_alone_ = false;    // batch: an instance that cannot get arena memory is deferred
+    bool park_ = false;           // batch: members reaching a level of tun_.park_min candidates wait
+    std::vector<Inst*> parked_;   // those members, resumed one at a time after the rest
 
     void sync();
     void h2d(void* dst, const void* src, size_t bytes);

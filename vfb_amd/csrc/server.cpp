@@ -31,8 +31,10 @@ struct Req {
 
 void reset_req(Req& r, const vfb_options& opt, uint64_t budget) {
     std::free(r.res.formula);
+    const uint32_t restarts = r.st.deferred ? r.res.restarts + 1 : 0;
     std::memset(&r.res, 0, sizeof r.res);
     r.res.status = VFB_NOT_FOUND;
+    r.res.restarts = restarts;
     r.st = Inst{};
     r.st.in = &r.inst;
     r.st.res = &r.res;

@@ -29,6 +29,7 @@ const Field kFields[] = {
     {"expand_blocks_per_cu", false, offsetof(vfb_tuning, expand_blocks_per_cu)},
     {"resolve_blocks_per_cu", false, offsetof(vfb_tuning, resolve_blocks_per_cu)},
     {"store_blocks_per_cu", false, offsetof(vfb_tuning, store_blocks_per_cu)},
+    {"park_min", true, offsetof(vfb_tuning, park_min)},
 };
 
 uint64_t get(const vfb_tuning& t, const Field& f) {
@@ -151,12 +152,13 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     static const uint64_t cand_expand[] = {4, 8, 16, 32};
     static const uint64_t cand_resolve[] = {8, 16, 32};
     static const uint64_t cand_store[] = {4, 8, 16};
+    static const uint64_t cand_park[] = {0, 1ull << 22, 1ull << 24, 1ull << 26};
     struct Axis {
         const Field* f;
         const uint64_t* v;
         size_t nv;
     } axes[] = {{&kFields[0], cand_pass, 5}, {&kFields[1], cand_wide, 4}, {&kFields[2], cand_expand, 4},
-                {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}};
+                {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}, {&kFields[5], cand_park, 4}};
     for (const Axis& ax : axes) {
         for (size_t i = 0; i < ax.nv && err == VFB_OK; ++i) {
             if (get(cur, *ax.f) == ax.v[i]) continue;
