@@ -277,6 +277,8 @@ int ora_enumerate_omp(const ora_instance* in, int max_size, int key_mode, uint64
         out->status = 2;
         return 2;
     }
+    for (int t = 0; t < in->T; ++t)   /* one word per trace */
+        if (in->len[t] < 1 || in->len[t] > 64) { out->status = 2; return 2; }
     const int T = in->T, A = in->A, P = in->P;
     octx c;
     memset(&c, 0, sizeof c);

@@ -18,6 +18,14 @@
  *   - Until evaluates every (l, r) pair in both orientations (the reference's
  *     processOperator only evaluates even linear pair indices, LTL_BS.cu:397).
  *   - No all-ones sentinel rows, so traces of length 64 are allowed.
+ *   - Traces may be longer than one word (up to ORA_MAX_LEN steps): a trace
+ *     of n steps is ceil(n/64) words, word w holding steps 64w..64w+63, and a
+ *     CS row is every trace's words in trace order (the reference stops at 64).
+ *     The operators carry across words (ora_apply_trace); the exact key and
+ *     the relaxed types 1 and 2 take the trace's bits at the same global
+ *     positions as the reference's formulas; type 3 mixes word w as
+ *     mix64(word + w * 0x9e3779b97f4a7c15), which is the reference's per-trace
+ *     mix64(word) for one-word traces.
  *   - Among candidates with equal keys the lowest canonical ordinal survives,
  *     and the returned formula is the lowest-ordinal matching survivor
  *     (the reference takes whichever thread's atomicCAS lands first).
@@ -73,6 +81,64 @@ uint64_t ora_apply(int op, uint64_t l, uint64_t r, int len)
     }
 }
 
+int ora_words(int len)
+{
+    return (len + 63) / 64;
+}
+
+/* column layout: trace t's words are columns col[t] .. col[t+1]-1 */
+static int col_prefix(const int* len, int T, int* col)
+{
+    col[0] = 0;
+    for (int t = 0; t < T; ++t) col[t + 1] = col[t] + ora_words(len[t]);
+    return col[T];
+}
+
+/* One operator over the words of one trace of len steps (word 0 = steps
+   0..63).  One word is LTL_BS.cu:183-238 as restated in ora_apply.  Longer
+   traces run from the last word back, carrying what the later steps decide:
+   X takes bit 0 of the next word into bit 63; F of a word is all ones once a
+   later word holds a step; G of a word is zero once a later step fails;
+   (l U r) at step t of a word is its in-word value or, when l holds from t to
+   the word's end, the value at the next word's step 0. */
+void ora_apply_trace(int op, const uint64_t* l, const uint64_t* r, int len, uint64_t* out)
+{
+    const int n = ora_words(len);
+    if (n == 1) { out[0] = ora_apply(op, l[0], r ? r[0] : 0, len); return; }
+    uint64_t carry = 0;
+    int later_any = 0, later_all = 1;
+    for (int w = n - 1; w >= 0; --w) {
+        const uint64_t m = w == n - 1 ? ora_mask(len - 64 * w) : ~(uint64_t)0;
+        const uint64_t a = l[w], b = r ? r[w] : 0;
+        uint64_t v = 0;
+        switch (op) {
+        case ORA_NOT: v = ~a & m; break;
+        case ORA_AND: v = a & b; break;
+        case ORA_OR: v = a | b; break;
+        case ORA_NEXT: v = (a >> 1) | (carry << 63); carry = a & 1; break;
+        case ORA_FINALLY: v = later_any ? m : suffix_or(a); later_any |= a != 0; break;
+        case ORA_GLOBALLY:
+            v = later_all ? (a & ~suffix_or(~a & m)) : 0;
+            later_all &= a == m;
+            break;
+        case ORA_UNTIL:
+            v = ora_apply(ORA_UNTIL, a, b, 64) | (carry ? ~suffix_or(~a) : 0);
+            carry = v & 1;
+            break;
+        default: break;
+        }
+        out[w] = v;
+    }
+}
+
+/* op over every trace of a row (columns as laid out by col_prefix) */
+static void row_apply(int op, const uint64_t* l, const uint64_t* r, const int* len, const int* col, int T,
+                      uint64_t* out)
+{
+    for (int t = 0; t < T; ++t)
+        ora_apply_trace(op, l + col[t], r ? r + col[t] : NULL, len[t], out + col[t]);
+}
+
 uint64_t ora_mix64(uint64_t x)
 {
     /* SplitMix64 output finaliser (Steele, Lea, Flood 2014), LTL_BS.cu:80-83 */
@@ -87,12 +153,17 @@ static void put_bit(int j, uint64_t* hi, uint64_t* lo)
     else if (j < 126) *hi |= (uint64_t)1 << (j - 63);
 }
 
+/* bit k (step k) of trace i: word k/64 of its columns */
+#define TBIT(i, k) ((cs[col[i] + (k) / 64] >> ((k) % 64)) & 1)
+
 void ora_key(const uint64_t* cs, const int* len, int T, int relaxed_type,
              uint64_t* hi_out, uint64_t* lo_out)
 {
     /* LTL_BS.cu:30-110, bit-serial exactly as written there */
     uint64_t hi = 0, lo = 0;
     int lenSum = 0;
+    int col[ORA_MAX_TRACES + 1];
+    col_prefix(len, T, col);
     for (int i = 0; i < T; ++i) lenSum += len[i];
 
     if (lenSum > 126) {
@@ -101,25 +172,27 @@ void ora_key(const uint64_t* cs, const int* len, int T, int relaxed_type,
             int j = 0;
             for (int i = 0; i < T && j < 126; ++i)
                 for (int k = 0; k < len[i] && j < 126; k += stride, ++j)
-                    if (cs[i] & ((uint64_t)1 << k)) put_bit(j, &hi, &lo);
+                    if (TBIT(i, k)) put_bit(j, &hi, &lo);
         } else if (relaxed_type == 2) {          /* :61-76 */
             int j = 0;
             for (int i = 0; i < T && j < 126; ++i) {
                 int maxbits = (126 * len[i] + lenSum) / lenSum;
+                /* steps past the trace read as 0 (the reference's shifted-out bitPtr) */
                 for (int k = 0; k < maxbits && j < 126; ++k, ++j)
-                    if (cs[i] & ((uint64_t)1 << k)) put_bit(j, &hi, &lo);
+                    if (k < 64 * (col[i + 1] - col[i]) && TBIT(i, k)) put_bit(j, &hi, &lo);
             }
         } else {                                 /* :78-88 (type 3) */
-            for (int i = 0; i < T; ++i) {
-                uint64_t x = ora_mix64(cs[i]);
-                if (i < T / 2) hi ^= x; else lo ^= x;
-            }
+            for (int i = 0; i < T; ++i)
+                for (int w = 0; w < col[i + 1] - col[i]; ++w) {
+                    uint64_t x = ora_mix64(cs[col[i] + w] + (uint64_t)w * UINT64_C(0x9e3779b97f4a7c15));
+                    if (i < T / 2) hi ^= x; else lo ^= x;
+                }
         }
     } else {                                     /* :94-106 exact */
         int j = 0;
         for (int i = 0; i < T; ++i)
             for (int k = 0; k < len[i]; ++k, ++j)
-                if (cs[i] & ((uint64_t)1 << k)) put_bit(j, &hi, &lo);
+                if (TBIT(i, k)) put_bit(j, &hi, &lo);
     }
     /* WarpCore rejects the empty (-1) and tombstone (-2) keys and the caller
        then uses slot id 0 for both (hash_set.cuh:198-204, LTL_BS.cu:262-265),
@@ -129,6 +202,7 @@ void ora_key(const uint64_t* cs, const int* len, int T, int relaxed_type,
     *hi_out = hi;
     *lo_out = lo;
 }
+#undef TBIT
 
 /* ------------------------------------------------------------------ */
 /* simple open-addressing sets                                         */
@@ -192,7 +266,7 @@ static int keyset_insert(keyset* s, uint64_t hi, uint64_t lo)
 /* ------------------------------------------------------------------ */
 
 typedef struct {
-    int T;
+    int T;              /* words per row (the columns of all traces) */
     uint64_t* cs;       /* rows*T */
     int32_t* left;
     int32_t* right;
@@ -268,11 +342,11 @@ static void row_to_string(const rowstore* st, int row, const char* const* names,
     }
 }
 
-static int matches_target(const uint64_t* cs, int T, int P)
+static int matches_target(const uint64_t* cs, const int* col, int T, int P)
 {
-    /* LTL_BS.cu:316-318 */
-    for (int i = 0; i < P; ++i) if (!(cs[i] & 1)) return 0;
-    for (int i = P; i < T; ++i) if (cs[i] & 1) return 0;
+    /* LTL_BS.cu:316-318: step 0 of every trace (bit 0 of its first word) */
+    for (int i = 0; i < P; ++i) if (!(cs[col[i]] & 1)) return 0;
+    for (int i = P; i < T; ++i) if (cs[col[i]] & 1) return 0;
     return 1;
 }
 
@@ -318,6 +392,7 @@ static int rowset_find_or_add(rowset* s, const rowstore* st, const uint64_t* cs,
 
 typedef struct {
     const ora_instance* in;
+    const int* col;      /* column prefix of the traces */
     int key_mode;
     uint64_t max_rows;
     const char* const* names;
@@ -357,7 +432,7 @@ static int candidate(enum_ctx* c, const uint64_t* cs, int op, int l, int r)
     c->out->candidates[c->level]++;
     if (c->check_only) {
         /* processOnTheFly, LTL_BS.cu:331-349 */
-        if (matches_target(cs, T, in->P)) { set_found(c, cs, op, l, r, ord); return 1; }
+        if (matches_target(cs, c->col, T, in->P)) { set_found(c, cs, op, l, r, ord); return 1; }
         return 0;
     }
     int fresh;
@@ -370,7 +445,7 @@ static int candidate(enum_ctx* c, const uint64_t* cs, int op, int l, int r)
     }
     if (fresh < 0) { c->done = -1; return 1; }
     if (!fresh) return 0;
-    if (matches_target(cs, T, in->P)) { set_found(c, cs, op, l, r, ord); return 1; }
+    if (matches_target(cs, c->col, T, in->P)) { set_found(c, cs, op, l, r, ord); return 1; }
     c->out->distinct[c->level]++;
     if (c->st.rows >= c->max_rows && c->level < c->max_size) {
         /* cache full: the survivor is checked but not stored (LTL_BS.cu:516-522) */
@@ -440,10 +515,10 @@ static void bs_text(const bsnodes* v, const rowstore* st, int64_t k, const char*
 
 /* BS levels max_size+1 .. bs_max over the rows of levels 1..max_size
    (start[l] = first row of level l); returns 1 if a formula was found */
-static int bs_stage(const ora_instance* in, const rowstore* st, const size_t* start, int max_size, int bs_max,
-                    const char* const* names, ora_result* out)
+static int bs_stage(const ora_instance* in, const int* col, const rowstore* st, const size_t* start, int max_size,
+                    int bs_max, const char* const* names, ora_result* out)
 {
-    const int T = in->T, P = in->P;
+    const int T = in->T, P = in->P, W = st->T;
     const uint64_t tmask = T >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << T) - 1);
     const uint64_t pmask = ((uint64_t)1 << P) - 1;
     bsnodes v;
@@ -457,7 +532,7 @@ static int bs_stage(const ora_instance* in, const rowstore* st, const size_t* st
         bstart[l] = v.n;
         for (size_t r = start[l]; r < start[l + 1]; ++r) {
             uint64_t w = 0;
-            for (int t = 0; t < T; ++t) w |= (st->cs[r * T + t] & 1) << t;
+            for (int t = 0; t < T; ++t) w |= (st->cs[r * W + col[t]] & 1) << t;
             if (keyset_insert(&seen, 0, w) == 1) bs_push(&v, w, 0, (int64_t)r, 0);
         }
     }
@@ -535,15 +610,19 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
     if (in->T < 1 || in->T > ORA_MAX_TRACES || in->A < 1 || in->A > ORA_MAX_ATOMS ||
         max_size < 1 || max_size > ORA_MAX_SIZE) { out->status = 2; return 2; }
     const int T = in->T;
+    for (int t = 0; t < T; ++t)
+        if (in->len[t] < 1 || in->len[t] > ORA_MAX_LEN) { out->status = 2; return 2; }
+    int col[ORA_MAX_TRACES + 1];
+    const int W = col_prefix(in->len, T, col);   /* words per row */
     enum_ctx c;
     memset(&c, 0, sizeof c);
-    c.in = in; c.key_mode = key_mode; c.names = names; c.out = out;
+    c.in = in; c.col = col; c.key_mode = key_mode; c.names = names; c.out = out;
     c.max_rows = max_rows ? max_rows : UINT64_MAX;
     c.max_size = max_size;
-    c.st.T = T;
+    c.st.T = W;
     if (key_mode != ORA_KEY_FULLROW && !keyset_init(&c.ks, 1024)) return -1;
 
-    uint64_t* cs = (uint64_t*)malloc(T * 8);
+    uint64_t* cs = (uint64_t*)malloc(W * 8);
     size_t start[ORA_MAX_SIZE + 2];
     /* level = cost (LTL_BS.cu:1085-1493): p c1, ~ c2, & c3, | c4, X c5, F c6,
        G c7, U c8; unit costs make it the formula size */
@@ -557,10 +636,13 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
        (hashSetsInitialisation, LTL_BS.cu:136-160, 1217) */
     c.level = c1;
     for (int a = 0; a < in->A && !c.done; ++a) {
-        for (int t = 0; t < T; ++t) cs[t] = in->atoms[a * T + t] & ora_mask(in->len[t]);
+        for (int t = 0; t < T; ++t)
+            for (int w = col[t]; w < col[t + 1]; ++w)
+                cs[w] = in->atoms[(size_t)a * W + w] & (w + 1 == col[t + 1] ? ora_mask(in->len[t] - 64 * (w - col[t]))
+                                                                              : ~(uint64_t)0);
         out->candidates[c1]++;
         c.local_ord++;
-        if (matches_target(cs, T, in->P)) {
+        if (matches_target(cs, col, T, in->P)) {
             char tmp[160];
             atom_name(a, names, tmp);
             snprintf(out->formula, sizeof out->formula, "%s", tmp);
@@ -587,12 +669,12 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
         c.level = s;
         c.local_ord = 0;
         const uint64_t* CS = NULL;
-#define ROW(i) (c.st.cs + (size_t)(i) * T)
+#define ROW(i) (c.st.cs + (size_t)(i) * W)
         /* ~ over level s-c2 (LTL_BS.cu:1236-1272; NegType 2: atoms only) */
         if (s - c2 >= c1 && (neg_type == 1 || s - c2 == c1))
             for (size_t x = start[s - c2]; x < start[s - c2 + 1] && !c.done; ++x) {
                 CS = ROW(x);
-                for (int t = 0; t < T; ++t) cs[t] = ora_apply(ORA_NOT, CS[t], 0, in->len[t]);
+                row_apply(ORA_NOT, CS, NULL, in->len, col, T, cs);
                 candidate(&c, cs, ORA_NOT, (int)x, 0);
             }
         /* & then | over splits c1 <= i, 2i <= s - cost, left-major (LTL_BS.cu:1274-1346) */
@@ -604,7 +686,7 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
                     for (size_t r = start[j]; r < start[j + 1] && !c.done; ++r) {
                         const uint64_t* L = ROW(l);
                         const uint64_t* R = ROW(r);
-                        for (int t = 0; t < T; ++t) cs[t] = ora_apply(op, L[t], R[t], in->len[t]);
+                        row_apply(op, L, R, in->len, col, T, cs);
                         candidate(&c, cs, op, (int)l, (int)r);
                     }
             }
@@ -615,7 +697,7 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
             if (lv < c1) continue;
             for (size_t x = start[lv]; x < start[lv + 1] && !c.done; ++x) {
                 CS = ROW(x);
-                for (int t = 0; t < T; ++t) cs[t] = ora_apply(op, CS[t], 0, in->len[t]);
+                row_apply(op, CS, NULL, in->len, col, T, cs);
                 candidate(&c, cs, op, (int)x, 0);
             }
         }
@@ -627,10 +709,10 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
                 for (size_t r = start[j]; r < start[j + 1] && !c.done; ++r) {
                     const uint64_t* L = ROW(l);
                     const uint64_t* R = ROW(r);
-                    for (int t = 0; t < T; ++t) cs[t] = ora_apply(ORA_UNTIL, L[t], R[t], in->len[t]);
+                    row_apply(ORA_UNTIL, L, R, in->len, col, T, cs);
                     if (candidate(&c, cs, ORA_UNTIL, (int)l, (int)r)) break;
                     L = ROW(l); R = ROW(r);   /* store may have moved */
-                    for (int t = 0; t < T; ++t) cs[t] = ora_apply(ORA_UNTIL, R[t], L[t], in->len[t]);
+                    row_apply(ORA_UNTIL, R, L, in->len, col, T, cs);
                     candidate(&c, cs, ORA_UNTIL, (int)r, (int)l);
                 }
         }
@@ -646,7 +728,7 @@ static int enumerate_core(const ora_instance* in, int max_size, int key_mode, ui
 
     /* nothing up to max_size: Boolean combinations of the stored CS (BS) */
     if (!c.done && !c.otf && unit && bs_max > max_size && bs_max <= ORA_MAX_SIZE && T <= 63 &&
-        bs_stage(in, &c.st, start, max_size, bs_max, names, out))
+        bs_stage(in, col, &c.st, start, max_size, bs_max, names, out))
         c.done = 1;
     free(cs);
     keyset_free(&c.ks);
@@ -777,6 +859,8 @@ int ora_bruteforce(const ora_instance* in, int max_size, const char* const* name
     memset(out, 0, sizeof *out);
     if (max_size < 1 || max_size > ORA_MAX_SIZE) { out->status = 2; return 2; }
     int T = in->T;
+    for (int t = 0; t < T; ++t)   /* ltl_nocache's traces are one word */
+        if (in->len[t] < 1 || in->len[t] > 64) { out->status = 2; return 2; }
     /* atoms first (ltl_nocache.cu:367-378) */
     for (int a = 0; a < in->A; ++a) {
         out->candidates[1]++;

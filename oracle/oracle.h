@@ -23,6 +23,7 @@ extern "C" {
 #define ORA_MAX_TRACES 128
 #define ORA_MAX_ATOMS 64
 #define ORA_MAX_SIZE 32
+#define ORA_MAX_LEN 4096    /* steps per trace; one 64-bit word per 64 steps */
 
 /* operator codes, in the reference's segment order (LTL_BS.cu:17) */
 enum { ORA_NOT = 0, ORA_AND = 1, ORA_OR = 2, ORA_NEXT = 3, ORA_FINALLY = 4,
@@ -36,8 +37,11 @@ typedef struct {
     int A;                  /* number of atomic propositions */
     int T;                  /* number of traces (positives first) */
     int P;                  /* number of positive traces */
-    const int* len;         /* [T] trace lengths, 1..64 */
-    const uint64_t* atoms;  /* [A*T]: atoms[a*T+t] bit k = AP a holds at step k of trace t */
+    const int* len;         /* [T] trace lengths, 1..ORA_MAX_LEN */
+    const uint64_t* atoms;  /* [A*W], W = sum over traces of ora_words(len): trace t's
+                               words follow those of traces < t; bit k of word w of
+                               trace t (atoms[a*W + col_t + w]) = AP a holds at step
+                               64w+k.  With every len <= 64: atoms[a*T+t]. */
 } ora_instance;
 
 typedef struct {
@@ -58,6 +62,10 @@ typedef struct {
 uint64_t ora_mask(int len);
 uint64_t ora_apply(int op, uint64_t l, uint64_t r, int len);
 uint64_t ora_mix64(uint64_t x);
+int ora_words(int len);     /* words of a trace of len steps: ceil(len / 64) */
+/* op over the ora_words(len) words of one trace (r NULL for unary ops) */
+void ora_apply_trace(int op, const uint64_t* l, const uint64_t* r, int len, uint64_t* out);
+/* key of a CS row of T traces (their words in trace order, see ora_instance) */
 void ora_key(const uint64_t* cs, const int* len, int T, int relaxed_type,
              uint64_t* hi, uint64_t* lo);
 
@@ -86,7 +94,7 @@ int ora_bruteforce(const ora_instance* in, int max_size,
                    const char* const* names, ora_result* out);
 /* multi-core CPU build of ora_enumerate (cpu_omp.c, OpenMP): same results
    (status, min_size, formula, distinct, candidates, found_ordinal) for
-   key modes 1-3 and no row cap; levels with at least pass_min candidates
+   key modes 1-3, no row cap and traces of at most 64 steps; levels with at least pass_min candidates
    (0: none) are built in three passes that stop at a settled match.  It is
    the CPU baseline of the benchmark, pinned against ora_enumerate. */
 int ora_enumerate_omp(const ora_instance* in, int max_size, int key_mode, uint64_t pass_min,
@@ -97,7 +105,9 @@ int ora_omp_threads(void);
 void ora_num_formulas(int A, int max_size, uint64_t* out /* [(max_size+1)*8] */);
 
 /* evaluate an RPN formula (tokens: 0..A-1 atoms, 100+op operators) on trace t
-   with the reference's bit-parallel operators; returns the CS word */
+   with the reference's bit-parallel operators; returns the CS word (traces of
+   at most 64 steps, as ltl_nocache's; ora_bruteforce returns status 2 for
+   longer ones) */
 uint64_t ora_eval_rpn(const ora_instance* in, const int* tokens, int n, int t);
 
 #ifdef __cplusplus

@@ -63,6 +63,9 @@ def lib():
         _lib.ora_apply.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_int]
         _lib.ora_mix64.restype = C.c_uint64
         _lib.ora_mix64.argtypes = [C.c_uint64]
+        _lib.ora_apply_trace.restype = None
+        _lib.ora_apply_trace.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int,
+                                         C.POINTER(C.c_uint64)]
         _lib.ora_key.restype = None
         _lib.ora_key.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_int), C.c_int, C.c_int,
                                  C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
@@ -83,8 +86,23 @@ def lib():
     return _lib
 
 
+def nwords(n):
+    """64-bit words of a trace of n steps."""
+    return (n + 63) // 64
+
+
+def split_words(x, n):
+    """Python int of an n-step trace -> its words, step 64w+k = bit k of word w."""
+    return [(x >> (64 * w)) & 0xFFFFFFFFFFFFFFFF for w in range(nwords(n))]
+
+
+def join_words(ws):
+    return sum(int(w) << (64 * k) for k, w in enumerate(ws))
+
+
 class Instance:
-    """Plain-Python instance: names, lens, atoms[a][t] words, P positives first."""
+    """Plain-Python instance: names, lens, atoms[a][t] (a Python int, bit k =
+    step k, any length), P positives first."""
 
     def __init__(self, names, lens, atoms, P):
         self.names = list(names)
@@ -122,7 +140,8 @@ class Instance:
     def _c(self):
         T = self.T
         lens = (C.c_int * T)(*self.lens)
-        flat = (C.c_uint64 * (self.A * T))(*[w for row in self.atoms for w in row])
+        words = [w for row in self.atoms for t, x in enumerate(row) for w in split_words(x, self.lens[t])]
+        flat = (C.c_uint64 * len(words))(*words)
         names = (C.c_char_p * self.A)(*[n.encode() for n in self.names])
         inst = OraInstance(self.A, T, self.P, lens, flat)
         return inst, (lens, flat, names), names
@@ -192,10 +211,22 @@ def mix64(x):
     return lib().ora_mix64(x)
 
 
-def key(words, lens, relaxed_type=3):
-    T = len(words)
+def apply_trace(op, l, r, length):
+    """op over one trace of `length` steps given as Python ints (any length)."""
+    n = nwords(length)
+    out = (C.c_uint64 * n)()
+    lw = (C.c_uint64 * n)(*split_words(l, length))
+    rw = (C.c_uint64 * n)(*split_words(r, length))
+    lib().ora_apply_trace(op, lw, rw, length, out)
+    return join_words(out)
+
+
+def key(cs, lens, relaxed_type=3):
+    """Key of a CS row: cs[t] is trace t's Python int (any length)."""
+    T = len(cs)
+    words = [w for t, x in enumerate(cs) for w in split_words(x, lens[t])]
     hi, lo = C.c_uint64(), C.c_uint64()
-    lib().ora_key((C.c_uint64 * T)(*words), (C.c_int * T)(*lens), T, relaxed_type,
+    lib().ora_key((C.c_uint64 * len(words))(*words), (C.c_int * T)(*lens), T, relaxed_type,
                   C.byref(hi), C.byref(lo))
     return hi.value, lo.value
 
