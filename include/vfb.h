@@ -26,7 +26,8 @@ extern "C" {
 #define VFB_MAX_SIZE 32     /* maximum formula size (ltl_nocache.cu:11) */
 #define VFB_MAX_TRACES 128  /* ltl_nocache allows 100 (ltl_nocache.cu:10), LTL_BS 63 (LTL_BS.cu:13) */
 #define VFB_MAX_ATOMS 64
-#define VFB_MAX_LEN 64      /* one 64-bit CS word per trace (ltl_nocache.cu:12) */
+#define VFB_MAX_LEN 4096    /* steps per trace: ceil(len / 64) 64-bit CS words per trace
+                               (the reference stops at one word, ltl_nocache.cu:12) */
 
 typedef enum {
     VFB_OK = 0,             /* formula found */
@@ -57,12 +58,24 @@ vfb_status vfb_instance_load_json(const char* path, vfb_instance** out);
    letters separated by ','.  APs are the sorted letters. */
 vfb_status vfb_instance_load_text(const char* path, vfb_instance** out);
 
-/* Builds an instance from bit-packed traces: atoms[a * T + t] bit k is the
-   value of AP a at step k of trace t; traces 0..num_pos-1 are positive
-   (the layout the reference builds at LTL_BS.cu:1127-1163). */
+/* Builds an instance from bit-packed traces of at most 64 steps:
+   atoms[a * T + t] bit k is the value of AP a at step k of trace t; traces
+   0..num_pos-1 are positive (the layout the reference builds at
+   LTL_BS.cu:1127-1163). */
 vfb_status vfb_instance_from_bits(int num_atoms, const char* const* atom_names,
                                   int num_pos, int num_neg, const int* lens,
                                   const uint64_t* atoms, vfb_instance** out);
+
+/* The same for traces of any length up to VFB_MAX_LEN: trace t of n steps is
+   ceil(n / 64) words, word w holding steps 64w..64w+63 (step k in bit k % 64),
+   and a trace's words follow those of the traces before it, so that
+   words[a * W + first(t) + w] is word w of AP a on trace t, W = the sum of all
+   traces' words (vfb_instance_words).  With every trace at most 64 steps this
+   is vfb_instance_from_bits.  No reference counterpart (one word per trace
+   there). */
+vfb_status vfb_instance_from_words(int num_atoms, const char* const* atom_names,
+                                   int num_pos, int num_neg, const int* lens,
+                                   const uint64_t* words, vfb_instance** out);
 
 /* Seeded synthetic instance: config 1 plain, 2 dedup stress, 3 limits,
    4 batched mix (5, the served stream, draws from config 4).  Instance k of
@@ -73,9 +86,12 @@ vfb_status vfb_instance_generate(int config, uint64_t index, vfb_instance** out)
    generating_formula / generating_seed for generated instances. */
 vfb_status vfb_instance_write_json(const vfb_instance* inst, const char* path);
 
-/* Read-only accessors.  lens: T ints; atoms: A*T words (layout above). */
+/* Read-only accessors.  lens: T ints; atoms: A*W words in the layout of
+   vfb_instance_from_words (A*T, that of vfb_instance_from_bits, when every
+   trace has at most 64 steps). */
 void vfb_instance_shape(const vfb_instance* inst, int* num_atoms, int* num_traces,
                         int* num_pos);
+int vfb_instance_words(const vfb_instance* inst);   /* W: words of one CS row */
 void vfb_instance_copy(const vfb_instance* inst, int* lens, uint64_t* atoms);
 const char* vfb_instance_atom_name(const vfb_instance* inst, int a);
 const char* vfb_instance_planted(const vfb_instance* inst); /* "" if none */

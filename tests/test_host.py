@@ -86,7 +86,13 @@ def test_generator_shapes():
 
 def test_limits_rejected():
     with pytest.raises(V.VfbError):
-        V.Instance.from_bits(["a"], [65], [[1]], 1)          # trace longer than 64
+        V.Instance.from_bits(["a"], [4097], [[1]], 1)        # trace longer than VFB_MAX_LEN
+    # the one-word entry point takes traces of at most 64 steps
+    h = C.c_void_p()
+    lib = V.lib()
+    st = lib.vfb_instance_from_bits(1, (C.c_char_p * 1)(b"a"), 1, 0, (C.c_int * 1)(65), (C.c_uint64 * 1)(1),
+                                    C.byref(h))
+    assert st == V.INVALID_INPUT and not h.value
     with pytest.raises(V.VfbError):
         V.Instance.from_bits(["a"], [3] * 129, [[1] * 129], 1)  # more than 128 traces
     with pytest.raises(V.VfbError):
@@ -226,3 +232,34 @@ def test_ltlbs_cli_argument_checks():
     out = subprocess.run([V.CLI_PATH, path] + ["1"] * 8 + ["40", "3", "2", "1"], capture_output=True, text=True,
                          timeout=60)
     assert "at most 32" in out.stdout
+
+
+def test_long_traces_roundtrip(tmp_path):
+    """Traces over 64 steps are several CS words: from_bits / arrays keep every
+    step, clear steps past the end, and the JSON writer and reader agree with
+    Python's json module."""
+    import random
+    rng = random.Random(12)
+    lens = [1, 64, 65, 128, 129, 300, 4096]
+    atoms = [[rng.getrandbits(n + 7) for n in lens] for _ in range(3)]
+    inst = V.Instance.from_bits(["x", "y", "z"], lens, atoms, 3)
+    names, got_lens, got, P = inst.arrays()
+    assert got_lens == lens and P == 3
+    assert got == [[w & ((1 << n) - 1) for w, n in zip(row, lens)] for row in atoms]
+    assert V.lib().vfb_instance_words(inst.handle) == sum((n + 63) // 64 for n in lens)
+    path = str(tmp_path / "long.json")
+    inst.write_json(path)
+    assert V.Instance.from_json(path).arrays() == inst.arrays() == _py_arrays(path)
+
+
+def test_text_reader_long_traces(tmp_path):
+    import random
+    rng = random.Random(13)
+    rows = []
+    for n in (3, 70, 200):
+        rows.append(";".join(",".join(str(rng.randint(0, 1)) for _ in range(2)) for _ in range(n)))
+    path = tmp_path / "long.txt"
+    path.write_text("\n".join([rows[0], rows[1], "---", rows[2], "---", "a,b"]) + "\n")
+    inst = V.Instance.from_text(str(path))
+    assert inst.arrays() == _read_text_py(str(path))
+    assert inst.arrays()[1] == [3, 70, 200]

@@ -61,8 +61,8 @@ TUNING_FIELDS = [f for f, _ in Tuning._fields_]
 
 # every symbol declared in include/vfb.h
 EXPORTS = [
-    "vfb_instance_load_json", "vfb_instance_load_text", "vfb_instance_from_bits", "vfb_instance_generate",
-    "vfb_instance_write_json", "vfb_instance_shape", "vfb_instance_copy",
+    "vfb_instance_load_json", "vfb_instance_load_text", "vfb_instance_from_bits", "vfb_instance_from_words",
+    "vfb_instance_generate", "vfb_instance_write_json", "vfb_instance_shape", "vfb_instance_words", "vfb_instance_copy",
     "vfb_instance_atom_name", "vfb_instance_planted", "vfb_instance_max_size",
     "vfb_instance_free", "vfb_options_default", "vfb_solve", "vfb_solve_batch",
     "vfb_result_free", "vfb_status_string", "vfb_device_info",
@@ -88,6 +88,9 @@ def lib():
     L.vfb_instance_load_text.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.vfb_instance_from_bits.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int,
                                          C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(vp)]
+    L.vfb_instance_from_words.argtypes = L.vfb_instance_from_bits.argtypes
+    L.vfb_instance_words.argtypes = [vp]
+    L.vfb_instance_words.restype = C.c_int
     L.vfb_instance_generate.argtypes = [C.c_int, C.c_uint64, C.POINTER(vp)]
     L.vfb_instance_write_json.argtypes = [vp, C.c_char_p]
     L.vfb_instance_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -185,12 +188,17 @@ class Instance:
 
     @classmethod
     def from_bits(cls, names, lens, atoms, num_pos):
-        """atoms[a][t]: word of AP a on trace t; traces 0..num_pos-1 positive."""
+        """atoms[a][t]: AP a on trace t as an integer, bit k = step k (a trace
+        of more than 64 steps is several CS words); traces 0..num_pos-1 are
+        positive."""
         A, T = len(names), len(lens)
-        flat = (C.c_uint64 * (A * T))(*[int(atoms[a][t]) for a in range(A) for t in range(T)])
+        nw = [(n + 63) // 64 for n in lens]
+        words = [(int(atoms[a][t]) >> (64 * w)) & 0xFFFFFFFFFFFFFFFF
+                 for a in range(A) for t in range(T) for w in range(nw[t])]
+        flat = (C.c_uint64 * max(1, len(words)))(*words)
         nm = (C.c_char_p * A)(*[n.encode() for n in names])
         h = C.c_void_p()
-        st = lib().vfb_instance_from_bits(A, nm, num_pos, T - num_pos, (C.c_int * T)(*lens), flat, C.byref(h))
+        st = lib().vfb_instance_from_words(A, nm, num_pos, T - num_pos, (C.c_int * T)(*lens), flat, C.byref(h))
         if st != OK:
             raise VfbError(f"from_bits: {status_string(st)}")
         return cls(h.value)
@@ -201,13 +209,18 @@ class Instance:
         return a.value, t.value, p.value
 
     def arrays(self):
-        """(names, lens, atoms[a][t], P)"""
+        """(names, lens, atoms[a][t] as integers (bit k = step k), P)"""
         A, T, P = self.shape()
+        W = lib().vfb_instance_words(self._h)
         lens = (C.c_int * T)()
-        flat = (C.c_uint64 * (A * T))()
+        flat = (C.c_uint64 * (A * W))()
         lib().vfb_instance_copy(self._h, lens, flat)
         names = [lib().vfb_instance_atom_name(self._h, a).decode() for a in range(A)]
-        atoms = [[flat[a * T + t] for t in range(T)] for a in range(A)]
+        col = [0]
+        for n in lens:
+            col.append(col[-1] + (n + 63) // 64)
+        atoms = [[sum(flat[a * W + c] << (64 * (c - col[t])) for c in range(col[t], col[t + 1]))
+                  for t in range(T)] for a in range(A)]
         return names, list(lens), atoms, P
 
     @property

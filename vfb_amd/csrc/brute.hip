@@ -208,6 +208,12 @@ static std::vector<uint64_t> num_formulas(int A, int max_size) {
 // VFB_CAPACITY if the count of some size overflows 64 bits before a match.
 vfb_status brute_solve(const vfb_instance& in, int max_size, int verbose, vfb_result* R) {
     const int A = in.A(), T = in.T();
+    for (int t = 0; t < T; ++t)
+        if (in.lens[t] > 64) {   // ltl_nocache's traces are one word (ltl_nocache.cu:12)
+            std::fprintf(stderr, "vfb: the brute-force search takes traces of at most 64 steps\n");
+            R->status = VFB_INVALID_INPUT;
+            return VFB_INVALID_INPUT;
+        }
     // atoms first (ltl_nocache.cu:367-378)
     for (int a = 0; a < A; ++a) {
         R->candidates[1]++;

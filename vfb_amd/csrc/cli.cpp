@@ -45,16 +45,18 @@ std::string ltlbs_syntax(const std::string& f) {
 void print_traces(const char* title, const vfb_instance* inst, int t0, int t1) {
     int A = 0, T = 0, P = 0;
     vfb_instance_shape(inst, &A, &T, &P);
-    std::vector<int> lens(T);
-    std::vector<uint64_t> atoms((size_t)A * T);
+    const int W = vfb_instance_words(inst);
+    std::vector<int> lens(T), col(T + 1, 0);
+    std::vector<uint64_t> atoms((size_t)A * W);
     vfb_instance_copy(inst, lens.data(), atoms.data());
+    for (int t = 0; t < T; ++t) col[t + 1] = col[t] + (lens[t] + 63) / 64;
     std::printf("\n%s: \n", title);
     for (int t = t0; t < t1; ++t) {
         std::printf("\t");
         for (int k = 0; k < lens[t]; ++k) {
             std::string s;
             for (int a = 0; a < A; ++a)
-                if ((atoms[(size_t)a * T + t] >> k) & 1) {
+                if ((atoms[(size_t)a * W + col[t] + k / 64] >> (k % 64)) & 1) {
                     if (!s.empty()) s += ", ";
                     s += vfb_instance_atom_name(inst, a);
                 }

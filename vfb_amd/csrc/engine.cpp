@@ -213,6 +213,7 @@ void Engine::release_all(Inst& I) {
     dfree(I, I.winbyte);
     dfree(I, I.blkcount);
     dfree(I, I.blkprefix);
+    dfree(I, I.cols);
     I.fresh_cap = I.nwords_cap = I.nblk_cap = 0;
     I.cap = 0;
 }
@@ -339,13 +340,63 @@ static void key_params(const vfb_instance& in, int relaxed_type, InstDesc& h) {
     }
 }
 
+// Key and mask parameters per column for rows whose traces span several
+// words: the same key as key_params (LTL_BS.cu:30-110), each trace's field or
+// samples split at its word boundaries; type 3 mixes each word on its own.
+static void col_params(const vfb_instance& in, int relaxed_type, const InstDesc& h, std::vector<ColInfo>* out) {
+    const int T = in.T();
+    int lenSum = 0;
+    for (int t = 0; t < T; ++t) lenSum += in.lens[t];
+    out->clear();
+    int j = 0;   // first key bit of the trace
+    for (int t = 0; t < T; ++t) {
+        const int len = in.lens[t], nw = trace_words(len);
+        int field = 0, nsamp = 0, off = std::min(j, 126);
+        if (h.key_kind == KEY_FIELD) {
+            const int mb = lenSum <= 126 ? len : (126 * len + lenSum) / lenSum;
+            field = lenSum <= 126 ? len : std::max(0, std::min(mb, 126 - j));
+            j += mb;
+        } else if (h.key_kind == KEY_STRIDE) {
+            const int n = (len - 1) / h.stride + 1;
+            nsamp = std::max(0, std::min(n, 126 - j));
+            j += n;
+        }
+        for (int w = 0; w < nw; ++w) {
+            ColInfo ci{};
+            ci.trace = t;
+            ci.word = (int16_t)w;
+            ci.lenmask = w == nw - 1 ? len_mask(len - 64 * w) : ~0ull;
+            if (h.key_kind == KEY_FIELD) {
+                const int m = std::max(0, std::min(64, field - 64 * w));
+                ci.kbits = (int16_t)m;
+                ci.kmask = len_mask(m);
+                ci.koff = (int16_t)(off + 64 * w);
+            } else if (h.key_kind == KEY_STRIDE) {
+                // samples q of the trace at steps q * stride inside this word
+                const int s = h.stride;
+                const int q0 = (64 * w + s - 1) / s;
+                const int end = std::min(len, 64 * w + 64);
+                const int q1 = std::min((end + s - 1) / s, nsamp);
+                ci.kbits = (int16_t)std::max(0, q1 - q0);
+                ci.kfirst = (int16_t)(q0 * s - 64 * w);
+                ci.koff = (int16_t)std::min(off + q0, 126);
+            }
+            out->push_back(ci);
+        }
+    }
+}
+
 void Engine::setup(Inst& I) {
     const vfb_instance& in = *I.in;
     const int A = in.A(), T = in.T();
     const int c1 = I.cost[0];           // the atoms' level (their cost)
+    const int C = in.C();
+    const std::vector<int> col = in.cols();
     std::memset(&I.h, 0, sizeof(I.h));
     I.h.T = T;
     I.h.P = in.P;
+    I.h.C = C;
+    I.h.mw = C > T ? 1 : 0;
     key_params(in, opt_.relaxed_type, I.h);
     I.rows.assign(kMaxS + 2, Buf{});
     I.nrows.assign(kMaxS + 2, 0);
@@ -356,7 +407,7 @@ void Engine::setup(Inst& I) {
         R.candidates[c1]++;
         bool ok = true;
         for (int t = 0; t < T && ok; ++t) {
-            bool b = in.atoms[(size_t)a * T + t] & 1;
+            bool b = in.atoms[(size_t)a * C + col[t]] & 1;
             if ((t < in.P) != b) ok = false;
         }
         if (ok) {
@@ -375,11 +426,30 @@ void Engine::setup(Inst& I) {
     R.level_winners[c1] = (uint64_t)A;
     R.level_candidates[c1] = (uint64_t)A;
     R.levels_done = c1;
-    std::vector<uint64_t> col((size_t)A * T);
+    if (I.h.mw) {
+        // column descriptors, then the first column of each trace
+        std::vector<ColInfo> ci;
+        col_params(in, opt_.relaxed_type, I.h, &ci);
+        const size_t n_ci = ci.size() * sizeof(ColInfo) / 8;
+        std::vector<uint64_t> blob(n_ci + (T + 2) / 2, 0);
+        std::memcpy(blob.data(), ci.data(), ci.size() * sizeof(ColInfo));
+        int32_t* tc = (int32_t*)(blob.data() + n_ci);
+        for (int t = 0; t <= T; ++t) tc[t] = col[t];
+        dalloc(I, I.cols, blob.size() * 8);
+        upload(I, I.cols.p, blob.data(), blob.size());
+        I.h.cinfo = (const ColInfo*)I.cols.p;
+        I.h.tcol = (const int32_t*)((const uint64_t*)I.cols.p + n_ci);
+    }
+    // column-major: word c of atom a at [c * A + a]; steps past a trace are 0
+    std::vector<uint64_t> rows((size_t)A * C);
     for (int a = 0; a < A; ++a)
-        for (int t = 0; t < T; ++t) col[(size_t)t * A + a] = in.atoms[(size_t)a * T + t] & len_mask(in.lens[t]);
-    dalloc(I, I.rows[c1], col.size() * 8);
-    upload(I, I.rows[c1].p, col.data(), col.size());
+        for (int t = 0; t < T; ++t)
+            for (int c = col[t]; c < col[t + 1]; ++c) {
+                const uint64_t m = c + 1 == col[t + 1] ? len_mask(in.lens[t] - 64 * (c - col[t])) : ~0ull;
+                rows[(size_t)c * A + a] = in.atoms[(size_t)a * C + c] & m;
+            }
+    dalloc(I, I.rows[c1], rows.size() * 8);
+    upload(I, I.rows[c1].p, rows.data(), rows.size());
     I.nrows[c1] = (uint32_t)A;
     I.stored = (uint64_t)A;
     I.level = c1;
@@ -537,7 +607,7 @@ static void pass_work(const Inst& I, uint64_t* cand, uint64_t* bytes) {
         const Unit& u = U[q];
         const uint64_t a = (u.op == VFB_OP_AND || u.op == VFB_OP_OR) ? 2 : 1;
         *cand += u.count;
-        *bytes += u.count * (a * 8 * (uint64_t)I.h.T + (checks_only(I) ? 0 : 16));
+        *bytes += u.count * (a * 8 * (uint64_t)I.h.C + (checks_only(I) ? 0 : 16));
     }
 }
 
@@ -618,7 +688,7 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
 std::string Engine::node_string(Inst& I, int level, uint64_t row) {
     if (level == I.cost[0] && !I.bs_mode) return I.in->names[row];
     uint64_t o = 0;
-    const uint64_t* par = (const uint64_t*)I.rows[level].p + (uint64_t)I.h.T * I.nrows[level];
+    const uint64_t* par = (const uint64_t*)I.rows[level].p + (uint64_t)I.h.C * I.nrows[level];
     CK(hipMemcpy(&o, par + row, 8, hipMemcpyDeviceToHost));
     // a BS leaf is the CS row it was taken from
     if (I.bs_mode && level <= I.ltl_levels) return node_string(*I.ltl, level, o);
@@ -679,7 +749,7 @@ void Engine::start_bs(Inst& I) {
     dalloc(I, words, std::max<uint64_t>(total, 1) * 8);
     uint64_t off = 0;
     for (int l = 1; l <= L; ++l) {
-        CK(launch_bs_words((const unsigned long long*)I.rows[l].p, I.nrows[l], T,
+        CK(launch_bs_words((const unsigned long long*)I.rows[l].p, I.nrows[l], T, I.h.mw ? I.h.tcol : nullptr,
                            (unsigned long long*)words.p + off, st_));
         off += I.nrows[l];
     }
@@ -703,6 +773,7 @@ void Engine::start_bs(Inst& I) {
     std::memset(&B->h, 0, sizeof(B->h));
     key_params(*B->bs_in, 3, B->h);      // one trace of T bits: the exact key is the word
     B->h.T = 1;
+    B->h.C = 1;
     B->h.P = 0;
     B->h.bs_mode = 1;
     B->h.bs_target = (1ull << I.in->P) - 1;
@@ -837,7 +908,9 @@ void Engine::epoch(std::vector<Inst*>& v) {
     if (waves) {
         time_begin(&ev, ekind);
         int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * tun_.expand_blocks_per_cu);
-        CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, grid, st_));
+        bool mw = false;
+        for (auto* I : v) mw |= I->h.mw != 0;
+        CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, mw ? 1 : 0, grid, st_));
         time_end(ev, ekind, v);
         for (auto* I : v) I->res->expand_launches++;
     }
@@ -1181,7 +1254,7 @@ void Engine::step(std::vector<Inst*>& act) {
             }
             // rows to keep: all survivors, or as many as max_rows / the budget allow
             const uint64_t W = I.ctr.winners;
-            const uint64_t row_bytes = 8 * (uint64_t)I.h.T + 8;
+            const uint64_t row_bytes = 8 * (uint64_t)I.h.C + 8;
             uint64_t keep = W;
             if (I.max_rows) keep = std::min(keep, I.max_rows > I.stored ? I.max_rows - I.stored : 0);
             uint64_t avail = I.budget > I.used ? I.budget - I.used : 0;
@@ -1213,12 +1286,12 @@ void Engine::step(std::vector<Inst*>& act) {
                 R.otf_level = s;
                 R.otf_stored = keep;
             }
-            R.bytes_store += keep * (8 * (uint64_t)I.h.T + 24);
+            R.bytes_store += keep * (8 * (uint64_t)I.h.C + 24);
             I.nrows[s] = (uint32_t)keep;
             I.h.out_n = keep;
             if (keep) {
                 I.h.out_rows = (unsigned long long*)I.rows[s].p;
-                I.h.out_par = (unsigned long long*)I.rows[s].p + (uint64_t)I.h.T * keep;
+                I.h.out_par = (unsigned long long*)I.rows[s].p + (uint64_t)I.h.C * keep;
                 st.push_back(&I);
             }
             const uint64_t N = std::max<uint64_t>(I.N, 1);
@@ -1339,6 +1412,7 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
         vfb_status worst = VFB_OK;
         for (auto* I : run) {
             const vfb_status s = brute_solve(*I->in, I->max_size, opt.verbose, I->res);
+            if (s == VFB_INVALID_INPUT && worst == VFB_OK) worst = s;
             if (s == VFB_DEVICE_ERROR) {
                 std::free(I->res->formula);
                 I->res->status = VFB_DEVICE_ERROR;
@@ -1352,6 +1426,7 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
             if (!out[k].formula) out[k].formula = strdup("Not found !");
         }
         if (worst == VFB_DEVICE_ERROR) return VFB_DEVICE_ERROR;
+        if (worst == VFB_INVALID_INPUT && n == 1) return VFB_INVALID_INPUT;
         return n == 1 ? out[0].status : VFB_OK;
     }
     try {

@@ -16,7 +16,7 @@
 
 namespace vfb {
 
-hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int wide, int grid,
+hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int wide, int mw, int grid,
                          hipStream_t st);
 hipError_t launch_insert_atoms(const InstDesc* D, uint32_t inst, uint32_t n, unsigned int* fail, hipStream_t st);
 hipError_t launch_rehash(const Slot* old, uint64_t old_cap, Slot* tab, uint64_t mask, uint64_t keep_below,
@@ -32,7 +32,8 @@ hipError_t launch_resolve_scan1(const InstDesc* D, hipStream_t st);
 hipError_t launch_insert_atoms_multi(const InstDesc* D, const Seg* segs, int nseg, uint64_t total, unsigned int* fail,
                                      hipStream_t st);
 hipError_t launch_fill_multi(const OpSeg* segs, int n, uint64_t total, int grid, hipStream_t st);
-hipError_t launch_bs_words(const unsigned long long* rows, uint32_t n, int T, unsigned long long* out, hipStream_t st);
+hipError_t launch_bs_words(const unsigned long long* rows, uint32_t n, int T, const int32_t* tcol,
+                           unsigned long long* out, hipStream_t st);
 hipError_t launch_insert_keys(Slot* tab, uint64_t mask, const unsigned long long* keys, uint64_t n, unsigned int* fail,
                               hipStream_t st);
 hipError_t launch_rehash_multi(const RehashSeg* segs, int n, uint64_t total, unsigned int* fail, int grid,
@@ -146,6 +147,7 @@ struct Inst {
     Buf table;
     uint64_t cap = 0, committed = 0;
     Buf fresh_ord, fresh_slot, winbyte, winbits, blkcount, blkprefix;
+    Buf cols;                   // multi-word rows: ColInfo[C] then tcol[T + 1]
     uint64_t fresh_cap = 0, nwords_cap = 0, nblk_cap = 0;
     double win_ratio = 1.0, fresh_ratio = 1.0;
     uint64_t N = 0;             // candidates of the level being built

@@ -1,10 +1,13 @@
 // Device-visible data structures of the CS enumeration engine.
 //
 // HBM layout (per instance):
-//   level block s : column-major uint64 words, word(t, row) = lvl[s][t * nrows[s] + row]
+//   level block s : column-major uint64 words, word(c, row) = lvl[s][c * nrows[s] + row]
 //                   so a wavefront whose 64 lanes take 64 consecutive rows reads
-//                   512 contiguous bytes per trace (coalesced), and a uniform row
-//                   (the left operand of a binary wave) is a broadcast.
+//                   512 contiguous bytes per column (coalesced), and a uniform row
+//                   (the left operand of a binary wave) is a broadcast.  Column c
+//                   is trace c when every trace has at most 64 steps; otherwise
+//                   trace t of n steps owns ceil(n/64) consecutive columns
+//                   tcol[t] .. tcol[t+1]-1, word w holding steps 64w..64w+63.
 //   parents       : one uint64 per stored row = its level-local canonical ordinal
 //                   (decodes to operator, operands), replacing the reference's
 //                   left/right index arrays and segment table (LTL_BS.cu:1189-1205).
@@ -42,6 +45,17 @@ struct alignas(32) Slot {
     unsigned long long hi, lo, ord, pad;
 };
 
+// One column of a row whose traces span several words (InstDesc::mw): its
+// trace, its word within the trace, the mask of its valid steps and its part
+// of the key (same meaning as InstDesc's per-trace kmask / koff / kbits;
+// kfirst = bit of the column's first KEY_STRIDE sample).
+struct ColInfo {
+    uint64_t lenmask;
+    uint64_t kmask;
+    int16_t koff, kbits, kfirst, word;
+    int32_t trace, pad;
+};
+
 struct Ctr {
     unsigned long long fresh_count;  // entries appended to the fresh list this epoch
     unsigned long long claims;       // slots claimed this epoch (= distinct new keys)
@@ -64,6 +78,8 @@ struct Unit {
 
 struct InstDesc {
     int T, P, key_kind, half;
+    int C;            // words per row (= T unless mw)
+    int mw;           // some trace spans several words: cinfo / tcol describe the columns
     int stride;       // KEY_STRIDE sampling stride
     int level;        // level being built
     int nunits;
@@ -74,6 +90,8 @@ struct InstDesc {
     uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
     int16_t koff[kMaxT];        // first key bit of the trace's field / samples
     int16_t kbits[kMaxT];       // field width (KEY_FIELD) or number of samples (KEY_STRIDE)
+    const ColInfo* cinfo;       // mw: [C] column descriptors
+    const int32_t* tcol;        // mw: [T + 1] first column of each trace
     const unsigned long long* lvl[kMaxS + 2];
     uint32_t nrows[kMaxS + 2];
     Slot* table;

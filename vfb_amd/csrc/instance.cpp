@@ -153,7 +153,7 @@ vfb_status validate(const vfb_instance& in, std::string* why) {
             *why = buf;
             return VFB_INVALID_INPUT;
         }
-    if (in.atoms.size() != (size_t)in.A() * in.T()) { *why = "atom table size mismatch"; return VFB_INVALID_INPUT; }
+    if (in.atoms.size() != (size_t)in.A() * in.C()) { *why = "atom table size mismatch"; return VFB_INVALID_INPUT; }
     return VFB_OK;
 }
 
@@ -319,7 +319,6 @@ extern "C" vfb_status vfb_instance_load_json(const char* path, vfb_instance** ou
         return VFB_INVALID_INPUT;
     }
     inst->lens.assign(T, 0);
-    inst->atoms.assign((size_t)A * T, 0);
     for (int t = 0; t < T; ++t) {
         const JVal* tr = traces[t];
         if (tr->k != JVal::Obj) return VFB_INVALID_INPUT;
@@ -336,16 +335,22 @@ extern "C" vfb_status vfb_instance_load_json(const char* path, vfb_instance** ou
                 std::fprintf(stderr, "This version supports traces of size 1..%d with equal lengths per AP.\n", VFB_MAX_LEN);
                 return VFB_INVALID_INPUT;
             }
-            uint64_t w = 0;
-            for (int k = 0; k < n; ++k) {
-                long long b;
-                if (!json_int(&vals->arr[k], &b)) return VFB_INVALID_INPUT;
-                if (b) w |= uint64_t(1) << k;
-            }
-            inst->atoms[(size_t)a * T + t] = w;
         }
         inst->lens[t] = len;
     }
+    // a trace of n steps is ceil(n / 64) words, step k in bit k % 64 of word k / 64
+    const int C = inst->C();
+    const std::vector<int> col = inst->cols();
+    inst->atoms.assign((size_t)A * C, 0);
+    for (int t = 0; t < T; ++t)
+        for (int a = 0; a < A; ++a) {
+            const JVal* vals = traces[t]->get(inst->names[a]);
+            for (int k = 0; k < inst->lens[t]; ++k) {
+                long long b;
+                if (!json_int(&vals->arr[k], &b)) return VFB_INVALID_INPUT;
+                if (b) inst->atoms[(size_t)a * C + col[t] + k / 64] |= uint64_t(1) << (k % 64);
+            }
+        }
     // the count fields are redundant; when present they must agree
     long long v;
     if (json_int(root.get("number_atomic_propositions"), &v) && v != A) return VFB_INVALID_INPUT;
@@ -424,17 +429,24 @@ extern "C" vfb_status vfb_instance_load_text(const char* path, vfb_instance** ou
     for (auto& t : pos) all.push_back(&t);
     for (auto& t : neg) all.push_back(&t);
     const int T = (int)all.size();
-    inst->atoms.assign((size_t)A * T, 0);
+    for (int t = 0; t < T; ++t) inst->lens.push_back((int)all[t]->size());
+    std::string why;
+    for (int n : inst->lens)
+        if (n < 1 || n > VFB_MAX_LEN) {
+            std::fprintf(stderr, "This version supports traces of 1..%d steps.\n", VFB_MAX_LEN);
+            return VFB_INVALID_INPUT;
+        }
+    const int C = inst->C();
+    const std::vector<int> col = inst->cols();
+    inst->atoms.assign((size_t)A * C, 0);
     for (int t = 0; t < T; ++t) {
         const auto& tr = *all[t];
-        inst->lens.push_back((int)tr.size());
-        for (size_t k = 0; k < tr.size() && k < 64; ++k)
+        for (size_t k = 0; k < tr.size(); ++k)
             for (char c : tr[k]) {
                 const int a = (int)(std::lower_bound(sorted.begin(), sorted.end(), c) - sorted.begin());
-                inst->atoms[(size_t)a * T + t] |= 1ull << k;
+                inst->atoms[(size_t)a * C + col[t] + k / 64] |= 1ull << (k % 64);
             }
     }
-    std::string why;
     if (validate(*inst, &why) != VFB_OK) {
         std::fprintf(stderr, "%s\n", why.c_str());
         return VFB_INVALID_INPUT;
@@ -452,6 +464,9 @@ extern "C" vfb_status vfb_instance_from_bits(int num_atoms, const char* const* a
     for (int a = 0; a < num_atoms; ++a)
         inst->names.push_back(atom_names && atom_names[a] ? std::string(atom_names[a]) : "a" + std::to_string(a));
     const int T = num_pos + num_neg;
+    if (T < 1 || T > VFB_MAX_TRACES) return VFB_INVALID_INPUT;
+    for (int t = 0; t < T; ++t)
+        if (lens[t] < 1 || lens[t] > 64) return VFB_INVALID_INPUT;   // one word per trace here
     inst->P = num_pos;
     inst->lens.assign(lens, lens + T);
     inst->atoms.assign(atoms, atoms + (size_t)num_atoms * T);
@@ -460,6 +475,36 @@ extern "C" vfb_status vfb_instance_from_bits(int num_atoms, const char* const* a
     if (st != VFB_OK) return st;
     for (int a = 0; a < num_atoms; ++a)
         for (int t = 0; t < T; ++t) inst->atoms[(size_t)a * T + t] &= len_mask(inst->lens[t]);
+    *out = inst.release();
+    return VFB_OK;
+}
+
+extern "C" vfb_status vfb_instance_from_words(int num_atoms, const char* const* atom_names, int num_pos,
+                                              int num_neg, const int* lens, const uint64_t* words,
+                                              vfb_instance** out) {
+    if (!out || !lens || !words || num_atoms < 1 || num_pos < 0 || num_neg < 0) return VFB_INVALID_INPUT;
+    *out = nullptr;
+    const int T = num_pos + num_neg;
+    if (T < 1 || T > VFB_MAX_TRACES) return VFB_INVALID_INPUT;
+    for (int t = 0; t < T; ++t)
+        if (lens[t] < 1 || lens[t] > VFB_MAX_LEN) return VFB_INVALID_INPUT;
+    auto inst = std::make_unique<vfb_instance>();
+    for (int a = 0; a < num_atoms; ++a)
+        inst->names.push_back(atom_names && atom_names[a] ? std::string(atom_names[a]) : "a" + std::to_string(a));
+    inst->P = num_pos;
+    inst->lens.assign(lens, lens + T);
+    const int C = inst->C();
+    const std::vector<int> col = inst->cols();
+    inst->atoms.assign(words, words + (size_t)num_atoms * C);
+    std::string why;
+    vfb_status st = validate(*inst, &why);
+    if (st != VFB_OK) return st;
+    // steps past a trace's end are cleared
+    for (int a = 0; a < num_atoms; ++a)
+        for (int t = 0; t < T; ++t) {
+            const int last = col[t + 1] - 1;
+            inst->atoms[(size_t)a * C + last] &= len_mask(inst->lens[t] - 64 * (last - col[t]));
+        }
     *out = inst.release();
     return VFB_OK;
 }
@@ -477,7 +522,8 @@ static void json_escape(std::string* o, const std::string& s) {
 extern "C" vfb_status vfb_instance_write_json(const vfb_instance* in, const char* path) {
     if (!in || !path) return VFB_INVALID_INPUT;
     std::string o = "{\n";
-    const int T = in->T();
+    const int T = in->T(), C = in->C();
+    const std::vector<int> col = in->cols();
     auto traces = [&](int t0, int t1) {
         o += "[";
         for (int t = t0; t < t1; ++t) {
@@ -488,7 +534,7 @@ extern "C" vfb_status vfb_instance_write_json(const vfb_instance* in, const char
                 o += ": [";
                 for (int k = 0; k < in->lens[t]; ++k) {
                     o += k ? "," : "";
-                    o += (in->atoms[(size_t)a * T + t] >> k) & 1 ? "1" : "0";
+                    o += (in->atoms[(size_t)a * C + col[t] + k / 64] >> (k % 64)) & 1 ? "1" : "0";
                 }
                 o += "]";
             }
@@ -528,6 +574,8 @@ extern "C" void vfb_instance_shape(const vfb_instance* in, int* A, int* T, int* 
     if (T) *T = in ? in->T() : 0;
     if (P) *P = in ? in->P : 0;
 }
+
+extern "C" int vfb_instance_words(const vfb_instance* in) { return in ? in->C() : 0; }
 
 extern "C" void vfb_instance_copy(const vfb_instance* in, int* lens, uint64_t* atoms) {
     if (!in) return;

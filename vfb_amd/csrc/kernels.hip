@@ -554,6 +554,208 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Traces longer than one word (InstDesc::mw).  A trace's words are evaluated
+// from its last word back so that X, F, G and U see its later steps: X takes
+// bit 0 of the next word into bit 63, F of a word is all ones once a later
+// word holds a step, G of a word is zero once a later step fails, and (l U r)
+// at step t is the in-word value or, if l holds from t to the word's end, the
+// value at the next word's step 0.  Key parts and masks are per column.
+// ---------------------------------------------------------------------------
+
+constexpr uint64_t kPhi = 0x9e3779b97f4a7c15ull;   // SplitMix64 increment
+
+// Folds column ci's word into the key.  KEY_MIX mixes word w of a trace as
+// dmix64(word + w * kPhi): for w = 0 the reference's per-trace mix
+// (LTL_BS.cu:78-88); the field and sample keys take the trace's bits at the
+// reference's global positions (LTL_BS.cu:45-76, 94-106).
+template <int KEY>
+__device__ __forceinline__ void key_acc_col(Key& k, uint64_t c, const ColInfo& ci, const InstDesc& d) {
+    if constexpr (KEY == KEY_MIX) {
+        const uint64_t x = dmix64(c + (uint64_t)ci.word * kPhi);
+        if (ci.trace < d.half) k.hi ^= x; else k.lo ^= x;
+    } else if constexpr (KEY == KEY_FIELD) {
+        const int m = ci.kbits;
+        if (m > 0) {
+            const uint64_t f = c & ci.kmask;
+            const int j = ci.koff;
+            if (j < 63) {
+                k.lo |= (f << j) & 0x7fffffffffffffffull;
+                if (j + m > 63) k.hi |= f >> (63 - j);
+            } else {
+                k.hi |= f << (j - 63);
+            }
+        }
+    } else {
+        const int n = ci.kbits;
+        const int s = d.stride;
+        int j = ci.koff;
+        for (int q = 0; q < n; ++q, ++j) {
+            const uint64_t b = (c >> (ci.kfirst + q * s)) & 1ull;
+            if (j < 63) k.lo |= b << j; else k.hi |= b << (j - 63);
+        }
+    }
+}
+
+// ~, X, F, G of one row over multi-word traces (SEL as in expand_unary)
+template <int KEY, bool CHECK, int G, int SEL>
+__device__ __forceinline__ void expand_unary_mw(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    constexpr int W = 64 / G;
+    const int sub = lane % G;
+    const uint64_t x = wl * W + (uint64_t)(lane / G);
+    const bool active = x < tk.nl;
+    const uint64_t xr = active ? x : 0;
+    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
+    Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    uint32_t mm = 0;
+    for (int t = sub; t < d.T; t += G) {
+        const int c0 = d.tcol[t];
+        const uint64_t want = t < d.P ? 1ull : 0ull;
+        uint64_t xin = 0;
+        bool later_any = false, later_all = true;
+        for (int c = d.tcol[t + 1] - 1; c >= c0; --c) {
+            const ColInfo& ci = d.cinfo[c];
+            const uint64_t m = ci.lenmask;
+            const uint64_t a = L[(uint64_t)c * tk.nl + xr];
+            const uint64_t v0 = ~a & m;                                  // ~
+            const uint64_t v1 = (a >> 1) | (xin << 63);                  // X
+            const uint64_t v2 = later_any ? m : suffix_or(a);            // F
+            const uint64_t v3 = later_all ? (a & ~suffix_or(v0)) : 0ull; // G
+            xin = a & 1ull;
+            later_any = later_any || a != 0ull;
+            later_all = later_all && a == m;
+            if (c == c0)
+                mm |= miss(v0, want, 1ull) | (miss(v1, want, 1ull) << 1) | (miss(v2, want, 1ull) << 2) |
+                      (miss(v3, want, 1ull) << 3);
+            if constexpr (!CHECK) {
+                if constexpr ((SEL & 1) != 0) key_acc_col<KEY>(k[0], v0, ci, d);
+                if constexpr ((SEL & 0xe) != 0) {
+                    key_acc_col<KEY>(k[1], v1, ci, d);
+                    key_acc_col<KEY>(k[2], v2, ci, d);
+                    key_acc_col<KEY>(k[3], v3, ci, d);
+                }
+            }
+        }
+    }
+    const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part) : 0u, lane);
+}
+
+// &, |, l U r, r U l of one pair over multi-word traces
+template <int KEY, bool CHECK, int G, int SEL>
+__device__ __forceinline__ void expand_binary_mw(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    constexpr int W = 64 / G;
+    const int sub = lane % G;
+    const uint64_t l = wl / tk.nchunk;
+    const uint64_t rc = wl - l * tk.nchunk;
+    const uint64_t r = rc * W + (uint64_t)(lane / G);
+    const bool active = r < tk.nr;
+    const uint64_t rr = active ? r : 0;
+    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
+    const unsigned long long* __restrict__ R = d.lvl[tk.lvl_r];
+    Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    uint32_t mm = 0;
+    constexpr bool kU = CHECK || (SEL & 0xc) != 0;
+    for (int t = sub; t < d.T; t += G) {
+        const int c0 = d.tcol[t];
+        const uint64_t want = t < d.P ? 1ull : 0ull;
+        uint64_t u2 = 0, u3 = 0;
+        for (int c = d.tcol[t + 1] - 1; c >= c0; --c) {
+            const ColInfo& ci = d.cinfo[c];
+            const uint64_t a = L[(uint64_t)c * tk.nl + l];
+            const uint64_t b = R[(uint64_t)c * tk.nr + rr];
+            const uint64_t v0 = a & b, v1 = a | b;
+            uint64_t v2 = 0, v3 = 0;
+            if constexpr (kU) {
+                v2 = until_op(a, b) | (u2 ? ~suffix_or(~a) : 0ull);
+                v3 = until_op(b, a) | (u3 ? ~suffix_or(~b) : 0ull);
+                u2 = v2 & 1ull;
+                u3 = v3 & 1ull;
+            }
+            if (c == c0)
+                mm |= miss(v0, want, 1ull) | (miss(v1, want, 1ull) << 1) | (miss(v2, want, 1ull) << 2) |
+                      (miss(v3, want, 1ull) << 3);
+            if constexpr (!CHECK) {
+                if constexpr ((SEL & 1) != 0) key_acc_col<KEY>(k[0], v0, ci, d);
+                if constexpr ((SEL & 2) != 0) key_acc_col<KEY>(k[1], v1, ci, d);
+                if constexpr ((SEL & 0xc) != 0) {
+                    key_acc_col<KEY>(k[2], v2, ci, d);
+                    key_acc_col<KEY>(k[3], v3, ci, d);
+                }
+            }
+        }
+    }
+    const uint64_t pair = l * tk.nr + r;
+    const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
+                             tk.base[2] + 2 * pair + 1};
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part) : 0u, lane);
+}
+
+template <int KEY, int G>
+__device__ __forceinline__ void expand_unary_mw_part(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    if (tk.part == SEL_NOT) expand_unary_mw<KEY, false, G, SEL_NOT>(d, tk, wl, lane);
+    else if (tk.part == SEL_XFG) expand_unary_mw<KEY, false, G, SEL_XFG>(d, tk, wl, lane);
+    else expand_unary_mw<KEY, false, G, SEL_ALL>(d, tk, wl, lane);
+}
+
+template <int KEY, int G>
+__device__ __forceinline__ void expand_binary_mw_part(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    switch (tk.part) {
+        case SEL_AND: expand_binary_mw<KEY, false, G, SEL_AND>(d, tk, wl, lane); break;
+        case SEL_ANDOR: expand_binary_mw<KEY, false, G, SEL_ANDOR>(d, tk, wl, lane); break;
+        case SEL_ORU: expand_binary_mw<KEY, false, G, SEL_ORU>(d, tk, wl, lane); break;
+        case SEL_U: expand_binary_mw<KEY, false, G, SEL_U>(d, tk, wl, lane); break;
+        default: expand_binary_mw<KEY, false, G, SEL_ALL>(d, tk, wl, lane); break;
+    }
+}
+
+// k_expand for epochs holding an instance with multi-word traces: the same
+// work mapping, each instance by its own path.  A separate kernel, so that the
+// one-word kernel's register allocation is that of its own path alone.
+template <int G>
+__global__ __launch_bounds__(256) void k_expand_mw(const InstDesc* __restrict__ D, const Task* __restrict__ tasks,
+                                                   int ntasks, uint64_t total_waves) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t wpb = blockDim.x >> 6;
+    const uint64_t stride = (uint64_t)gridDim.x * wpb;
+    for (uint64_t w = (uint64_t)blockIdx.x * wpb + wib; w < total_waves; w += stride) {
+        int lo = 0, hi = ntasks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tasks[mid].wave_begin <= w) lo = mid; else hi = mid - 1;
+        }
+        const Task& tk = tasks[lo];
+        const InstDesc& d = D[tk.inst];
+        const uint64_t wl = w - tk.wave_begin;
+        if (!d.mw) {
+            if (d.check_only) {
+                if (tk.kind == 0) expand_unary<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
+                else expand_binary<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
+            } else if (tk.kind == 0) {
+                if (d.key_kind == KEY_MIX) expand_unary_part<KEY_MIX, G>(d, tk, wl, lane);
+                else if (d.key_kind == KEY_FIELD) expand_unary_part<KEY_FIELD, G>(d, tk, wl, lane);
+                else expand_unary_part<KEY_STRIDE, G>(d, tk, wl, lane);
+            } else {
+                if (d.key_kind == KEY_MIX) expand_binary_part<KEY_MIX, G>(d, tk, wl, lane);
+                else if (d.key_kind == KEY_FIELD) expand_binary_part<KEY_FIELD, G>(d, tk, wl, lane);
+                else expand_binary_part<KEY_STRIDE, G>(d, tk, wl, lane);
+            }
+        } else if (d.check_only) {
+            if (tk.kind == 0) expand_unary_mw<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
+            else expand_binary_mw<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
+        } else if (tk.kind == 0) {
+            if (d.key_kind == KEY_MIX) expand_unary_mw_part<KEY_MIX, G>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_unary_mw_part<KEY_FIELD, G>(d, tk, wl, lane);
+            else expand_unary_mw_part<KEY_STRIDE, G>(d, tk, wl, lane);
+        } else {
+            if (d.key_kind == KEY_MIX) expand_binary_mw_part<KEY_MIX, G>(d, tk, wl, lane);
+            else if (d.key_kind == KEY_FIELD) expand_binary_mw_part<KEY_FIELD, G>(d, tk, wl, lane);
+            else expand_binary_mw_part<KEY_STRIDE, G>(d, tk, wl, lane);
+        }
+    }
+}
+
 // index of the segment holding flat index g (segments sorted by begin)
 template <class S>
 __device__ __forceinline__ int seg_of(const S* segs, int n, uint64_t g) {
@@ -571,7 +773,11 @@ template <int KEY>
 __device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned int* fail) {
     const unsigned long long* L = d.lvl[1];
     Key k{0, 0};
-    for (int t = 0; t < d.T; ++t) key_acc<KEY>(k, L[(uint64_t)t * n + x], d, t);
+    if (d.mw) {
+        for (int c = 0; c < d.C; ++c) key_acc_col<KEY>(k, L[(uint64_t)c * n + x], d.cinfo[c], d);
+    } else {
+        for (int t = 0; t < d.T; ++t) key_acc<KEY>(k, L[(uint64_t)t * n + x], d, t);
+    }
     key_fin(k);
     uint32_t slot;
     int claimed = 0;
@@ -579,13 +785,14 @@ __device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned i
 }
 
 // BS stage: bit 0 of every trace of each stored row, packed into one word
-// (transfer, LTL_BS.cu:808-822); rows are T columns of n words
+// (transfer, LTL_BS.cu:808-822); rows are columns of n words, trace t's
+// step 0 in column tcol[t] (column t when tcol is null)
 __global__ void k_bs_words(const unsigned long long* __restrict__ rows, uint32_t n, int T,
-                           unsigned long long* __restrict__ out) {
+                           const int32_t* __restrict__ tcol, unsigned long long* __restrict__ out) {
     const uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (r >= n) return;
     uint64_t w = 0;
-    for (int t = 0; t < T; ++t) w |= (rows[(uint64_t)t * n + r] & 1ull) << t;
+    for (int t = 0; t < T; ++t) w |= (rows[(uint64_t)(tcol ? tcol[t] : t) * n + r] & 1ull) << t;
     out[r] = w;
 }
 
@@ -907,6 +1114,39 @@ __device__ __forceinline__ uint64_t cand_word(const Cand& c, int t, uint64_t m) 
     return apply_op(c.op, a, b, m);
 }
 
+// trace t of candidate c over its words (multi-word rows), last word first
+__device__ __forceinline__ void store_trace_mw(const InstDesc& d, const Cand& cd, int t, unsigned long long* out,
+                                               uint64_t out_n, uint64_t row) {
+    const int c0 = d.tcol[t];
+    const bool unary = cd.op == VFB_OP_NOT || cd.op == VFB_OP_NEXT || cd.op == VFB_OP_FINALLY ||
+                       cd.op == VFB_OP_GLOBALLY;
+    uint64_t carry = 0;
+    bool later_any = false, later_all = true;
+    for (int c = d.tcol[t + 1] - 1; c >= c0; --c) {
+        const uint64_t m = d.cinfo[c].lenmask;
+        uint64_t a = cd.L[(uint64_t)c * cd.nl + cd.l];
+        uint64_t b = unary ? 0ull : cd.R[(uint64_t)c * cd.nr + cd.r];
+        if (cd.swap) { const uint64_t x = a; a = b; b = x; }   // r U l
+        uint64_t v;
+        switch (cd.op) {
+        case VFB_OP_NOT: v = ~a & m; break;
+        case VFB_OP_AND: v = a & b; break;
+        case VFB_OP_OR: v = a | b; break;
+        case VFB_OP_NEXT: v = (a >> 1) | (carry << 63); carry = a & 1ull; break;
+        case VFB_OP_FINALLY: v = later_any ? m : suffix_or(a); later_any = later_any || a != 0ull; break;
+        case VFB_OP_GLOBALLY:
+            v = later_all ? (a & ~suffix_or(~a & m)) : 0ull;
+            later_all = later_all && a == m;
+            break;
+        default:
+            v = until_op(a, b) | (carry ? ~suffix_or(~a) : 0ull);
+            carry = v & 1ull;
+            break;
+        }
+        out[(uint64_t)c * out_n + row] = v;
+    }
+}
+
 // stable compaction of the survivors.  Work item = (chunk of 4096 ordinals,
 // one of kStoreParts interleaved survivor subsets, slice of kStoreTraces
 // traces), one wavefront each: the chunk's set bits
@@ -957,7 +1197,11 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
             const uint64_t o = obase + list[i];
             const Cand cd = decode_from(d, o, u0);
             const uint64_t row = rank0 + i;
-            for (int t = t0; t < t1; ++t) out_rows[(uint64_t)t * out_n + row] = cand_word(cd, t, d.lenmask[t]);
+            if (d.mw) {
+                for (int t = t0; t < t1; ++t) store_trace_mw(d, cd, t, out_rows, out_n, row);
+            } else {
+                for (int t = t0; t < t1; ++t) out_rows[(uint64_t)t * out_n + row] = cand_word(cd, t, d.lenmask[t]);
+            }
             if (t0 == 0) out_par[row] = o;
         }
         __builtin_amdgcn_wave_barrier();
@@ -968,9 +1212,13 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
 // launch wrappers (host)
 // ---------------------------------------------------------------------------
 
-hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int wide, int grid,
-                         hipStream_t st) {
-    if (wide)
+hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int wide, int mw,
+                         int grid, hipStream_t st) {
+    if (mw && wide)
+        hipLaunchKernelGGL(k_expand_mw<kWideLanes>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
+    else if (mw)
+        hipLaunchKernelGGL(k_expand_mw<1>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
+    else if (wide)
         hipLaunchKernelGGL(k_expand<kWideLanes>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
     else
         hipLaunchKernelGGL(k_expand<1>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
@@ -984,8 +1232,9 @@ hipError_t launch_insert_atoms_multi(const InstDesc* D, const Seg* segs, int nse
     return hipGetLastError();
 }
 
-hipError_t launch_bs_words(const unsigned long long* rows, uint32_t n, int T, unsigned long long* out, hipStream_t st) {
-    if (n) hipLaunchKernelGGL(k_bs_words, dim3((n + 255) / 256), dim3(256), 0, st, rows, n, T, out);
+hipError_t launch_bs_words(const unsigned long long* rows, uint32_t n, int T, const int32_t* tcol,
+                           unsigned long long* out, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_bs_words, dim3((n + 255) / 256), dim3(256), 0, st, rows, n, T, tcol, out);
     return hipGetLastError();
 }
 

@@ -7,16 +7,32 @@
 
 #include "../../include/vfb.h"
 
+// 64-bit words of a trace of len steps (word w holds steps 64w..64w+63)
+inline int trace_words(int len) { return (len + 63) / 64; }
+
 struct vfb_instance {
     std::vector<std::string> names;   // AP names, in input order
     std::vector<int> lens;            // per trace, positives first
-    std::vector<uint64_t> atoms;      // [a * T + t]
+    std::vector<uint64_t> atoms;      // [a * C() + cols()[t] + w]: word w of trace t
+                                      // (= [a * T + t] when every trace has <= 64 steps)
     int P = 0;                        // positive traces
     std::string planted;              // generating formula (generated instances)
     uint64_t seed = 0;                // generating seed
     int default_max_size = 0;         // config default (generated instances)
     int A() const { return (int)names.size(); }
     int T() const { return (int)lens.size(); }
+    // words per row: the traces' words in trace order
+    int C() const {
+        int c = 0;
+        for (int n : lens) c += trace_words(n);
+        return c;
+    }
+    // first word of each trace, and C() last ([T + 1])
+    std::vector<int> cols() const {
+        std::vector<int> c(lens.size() + 1, 0);
+        for (size_t t = 0; t < lens.size(); ++t) c[t + 1] = c[t] + trace_words(lens[t]);
+        return c;
+    }
 };
 
 namespace vfb {
