@@ -193,6 +193,47 @@ def limits(V, ks, max_size):
     return out
 
 
+def _stutter(w, n, k):
+    """An n-step trace with every step held for k steps (k * n steps)."""
+    out = 0
+    for t in range(n):
+        if (w >> t) & 1:
+            out |= ((1 << k) - 1) << (k * t)
+    return out
+
+
+def multiword(V, n, stretch, max_size):
+    """Traces over one CS word: config-2 instances 0..n-1 with every step held
+    for `stretch` steps (traces of 16*stretch..32*stretch steps, labels kept),
+    plain solves.  The combine+dedup kernel is k_expand_mw; its algorithmic
+    bytes are (a * 8W + 16) per candidate, W = the row's words."""
+    insts, words = [], []
+    for k in range(n):
+        names, lens, atoms, P = V.Instance.generate(2, k).arrays()
+        st = [[_stutter(w, L, stretch) for w, L in zip(row, lens)] for row in atoms]
+        inst = V.Instance.from_bits(names, [L * stretch for L in lens], st, P)
+        insts.append(inst)
+        words.append(V.lib().vfb_instance_words(inst.handle))
+    V.solve(insts[0], max_size=max_size)          # warm
+    t0 = time.perf_counter()
+    rs = [V.solve(i, max_size=max_size, timing=2) for i in insts]
+    dt = time.perf_counter() - t0
+    ms = sum(r["ms_expand"] for r in rs)
+    by = sum(r["bytes_expand"] for r in rs)
+    ach = by / (ms * 1e-3) / 1e9 if ms else None
+    st = {}
+    for r in rs:
+        st[V.status_string(r["status"])] = st.get(V.status_string(r["status"]), 0) + 1
+    return {"workload": f"config-2 instances 0..{n - 1}, every step held {stretch}x "
+                        f"({16 * stretch}-{32 * stretch} steps per trace), max_formula_size {max_size}",
+            "words_per_row": [min(words), max(words)], "seconds": dt,
+            "distinct_per_s": sum(sum(r["distinct"]) for r in rs) / dt,
+            "candidates_per_s": sum(sum(r["level_candidates"]) for r in rs) / dt,
+            "statuses": st, "min_sizes": [r["min_size"] for r in rs],
+            "k_expand_mw_ms": ms, "k_expand_mw_launches": sum(r["expand_launches"] for r in rs),
+            "k_expand_mw_alg_GBps": ach, "k_expand_mw_frac_of_hbm": ach / HBM_PEAK_GBS if ach else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,7 +249,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-active", type=int, default=128)
     ap.add_argument("--limits", default="4,10", help="config-3 instances to time (empty: skip)")
-    ap.add_argument("--only", choices=["main", "limits", "mix"], default=None,
+    ap.add_argument("--multiword", default="8,8,12",
+                    help="instances,stretch,max_size of the long-trace workload (empty: skip)")
+    ap.add_argument("--only", choices=["main", "limits", "mix", "multiword"], default=None,
                     help="run one section alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -227,6 +270,10 @@ def main():
         return
     if args.only == "mix":
         print(json.dumps({"batched_mix": batched_mix(V, args.mix, 16)}))
+        return
+    if args.only == "multiword":
+        n, k, m = (int(x) for x in args.multiword.split(","))
+        print(json.dumps({"multiword": multiword(V, n, k, m)}))
         return
     insts = [V.Instance.generate(2, k) for k in range(args.instances)]
 
@@ -311,6 +358,10 @@ def main():
     if args.limits:
         print("[bench] limits config", file=sys.stderr, flush=True)
         out["limits"] = limits(V, [int(x) for x in args.limits.split(",")], 18)
+    if args.multiword:
+        print("[bench] multi-word traces", file=sys.stderr, flush=True)
+        n, k, m = (int(x) for x in args.multiword.split(","))
+        out["multiword"] = multiword(V, n, k, m)
     if args.mix:
         print("[bench] batched mix", file=sys.stderr, flush=True)
         out["batched_mix"] = batched_mix(V, args.mix, 16)
