@@ -237,10 +237,31 @@ class Instance:
             raise VfbError(f"write_json: {status_string(st)}")
 
 
+_DEFAULT_OPTS = None
+_OPTS_CACHE = {}
+
+
 def _opts(max_size, relaxed_type, timing, budget, verbose, max_rows=0, search=SEARCH_CS, bs_max_size=0,
           costs=None, neg_type=1):
-    o = Options()
-    lib().vfb_options_default(C.byref(o))
+    """vfb_options for these arguments (the library only reads them, so equal
+    arguments share one structure)."""
+    key = (max_size, relaxed_type, timing, budget, verbose, max_rows, search, bs_max_size,
+           None if costs is None else tuple(costs), neg_type)
+    o = _OPTS_CACHE.get(key)
+    if o is None:
+        o = _make_opts(*key)
+        if len(_OPTS_CACHE) < 256:
+            _OPTS_CACHE[key] = o
+    return o
+
+
+def _make_opts(max_size, relaxed_type, timing, budget, verbose, max_rows, search, bs_max_size, costs, neg_type):
+    global _DEFAULT_OPTS
+    if _DEFAULT_OPTS is None:
+        d = Options()
+        lib().vfb_options_default(C.byref(d))
+        _DEFAULT_OPTS = bytes(d)
+    o = Options.from_buffer_copy(_DEFAULT_OPTS)
     o.search = search
     o.bs_max_size = bs_max_size
     if costs is not None:
@@ -262,10 +283,10 @@ def _todict(r):
     return {
         "status": r.status, "min_size": r.min_size, "top_op": r.top_op,
         "levels_done": r.levels_done,
-        "distinct": [r.distinct[i] for i in range(n)],
-        "candidates": [r.candidates[i] for i in range(n)],
-        "level_winners": [r.level_winners[i] for i in range(n)],
-        "level_candidates": [r.level_candidates[i] for i in range(n)],
+        "distinct": r.distinct[:n],
+        "candidates": r.candidates[:n],
+        "level_winners": r.level_winners[:n],
+        "level_candidates": r.level_candidates[:n],
         "found_ordinal": r.found_ordinal,
         "formula": (r.formula or b"").decode(),
         "ms_total": r.ms_total, "ms_expand": r.ms_expand,

@@ -52,6 +52,7 @@ Engine::~Engine() {
     if (d_step_) (void)hipFree(d_step_);
     if (d_ops_) (void)hipFree(d_ops_);
     if (h_flag_) (void)hipHostFree(h_flag_);
+    if (h_ctr_) (void)hipHostFree(h_ctr_);
     if (d_flag_) (void)hipFree(d_flag_);
     if (pin_) (void)hipHostFree(pin_);
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
@@ -944,10 +945,18 @@ void Engine::epoch(std::vector<Inst*>& v) {
         }
         time_end(ev, 1, dd);
     }
-    std::vector<Ctr> back(n);
-    CK(hipMemcpyAsync(back.data(), d + o_ctr, n * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
+    // pinned read-back: the copy stays asynchronous, and the stream sync
+    // below is the level's one host round trip
+    if (n > h_ctr_cap_) {
+        sync();
+        if (h_ctr_) CK(hipHostFree(h_ctr_));
+        h_ctr_ = nullptr;
+        h_ctr_cap_ = std::max<size_t>(n, std::max<size_t>(2 * h_ctr_cap_, 64));
+        CK(hipHostMalloc((void**)&h_ctr_, h_ctr_cap_ * sizeof(Ctr), hipHostMallocDefault));
+    }
+    CK(hipMemcpyAsync(h_ctr_, d + o_ctr, n * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
     sync();
-    for (size_t k = 0; k < n; ++k) v[k]->ctr = back[k];
+    for (size_t k = 0; k < n; ++k) v[k]->ctr = h_ctr_[k];
 }
 
 void Engine::store(std::vector<Inst*>& v) {

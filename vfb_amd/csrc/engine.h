@@ -201,6 +201,8 @@ class Engine {
     size_t step_cap_ = 0;
     unsigned int* d_flag_ = nullptr;   // probe-limit failure of rehash / atom insertion
     unsigned int* h_flag_ = nullptr;   // its pinned host copy
+    Ctr* h_ctr_ = nullptr;        // pinned read-back of an epoch's counters
+    size_t h_ctr_cap_ = 0;
     bool flag_dirty_ = false;
     std::vector<void*> pending_free_;
     // batch mode: fills, uploads and rehashes of many instances are queued and
