@@ -627,11 +627,14 @@ void Engine::prepare_scratch(Inst& I) {
         I.fresh_cap = fcap;
     }
     if (nwords > I.nwords_cap) {
-        ensure(I, I.winbits, std::max<uint64_t>(nwords, 1) * 8);
-        ensure(I, I.winbyte, std::max<uint64_t>(nwords, 1) * 64);
+        // at least 64K candidates' worth, so that a solve's small levels
+        // share one clearing fill
+        const uint64_t nw = std::max<uint64_t>(nwords, kMinWinWords);
+        ensure(I, I.winbits, nw * 8);
+        ensure(I, I.winbyte, nw * 64);
         // cleared once here; the count / scan kernels clear what they pack
-        fill(I, I.winbyte.p, std::max<uint64_t>(nwords, 1) * 8, 0);
-        I.nwords_cap = nwords;
+        fill(I, I.winbyte.p, nw * 8, 0);
+        I.nwords_cap = nw;
     }
     if (nblk > I.nblk_cap) {
         ensure(I, I.blkcount, std::max<uint64_t>(nblk, 1) * 4);
@@ -878,6 +881,17 @@ void Engine::epoch(std::vector<Inst*>& v) {
     const size_t o_ids = o_bseg + align256(std::max<size_t>(bsegs.size(), 1) * sizeof(Seg));
     const size_t bytes = o_ids + align256(std::max<size_t>(ids.size(), 1) * 4);
     char* d = step_buf(bytes);
+    // host-coherent counters: when every instance's epoch ends with a scan,
+    // the scan writes them there and no read-back copy is needed
+    if (n > h_ctr_cap_) {
+        sync();
+        if (h_ctr_) CK(hipHostFree(h_ctr_));
+        h_ctr_ = nullptr;
+        h_ctr_cap_ = std::max<size_t>(n, std::max<size_t>(2 * h_ctr_cap_, 64));
+        CK(hipHostMalloc((void**)&h_ctr_, h_ctr_cap_ * sizeof(Ctr), hipHostMallocCoherent));
+    }
+    bool all_scan = true;
+    for (auto* I : v) all_scan &= !checks_only(*I);
     std::vector<char> host(bytes, 0);
     InstDesc* hd = (InstDesc*)host.data();
     Ctr* hc = (Ctr*)(host.data() + o_ctr);
@@ -887,6 +901,7 @@ void Engine::epoch(std::vector<Inst*>& v) {
         I.h.mark = 1;
         I.h.found_below = I.pass == P_FIRST ? I.first_prefix : ~0ull;
         hd[k] = I.h;
+        hd[k].ctr_host = all_scan ? h_ctr_ + k : nullptr;
         hd[k].check_only = checks_only(I) ? 1 : 0;
         if (I.pass != P_REST) {
             hc[k] = Ctr{};          // first pass of the level's epoch: fresh counters
@@ -945,16 +960,9 @@ void Engine::epoch(std::vector<Inst*>& v) {
         }
         time_end(ev, 1, dd);
     }
-    // pinned read-back: the copy stays asynchronous, and the stream sync
-    // below is the level's one host round trip
-    if (n > h_ctr_cap_) {
-        sync();
-        if (h_ctr_) CK(hipHostFree(h_ctr_));
-        h_ctr_ = nullptr;
-        h_ctr_cap_ = std::max<size_t>(n, std::max<size_t>(2 * h_ctr_cap_, 64));
-        CK(hipHostMalloc((void**)&h_ctr_, h_ctr_cap_ * sizeof(Ctr), hipHostMallocDefault));
-    }
-    CK(hipMemcpyAsync(h_ctr_, d + o_ctr, n * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
+    // otherwise (a check-only pass) an asynchronous read-back; the stream sync
+    // is the level's one host round trip
+    if (!all_scan) CK(hipMemcpyAsync(h_ctr_, d + o_ctr, n * sizeof(Ctr), hipMemcpyDeviceToHost, st_));
     sync();
     for (size_t k = 0; k < n; ++k) v[k]->ctr = h_ctr_[k];
 }

@@ -38,6 +38,7 @@ constexpr uint64_t kWideMaxWaves = 2048; // levels below this many wavefronts us
 constexpr int kTraceGroup = 8;        // traces whose operand loads k_expand issues together
 constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are counted in k_scan
 constexpr uint64_t kSmallFresh = 16384; // fresh lists up to this long are resolved in the scan's workgroup
+constexpr uint64_t kMinWinWords = 1024; // winner maps are allocated for at least 64K candidates
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 
@@ -113,6 +114,7 @@ struct InstDesc {
     unsigned long long* out_par;
     uint64_t out_n;             // rows stored of this level (survivors past it are dropped)
     Ctr* ctr;
+    Ctr* ctr_host;              // host-coherent copy the scan writes at the end of the epoch (or null)
     Unit units[kMaxUnits];
 };
 

@@ -1040,6 +1040,16 @@ __device__ void scan_one(const InstDesc& d) {
         if (threadIdx.x == 0) {
             d.ctr->winners = carry;
             d.ctr->found_rank = rank;
+            if (d.ctr_host) {
+                // the epoch's outcome straight to host memory: no read-back copy
+                Ctr* h = d.ctr_host;
+                h->fresh_count = d.ctr->fresh_count;
+                h->claims = d.ctr->claims;
+                h->found = f;
+                h->winners = carry;
+                h->found_rank = rank;
+                h->overflow = d.ctr->overflow;
+            }
         }
     }
 }
