@@ -148,3 +148,23 @@ def test_multiword_brute_search_rejected():
     inst, _ = long_instance(1400)
     r = V.solve(inst, max_size=4, search=V.SEARCH_BRUTE)
     assert r["status"] == V.INVALID_INPUT
+
+
+def test_multiword_parked_batch_and_server():
+    """Multi-word instances in a batch that parks every member (park_min=1)
+    and through the server: each result equals the plain solve."""
+    insts = [long_instance(1500 + k)[0] for k in range(6)] + [V.Instance.generate(4, k) for k in range(4)]
+    plain = [V.solve(i, max_size=7) for i in insts]
+    old = V.set_tuning(park_min=1)
+    try:
+        print("[parity] multiword parked batch", flush=True)
+        batch = V.solve_batch(insts, max_size=7)
+    finally:
+        V.set_tuning(**old)
+    print("[parity] multiword server", flush=True)
+    with V.Server(max_size=7, max_active=4) as srv:
+        got = [srv.wait(t) for t in [srv.submit(i) for i in insts]]
+    for k in range(len(insts)):
+        for f in FIELDS + ("level_winners",):
+            assert batch[k][f] == plain[k][f], ("batch", k, f)
+            assert got[k][f] == plain[k][f], ("server", k, f)
