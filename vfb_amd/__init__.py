@@ -17,6 +17,9 @@ LIB_PATH = os.path.join(_HERE, "lib", "libvfb.so")
 CLI_PATH = os.path.join(_HERE, "bin", "vfb")
 
 MAX_SIZE = 32
+MAX_TRACES = 128
+MAX_ATOMS = 64
+MAX_LEN = 4096
 OK, NOT_FOUND, INVALID_INPUT, CAPACITY, DEVICE_ERROR, IO_ERROR = range(6)
 OP_NAMES = ["~", "&", "|", "X", "F", "G", "U", "atom"]
 
