@@ -1507,7 +1507,7 @@ extern "C" vfb_status vfb_solve_batch(const vfb_instance* const* insts, size_t n
 
 extern "C" void vfb_tuning_default(vfb_tuning* t) {
     if (!t) return;
-    t->pass_min = 1ull << 24;
+    t->pass_min = 1ull << 22;
     t->wide_max_waves = kWideMaxWaves;
     t->expand_blocks_per_cu = 8;
     t->resolve_blocks_per_cu = 16;
