@@ -34,7 +34,7 @@ constexpr int kProbeLimit = 4096;
 constexpr int kStoreTraces = 8;       // traces per k_store work item
 constexpr int kStoreParts = 4;        // survivor subsets per chunk in k_store
 constexpr int kWideLanes = 8;         // lanes per candidate in k_expand's small-level mode
-constexpr uint64_t kWideMaxWaves = 2048; // levels below this many wavefronts use it
+constexpr uint64_t kWideMaxWaves = 512;  // levels below this many wavefronts use it (default)
 constexpr int kTraceGroup = 8;        // traces whose operand loads k_expand issues together
 constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are counted in k_scan
 constexpr uint64_t kSmallFresh = 16384; // fresh lists up to this long are resolved in the scan's workgroup
