@@ -148,7 +148,7 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     const double t_default = time_setting(cur);
     double t_best = t_default;
     static const uint64_t cand_pass[] = {0, 1ull << 20, 1ull << 22, 1ull << 24, 1ull << 26};
-    static const uint64_t cand_wide[] = {0, 512, 2048, 8192};
+    static const uint64_t cand_wide[] = {0, 128, 256, 512, 1024, 2048, 8192};
     static const uint64_t cand_expand[] = {4, 8, 16, 32};
     static const uint64_t cand_resolve[] = {8, 16, 32};
     static const uint64_t cand_store[] = {4, 8, 16};
@@ -157,7 +157,7 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
         const Field* f;
         const uint64_t* v;
         size_t nv;
-    } axes[] = {{&kFields[0], cand_pass, 5}, {&kFields[1], cand_wide, 4}, {&kFields[2], cand_expand, 4},
+    } axes[] = {{&kFields[0], cand_pass, 5}, {&kFields[1], cand_wide, 7}, {&kFields[2], cand_expand, 4},
                 {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}, {&kFields[5], cand_park, 4}};
     for (const Axis& ax : axes) {
         for (size_t i = 0; i < ax.nv && err == VFB_OK; ++i) {
