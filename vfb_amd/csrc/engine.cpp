@@ -971,11 +971,15 @@ void Engine::store(std::vector<Inst*>& v) {
     const size_t n = v.size();
     std::vector<Seg> segs;
     uint64_t tot = 0;
+    // levels of few chunks split each chunk's survivors into more subsets
+    bool small = true;
+    for (size_t k = 0; k < n; ++k) small &= !v[k]->h.out_n || v[k]->h.nblk <= kStoreSmallBlk;
+    const uint64_t parts = small ? kStorePartsSmall : kStoreParts;
     for (size_t k = 0; k < n; ++k) {
         Inst& I = *v[k];
         if (!I.h.out_n || !I.h.nblk) continue;
         segs.push_back(Seg{(uint32_t)k, 0, tot});
-        tot += I.h.nblk * kStoreParts * (uint64_t)((I.h.T + kStoreTraces - 1) / kStoreTraces);
+        tot += I.h.nblk * parts * (uint64_t)((I.h.T + kStoreTraces - 1) / kStoreTraces);
     }
     if (!tot) return;
     hipEvent_t ev = nullptr;
@@ -985,7 +989,7 @@ void Engine::store(std::vector<Inst*>& v) {
         const Inst& I = *v[0];
         const StorePatch p{I.h.out_rows, I.h.out_par, I.h.out_n};
         time_begin(&ev, 2);
-        CK(launch_store((const InstDesc*)d_step_, nullptr, 0, tot, p, grid, st_));
+        CK(launch_store((const InstDesc*)d_step_, nullptr, 0, tot, p, small ? 1 : 0, grid, st_));
         time_end(ev, 2, v);
         return;
     }
@@ -998,7 +1002,8 @@ void Engine::store(std::vector<Inst*>& v) {
     h2d(d, host.data(), bytes);
     epoch_single_ = nullptr;
     time_begin(&ev, 2);
-    CK(launch_store((const InstDesc*)d, (const Seg*)(d + o_seg), (int)segs.size(), tot, StorePatch{}, grid, st_));
+    CK(launch_store((const InstDesc*)d, (const Seg*)(d + o_seg), (int)segs.size(), tot, StorePatch{}, small ? 1 : 0, grid,
+                    st_));
     time_end(ev, 2, v);
 }
 

@@ -38,7 +38,8 @@ hipError_t launch_insert_keys(Slot* tab, uint64_t mask, const unsigned long long
                               hipStream_t st);
 hipError_t launch_rehash_multi(const RehashSeg* segs, int n, uint64_t total, unsigned int* fail, int grid,
                                hipStream_t st);
-hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch,
+// small: every level of the launch has at most kStoreSmallBlk chunks (kStorePartsSmall subsets)
+hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch, int small,
                         int grid, hipStream_t st);
 
 struct DeviceError : std::runtime_error {

@@ -33,6 +33,8 @@ constexpr int kChunkWords = 64;        // bitmap words per scan chunk (4096 ordi
 constexpr int kProbeLimit = 4096;
 constexpr int kStoreTraces = 8;       // traces per k_store work item
 constexpr int kStoreParts = 4;        // survivor subsets per chunk in k_store
+constexpr int kStorePartsSmall = 16;  // the same when every level of the launch has at most
+constexpr uint64_t kStoreSmallBlk = 16;  // this many chunks (more, shorter wavefronts)
 constexpr int kWideLanes = 8;         // lanes per candidate in k_expand's small-level mode
 constexpr uint64_t kWideMaxWaves = 512;  // levels below this many wavefronts use it (default)
 constexpr int kTraceGroup = 8;        // traces whose operand loads k_expand issues together

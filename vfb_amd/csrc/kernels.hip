@@ -1158,11 +1158,12 @@ __device__ __forceinline__ void store_trace_mw(const InstDesc& d, const Cand& cd
 }
 
 // stable compaction of the survivors.  Work item = (chunk of 4096 ordinals,
-// one of kStoreParts interleaved survivor subsets, slice of kStoreTraces
+// one of SP interleaved survivor subsets, slice of kStoreTraces
 // traces), one wavefront each: the chunk's set bits
 // become an LDS list of 12-bit offsets, then the 64 lanes write consecutive
 // rows of the slice's columns (coalesced stores).  Slice 0 also writes the
 // parent ordinals.
+template <int SP>
 __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
                                                int nseg, uint64_t total_items, StorePatch patch) {
     __shared__ uint16_t lists[4][64 * kChunkWords];
@@ -1180,8 +1181,8 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         unsigned long long* const out_par = nseg ? d.out_par : patch.out_par;
         const uint64_t out_n = nseg ? d.out_n : patch.out_n;
         const uint32_t nsl = (uint32_t)(d.T + kStoreTraces - 1) / kStoreTraces;
-        const uint64_t c = item / (nsl * kStoreParts);
-        const uint32_t rem = (uint32_t)(item - c * nsl * kStoreParts);
+        const uint64_t c = item / (nsl * SP);
+        const uint32_t rem = (uint32_t)(item - c * nsl * SP);
         const uint32_t part = rem / nsl;
         const int t0 = (int)(rem - part * nsl) * kStoreTraces;
         const int t1 = t0 + kStoreTraces < d.T ? t0 + kStoreTraces : d.T;
@@ -1203,7 +1204,7 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         const uint32_t lim = rank0 + n <= out_n ? n : (uint32_t)(out_n - rank0);
         // the chunk's 4096 ordinals span few units: find the first one once
         const int u0 = unit_of(d, obase);
-        for (uint32_t i = part * 64 + lane; i < lim; i += 64 * kStoreParts) {
+        for (uint32_t i = part * 64 + lane; i < lim; i += 64 * SP) {
             const uint64_t o = obase + list[i];
             const Cand cd = decode_from(d, o, u0);
             const uint64_t row = rank0 + i;
@@ -1310,8 +1311,11 @@ hipError_t launch_scan(const InstDesc* D, const uint32_t* insts, int n, hipStrea
 }
 
 hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch,
-                        int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_store, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_items, patch);
+                        int small, int grid, hipStream_t st) {
+    if (small)
+        hipLaunchKernelGGL(k_store<kStorePartsSmall>, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_items, patch);
+    else
+        hipLaunchKernelGGL(k_store<kStoreParts>, dim3(grid), dim3(256), 0, st, D, segs, nseg, total_items, patch);
     return hipGetLastError();
 }
 
