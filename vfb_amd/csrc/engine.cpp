@@ -658,7 +658,9 @@ void Engine::prepare_scratch(Inst& I) {
 // (re)builds the hash set for want_keys keys at load <= 0.7; keeps only keys
 // of completed epochs (ord < epoch_base)
 void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
-    uint64_t need = next_pow2(std::max<uint64_t>(4096, (uint64_t)(want_keys / 0.7) + 1));
+    // at least kMinSlots: the small levels of a solve then share one set, with
+    // no fill and rehash as it grows
+    uint64_t need = next_pow2(std::max<uint64_t>(kMinSlots, (uint64_t)(want_keys / 0.7) + 1));
     // no headroom: plain, batched and served solves size their sets alike, so
     // their HBM-budget decisions (and hence their results) agree
     if (need > (1ull << 32)) need = 1ull << 32;

@@ -41,6 +41,7 @@ constexpr int kTraceGroup = 8;        // traces whose operand loads k_expand iss
 constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are counted in k_scan
 constexpr uint64_t kSmallFresh = 16384; // fresh lists up to this long are resolved in the scan's workgroup
 constexpr uint64_t kMinWinWords = 1024; // winner maps are allocated for at least 64K candidates
+constexpr uint64_t kMinSlots = 1ull << 17; // hash sets start at 4 MB
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 
