@@ -39,7 +39,7 @@ constexpr int kWideLanes = 8;         // lanes per candidate in k_expand's small
 constexpr uint64_t kWideMaxWaves = 512;  // levels below this many wavefronts use it (default)
 constexpr int kTraceGroup = 8;        // traces whose operand loads k_expand issues together
 constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are counted in k_scan
-constexpr uint64_t kSmallFresh = 16384; // fresh lists up to this long are resolved in the scan's workgroup
+constexpr uint64_t kSmallFresh = 4096;  // fresh lists up to this long are resolved in the scan's workgroup
 constexpr uint64_t kMinWinWords = 1024; // winner maps are allocated for at least 64K candidates
 constexpr uint64_t kMinSlots = 1ull << 17; // hash sets start at 4 MB
 
