@@ -32,6 +32,10 @@
  *   - The row cap truncates the overflowing level in ordinal order after the
  *     whole level was deduplicated and checked (the reference stops deduping
  *     mid-level at a device-dependent point); the next level is check-only.
+ *
+ * Parity unpinned: the folding of a type-3 half-key equal to -1 onto -2
+ * (at the end of ora_key, hash_set.cuh:198-204).  It happens with probability
+ * 2^-64 per key, so no test input reaches it; it is a restatement only.
  */
 #define _POSIX_C_SOURCE 200809L
 #include "oracle.h"
