@@ -96,6 +96,15 @@ void vfb_instance_copy(const vfb_instance* inst, int* lens, uint64_t* atoms);
 const char* vfb_instance_atom_name(const vfb_instance* inst, int a);
 const char* vfb_instance_planted(const vfb_instance* inst); /* "" if none */
 int vfb_instance_max_size(const vfb_instance* inst);        /* config default, 0 if none */
+
+/* Copies the instance's atom rows (its level-1 CS rows, A*W words) to device
+   memory once.  Later vfb_solve calls on this handle read them there instead
+   of uploading them per call (the reference uploads them in every LTLI call,
+   LTL_BS.cu:1216).  Batches of several instances and the server upload
+   their own copies.
+   The device copy lives until vfb_instance_free and never travels with a copy
+   of the instance.  VFB_DEVICE_ERROR without a usable GPU. */
+vfb_status vfb_instance_upload(vfb_instance* inst);
 void vfb_instance_free(vfb_instance* inst);
 
 /* ---- solving ----------------------------------------------------------- */

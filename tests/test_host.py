@@ -124,6 +124,8 @@ def _has_gpu():
 def test_no_gpu_fails_loudly():
     r = V.solve(V.Instance.generate(1, 0), max_size=5)
     assert r["status"] == V.DEVICE_ERROR
+    with pytest.raises(V.VfbError):
+        V.Instance.generate(1, 0).upload()
 
 
 def test_cli_usage_and_missing_file():

@@ -67,7 +67,7 @@ EXPORTS = [
     "vfb_instance_load_json", "vfb_instance_load_text", "vfb_instance_from_bits", "vfb_instance_from_words",
     "vfb_instance_generate", "vfb_instance_write_json", "vfb_instance_shape", "vfb_instance_words", "vfb_instance_copy",
     "vfb_instance_atom_name", "vfb_instance_planted", "vfb_instance_max_size",
-    "vfb_instance_free", "vfb_options_default", "vfb_solve", "vfb_solve_batch",
+    "vfb_instance_upload", "vfb_instance_free", "vfb_options_default", "vfb_solve", "vfb_solve_batch",
     "vfb_result_free", "vfb_status_string", "vfb_device_info",
     "vfb_server_create", "vfb_server_submit", "vfb_server_poll", "vfb_server_wait",
     "vfb_server_stats", "vfb_server_destroy",
@@ -105,6 +105,7 @@ def lib():
     L.vfb_instance_planted.argtypes = [vp]
     L.vfb_instance_planted.restype = C.c_char_p
     L.vfb_instance_max_size.argtypes = [vp]
+    L.vfb_instance_upload.argtypes = [vp]
     L.vfb_instance_free.argtypes = [vp]
     L.vfb_instance_free.restype = None
     L.vfb_options_default.argtypes = [C.POINTER(Options)]
@@ -163,6 +164,14 @@ class Instance:
     @property
     def handle(self):
         return self._h
+
+    def upload(self):
+        """Keeps the atom rows resident in device memory: later solve() calls
+        on this instance read them there instead of uploading them."""
+        st = lib().vfb_instance_upload(self._h)
+        if st != OK:
+            raise VfbError(f"upload: {status_string(st)}")
+        return self
 
     @classmethod
     def from_json(cls, path):

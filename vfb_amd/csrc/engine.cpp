@@ -75,7 +75,7 @@ void Engine::sync() {
 // a buffer still read by enqueued kernels goes back to the arena at the next
 // sync; its bytes stop counting against the instance now
 void Engine::release_after_sync(Inst& I, Buf& b) {
-    if (b.p) {
+    if (b.p && !b.borrowed) {
         pending_free_.push_back(b.p);
         I.used -= b.bytes;
     }
@@ -191,7 +191,7 @@ void Engine::dalloc(Inst& I, Buf& b, size_t bytes) {
 }
 
 void Engine::dfree(Inst& I, Buf& b) {
-    if (b.p) {
+    if (b.p && !b.borrowed) {
         arena_->release(b.p);
         I.used -= b.bytes;
     }
@@ -387,6 +387,20 @@ static void col_params(const vfb_instance& in, int relaxed_type, const InstDesc&
     }
 }
 
+// Level-1 rows of an instance, column-major: word c of atom a at [c * A + a];
+// steps past a trace are 0
+static void atom_rows(const vfb_instance& in, std::vector<uint64_t>* rows) {
+    const int A = in.A(), T = in.T(), C = in.C();
+    const std::vector<int> col = in.cols();
+    rows->assign((size_t)A * C, 0);
+    for (int a = 0; a < A; ++a)
+        for (int t = 0; t < T; ++t)
+            for (int c = col[t]; c < col[t + 1]; ++c) {
+                const uint64_t m = c + 1 == col[t + 1] ? len_mask(in.lens[t] - 64 * (c - col[t])) : ~0ull;
+                (*rows)[(size_t)c * A + a] = in.atoms[(size_t)a * C + c] & m;
+            }
+}
+
 void Engine::setup(Inst& I) {
     const vfb_instance& in = *I.in;
     const int A = in.A(), T = in.T();
@@ -441,16 +455,15 @@ void Engine::setup(Inst& I) {
         I.h.cinfo = (const ColInfo*)I.cols.p;
         I.h.tcol = (const int32_t*)((const uint64_t*)I.cols.p + n_ci);
     }
-    // column-major: word c of atom a at [c * A + a]; steps past a trace are 0
-    std::vector<uint64_t> rows((size_t)A * C);
-    for (int a = 0; a < A; ++a)
-        for (int t = 0; t < T; ++t)
-            for (int c = col[t]; c < col[t + 1]; ++c) {
-                const uint64_t m = c + 1 == col[t + 1] ? len_mask(in.lens[t] - 64 * (c - col[t])) : ~0ull;
-                rows[(size_t)c * A + a] = in.atoms[(size_t)a * C + c] & m;
-            }
-    dalloc(I, I.rows[c1], rows.size() * 8);
-    upload(I, I.rows[c1].p, rows.data(), rows.size());
+    if (!batch_ && in.dev.p) {
+        // resident rows (vfb_instance_upload): read in place, never written
+        I.rows[c1] = Buf{in.dev.p, 0, true};
+    } else {
+        std::vector<uint64_t> rows;
+        atom_rows(in, &rows);
+        dalloc(I, I.rows[c1], rows.size() * 8);
+        upload(I, I.rows[c1].p, rows.data(), rows.size());
+    }
     I.nrows[c1] = (uint32_t)A;
     I.stored = (uint64_t)A;
     I.level = c1;
@@ -1372,6 +1385,8 @@ Engine& engine_acquire(size_t want) {
 
 void engine_drop() { g_engine.reset(); }
 
+void dev_rows_free(void* p) { (void)hipFree(p); }
+
 static void reset_result(vfb_result* r) {
     std::free(r->formula);
     std::memset(r, 0, sizeof(vfb_result));
@@ -1510,6 +1525,29 @@ extern "C" vfb_status vfb_solve_batch(const vfb_instance* const* insts, size_t n
     if (!insts || !out) return VFB_INVALID_INPUT;
     if (n == 0) return VFB_OK;
     return solve_many(insts, n, opt, out);
+}
+
+extern "C" vfb_status vfb_instance_upload(vfb_instance* inst) {
+    if (!inst) return VFB_INVALID_INPUT;
+    std::string why;
+    if (validate(*inst, &why) != VFB_OK) return VFB_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (inst->dev.p) return VFB_OK;
+    std::vector<uint64_t> rows;
+    atom_rows(*inst, &rows);
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(rows.size(), 1) * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return VFB_DEVICE_ERROR;
+    }
+    if (hipMemcpy(p, rows.data(), rows.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(p);
+        return VFB_DEVICE_ERROR;
+    }
+    inst->dev.p = p;
+    inst->dev.words = rows.size();
+    return VFB_OK;
 }
 
 extern "C" void vfb_tuning_default(vfb_tuning* t) {

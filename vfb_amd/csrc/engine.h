@@ -116,6 +116,7 @@ class Arena {
 struct Buf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;   // not from the arena (an instance's resident atom rows): never released
 };
 
 // budget (the solve's own limit) or arena (memory shared with a batch) exhausted

@@ -10,6 +10,30 @@
 // 64-bit words of a trace of len steps (word w holds steps 64w..64w+63)
 inline int trace_words(int len) { return (len + 63) / 64; }
 
+namespace vfb {
+void dev_rows_free(void* p);   // hipFree of a resident atom-row buffer (engine.cpp)
+}
+
+// Device copy of an instance's atom rows (vfb_instance_upload).  A copy or an
+// assignment of the instance never carries it: the rows it holds belong to
+// the handle they were uploaded for.
+struct DevRows {
+    void* p = nullptr;
+    size_t words = 0;
+    DevRows() = default;
+    DevRows(const DevRows&) {}
+    DevRows& operator=(const DevRows&) {
+        release();
+        return *this;
+    }
+    ~DevRows() { release(); }
+    void release() {
+        if (p) vfb::dev_rows_free(p);
+        p = nullptr;
+        words = 0;
+    }
+};
+
 struct vfb_instance {
     std::vector<std::string> names;   // AP names, in input order
     std::vector<int> lens;            // per trace, positives first
@@ -19,6 +43,7 @@ struct vfb_instance {
     std::string planted;              // generating formula (generated instances)
     uint64_t seed = 0;                // generating seed
     int default_max_size = 0;         // config default (generated instances)
+    DevRows dev;                      // resident atom rows, if uploaded
     int A() const { return (int)names.size(); }
     int T() const { return (int)lens.size(); }
     // words per row: the traces' words in trace order
