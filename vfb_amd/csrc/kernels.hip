@@ -776,7 +776,17 @@ __device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned i
     if (d.mw) {
         for (int c = 0; c < d.C; ++c) key_acc_col<KEY>(k, L[(uint64_t)c * n + x], d.cinfo[c], d);
     } else {
-        for (int t = 0; t < d.T; ++t) key_acc<KEY>(k, L[(uint64_t)t * n + x], d, t);
+        // 16 independent row loads in flight before their keys fold in
+        // (key_acc folds by XOR/OR, so the grouping leaves the key unchanged)
+        constexpr int B = 16;
+        for (int t0 = 0; t0 < d.T; t0 += B) {
+            uint64_t w[B];
+#pragma unroll
+            for (int j = 0; j < B; ++j) w[j] = t0 + j < d.T ? L[(uint64_t)(t0 + j) * n + x] : 0ull;
+#pragma unroll
+            for (int j = 0; j < B; ++j)
+                if (t0 + j < d.T) key_acc<KEY>(k, w[j], d, t0 + j);
+        }
     }
     key_fin(k);
     uint32_t slot;
