@@ -69,3 +69,18 @@ def test_brute_search_formula_line(tmp_path):
     assert want["status"] == V.OK
     out = _cli([path, "6"], VFB_SEARCH="brute")
     assert out.stdout == f"{path}, {want['formula']}\n", out.stdout
+
+
+@pytest.mark.parametrize("rt", [1, 2, 3])
+def test_relaxed_type_env(tmp_path, rt):
+    """VFB_RELAXED_TYPE picks the relaxed key (total length 965 > 126 bits).
+    On this instance types 1 and 2 return G(F(...)) and type 3 F(... G(...)),
+    so the line shows which key the GPU used."""
+    path = str(tmp_path / "cfg2_1.json")
+    V.Instance.generate(2, 1).write_json(path)
+    names, lens, atoms, P = V.Instance.from_json(path).arrays()
+    assert sum(lens) > 126
+    want = O.enumerate_cs(O.Instance(names, lens, atoms, P), 14, rt)
+    assert want["status"] == V.OK
+    out = _cli([path, "14"], VFB_RELAXED_TYPE=str(rt))
+    assert out.stdout == f"{path}, {want['formula']}\n", out.stdout
