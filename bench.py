@@ -251,7 +251,7 @@ def main():
     ap.add_argument("--limits", default="4,10", help="config-3 instances to time (empty: skip)")
     ap.add_argument("--multiword", default="8,8,12",
                     help="instances,stretch,max_size of the long-trace workload (empty: skip)")
-    ap.add_argument("--only", choices=["main", "limits", "mix", "multiword"], default=None,
+    ap.add_argument("--only", choices=["main", "limits", "mix", "multiword", "serve"], default=None,
                     help="run one section alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -270,6 +270,9 @@ def main():
         return
     if args.only == "mix":
         print(json.dumps({"batched_mix": batched_mix(V, args.mix, 16)}))
+        return
+    if args.only == "serve":
+        print(json.dumps({"served": served(V, args.serve_rate, args.serve_seconds, 16, args.serve_active, 256)}))
         return
     if args.only == "multiword":
         n, k, m = (int(x) for x in args.multiword.split(","))
