@@ -218,10 +218,15 @@ vfb_status vfb_server_submit(vfb_server* srv, const vfb_instance* inst, uint64_t
 
 /* Non-blocking: when the request has finished, *done = 1, *out receives its
    result (ms_total = completion minus submission) and the ticket is released;
-   otherwise *done = 0.  VFB_INVALID_INPUT for an unknown ticket. */
+   otherwise *done = 0.  The return value is then the result's status.
+   A ticket is consumed once: VFB_INVALID_INPUT (and *done = 0, *out
+   untouched) for an unknown or already consumed ticket. */
 vfb_status vfb_server_poll(vfb_server* srv, uint64_t ticket, vfb_result* out, int* done);
 
-/* Blocks until the request has finished, then as vfb_server_poll. */
+/* Blocks until the request has finished, then as vfb_server_poll.  A caller
+   still blocked when another caller consumes the ticket, or when
+   vfb_server_destroy stops the server, returns VFB_INVALID_INPUT with *out
+   untouched; vfb_server_destroy returns only after such callers have left. */
 vfb_status vfb_server_wait(vfb_server* srv, uint64_t ticket, vfb_result* out);
 
 /* Requests waiting, running, and device bytes held by the running ones;
@@ -255,6 +260,9 @@ typedef struct {
                                       others are done, then goes on alone (oldest first), so
                                       such members neither share the device nor evict one
                                       another mid-solve; 0 = never park */
+    uint64_t table_min_slots;      /* hash sets start at least this large (rounded up to a power
+                                      of two, at most 2^32; default 2^17 = 4 MB); a set is
+                                      otherwise grown by rehash as its keys need */
 } vfb_tuning;
 
 void vfb_tuning_default(vfb_tuning* t);
@@ -277,6 +285,16 @@ vfb_status vfb_tuning_load(const char* path, vfb_tuning* t);
 vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, const vfb_options* opt,
                         int batched, int reps, const char* cache_path, vfb_tuning* best,
                         double* ms_default, double* ms_best);
+
+/* Size of the device arena that every solve of the process allocates its
+   device memory from (levels, hash sets, scratch): 0 = 85% of the HBM free
+   at first use (the default), else at least 64 MiB.  The engine is made
+   again with it at the next solve, so no call may be running; batch members
+   and served requests that do not fit together evict one another and are
+   restarted, with unchanged results.  VFB_INVALID_INPUT while a server runs
+   or below 64 MiB.  Replaces the reference's sizing of its cache from the
+   device's memory in every LTLI call (LTL_BS.cu:1179-1183). */
+vfb_status vfb_set_arena_bytes(uint64_t bytes);
 
 /* Device name and compute-unit count, for reports.  Returns VFB_DEVICE_ERROR
    when no HIP device is usable. */

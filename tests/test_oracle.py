@@ -463,3 +463,58 @@ def test_unit_costs_equal_plain():
         a = O.enumerate_cs(inst, 7, 3)
         b = O.enumerate_costs(inst, 7, [1] * 8, 1)
         assert (a["formula"], a["distinct"], a["candidates"]) == (b["formula"], b["distinct"], b["candidates"])
+
+
+# ---- the -1 / -2 half-key fold (hash_set.cuh:198-204, LTL_BS.cu:262-265) ----
+
+from fold_instance import fold_instance, mix64 as py_mix64, unmix64, type3_key  # noqa: E402
+
+
+def test_unmix64_inverts_the_published_finaliser():
+    rng = random.Random(9)
+    g = 0x9E3779B97F4A7C15
+    assert py_mix64(g) == 0xE220A8397B1DCDAF       # published SplitMix64 output
+    for _ in range(200):
+        x = rng.getrandbits(64)
+        assert unmix64(py_mix64(x)) == x
+        assert py_mix64(unmix64(x)) == x
+        assert O.mix64(x) == py_mix64(x)
+
+
+@pytest.mark.parametrize("half", ["hi", "lo"])
+def test_fold_instance_reaches_minus_one_and_minus_two(half):
+    names, lens, atoms, P, (na, nb) = fold_instance(half)
+    ha, hb = type3_key(na), type3_key(nb)
+    k = 0 if half == "hi" else 1
+    assert ha[k] == M64 and hb[k] == M64 - 1          # raw halves -1 and -2
+    assert ha[1 - k] == hb[1 - k]                      # the other halves agree
+    assert na != nb                                     # different rows
+    # the oracle's key (restated from the reference) merges them: -1 -> -2
+    ka, kb = O.key(na, lens, 3), O.key(nb, lens, 3)
+    assert ka == kb and ka[k] == M64 - 1
+    # ~ of the atoms is exactly these rows
+    assert [O.apply(O.NOT, w, 0, 64) for w in atoms[0]] == na
+    assert [O.apply(O.NOT, w, 0, 64) for w in atoms[1]] == nb
+
+
+@pytest.mark.parametrize("half", ["hi", "lo"])
+def test_fold_drops_one_distinct_cs_at_level_two(half):
+    names, lens, atoms, P, _ = fold_instance(half)
+    inst = O.Instance(names, lens, atoms, P)
+    hashed = O.enumerate_cs(inst, 2, 3)
+    full = O.enumerate_cs(inst, 2, O.KEY_FULLROW)
+    assert hashed["status"] == full["status"] == 1     # level 2 complete, nothing found
+    assert hashed["distinct"][1] == full["distinct"][1] == 2
+    # full-row dedup of the 8 unary candidates against each other and the
+    # atoms, counted here on Python tuples; the fold keeps one fewer
+    seen = {tuple(x) for x in atoms}
+    new = 0
+    for op in (O.NOT, O.NEXT, O.FINALLY, O.GLOBALLY):
+        for x in atoms:
+            row = tuple(O.apply(op, w, 0, 64) for w in x)
+            if row not in seen:
+                seen.add(row)
+                new += 1
+    assert full["distinct"][2] == new
+    assert hashed["distinct"][2] == full["distinct"][2] - 1
+    assert hashed["candidates"] == full["candidates"]

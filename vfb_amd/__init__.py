@@ -56,7 +56,8 @@ class Result(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("pass_min", C.c_uint64), ("wide_max_waves", C.c_uint32),
                 ("expand_blocks_per_cu", C.c_uint32), ("resolve_blocks_per_cu", C.c_uint32),
-                ("store_blocks_per_cu", C.c_uint32), ("park_min", C.c_uint64)]
+                ("store_blocks_per_cu", C.c_uint32), ("park_min", C.c_uint64),
+                ("table_min_slots", C.c_uint64)]
 
 
 TUNING_FIELDS = [f for f, _ in Tuning._fields_]
@@ -72,7 +73,7 @@ EXPORTS = [
     "vfb_server_create", "vfb_server_submit", "vfb_server_poll", "vfb_server_wait",
     "vfb_server_stats", "vfb_server_destroy",
     "vfb_tuning_default", "vfb_get_tuning", "vfb_set_tuning", "vfb_tuning_save",
-    "vfb_tuning_load", "vfb_autotune",
+    "vfb_tuning_load", "vfb_autotune", "vfb_set_arena_bytes",
 ]
 
 _lib = None
@@ -133,6 +134,7 @@ def lib():
     L.vfb_set_tuning.argtypes = [TP]
     L.vfb_tuning_save.argtypes = [TP, C.c_char_p]
     L.vfb_tuning_load.argtypes = [C.c_char_p, TP]
+    L.vfb_set_arena_bytes.argtypes = [C.c_uint64]
     L.vfb_autotune.argtypes = [C.POINTER(vp), C.c_size_t, C.POINTER(Options), C.c_int, C.c_int,
                                C.c_char_p, TP, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
@@ -397,6 +399,13 @@ def autotune(insts, max_size=MAX_SIZE, batched=True, reps=2, cache=None, relaxed
     return {"tuning": _tdict(best), "ms_default": t0.value, "ms_best": t1.value}
 
 
+def set_arena_bytes(nbytes):
+    """Device arena of later solves (0 = 85% of the free HBM, the default)."""
+    st = lib().vfb_set_arena_bytes(int(nbytes))
+    if st != OK:
+        raise VfbError(f"set_arena_bytes: {status_string(st)}")
+
+
 def device_info():
     buf = C.create_string_buffer(256)
     cus = C.c_int()
@@ -428,10 +437,14 @@ class Server:
         return t.value
 
     def poll(self, ticket):
+        """The result if the request has finished (the ticket is then
+        consumed), else None; VfbError for an unknown or consumed ticket."""
         r = Result()
         done = C.c_int()
-        lib().vfb_server_poll(self._h, ticket, C.byref(r), C.byref(done))
+        st = lib().vfb_server_poll(self._h, ticket, C.byref(r), C.byref(done))
         if not done.value:
+            if st == INVALID_INPUT:
+                raise VfbError(f"server_poll: unknown or consumed ticket {ticket}")
             return None
         try:
             return _todict(r)
@@ -439,8 +452,12 @@ class Server:
             lib().vfb_result_free(C.byref(r))
 
     def wait(self, ticket):
+        """Blocks until the request has finished; VfbError for an unknown or
+        consumed ticket, or when the server stops first."""
         r = Result()
-        lib().vfb_server_wait(self._h, ticket, C.byref(r))
+        st = lib().vfb_server_wait(self._h, ticket, C.byref(r))
+        if st == INVALID_INPUT and not r.formula:
+            raise VfbError(f"server_wait: unknown or consumed ticket {ticket}, or the server stopped")
         try:
             return _todict(r)
         finally:

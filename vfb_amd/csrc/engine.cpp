@@ -243,6 +243,11 @@ void Engine::purge_ops(const Inst& I) {
                  fills_.end());
     rehashes_.erase(std::remove_if(rehashes_.begin(), rehashes_.end(), [&](const RehashOp& r) { return r.I == &I; }),
                     rehashes_.end());
+    // sources of dropped rehashes: only kernels already enqueued may read them
+    for (auto& r : rehash_src_)
+        if (r.I == &I) pending_free_.push_back(r.p);
+    rehash_src_.erase(std::remove_if(rehash_src_.begin(), rehash_src_.end(), [&](const RehashSrc& r) { return r.I == &I; }),
+                      rehash_src_.end());
 }
 
 // one upload of the segment tables (and staged data), one fill / copy launch,
@@ -287,6 +292,9 @@ void Engine::flush_ops() {
         CK(launch_rehash_multi((const RehashSeg*)(d_ops_ + o_rs), (int)nr, rtot, d_flag_, grid, st_));
         flag_dirty_ = true;
     }
+    // the rehash sources go back to the arena once the launch above has run
+    for (auto& r : rehash_src_) pending_free_.push_back(r.p);
+    rehash_src_.clear();
     fills_.clear();
     rehashes_.clear();
     staged_.clear();
@@ -671,9 +679,10 @@ void Engine::prepare_scratch(Inst& I) {
 // (re)builds the hash set for want_keys keys at load <= 0.7; keeps only keys
 // of completed epochs (ord < epoch_base)
 void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
-    // at least kMinSlots: the small levels of a solve then share one set, with
-    // no fill and rehash as it grows
-    uint64_t need = next_pow2(std::max<uint64_t>(kMinSlots, (uint64_t)(want_keys / 0.7) + 1));
+    // at least kMinSlots (or the tuned start size): the small levels of a
+    // solve then share one set, with no fill and rehash as it grows
+    const uint64_t start = std::max<uint64_t>(kMinSlots, tun_.table_min_slots);
+    uint64_t need = next_pow2(std::max<uint64_t>(start, (uint64_t)(want_keys / 0.7) + 1));
     // no headroom: plain, batched and served solves size their sets alike, so
     // their HBM-budget decisions (and hence their results) agree
     if (need > (1ull << 32)) need = 1ull << 32;
@@ -688,13 +697,18 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     if (I.table.p) {
         // the probe-limit flag is read at the next stream sync (check_flag)
         if (batch_) {
+            // the rehash is only queued: the old table must outlive the flush
+            // that launches it (a sync before then must not hand it back)
             rehashes_.push_back(RehashOp{&I, (const Slot*)I.table.p, I.cap, (Slot*)nt.p, need - 1, I.gbase});
+            rehash_src_.push_back(RehashSrc{&I, I.table.p});
+            I.used -= I.table.bytes;
+            I.table = Buf{};
         } else {
             int grid = (int)std::min<uint64_t>((I.cap + 255) / 256, (uint64_t)cus_ * 8);
             CK(launch_rehash((const Slot*)I.table.p, I.cap, (Slot*)nt.p, need - 1, I.gbase, d_flag_, grid, st_));
             flag_dirty_ = true;
+            release_after_sync(I, I.table);
         }
-        release_after_sync(I, I.table);
     }
     I.table = nt;
     I.cap = need;
@@ -1063,6 +1077,35 @@ void Engine::abandon(Inst& I) {
     I.done = true;
 }
 
+// After an error inside a call: waits for the stream, clears the failure
+// flag, and removes every trace of these instances (queued operations, timing
+// records, device memory), leaving the engine -- and the other instances it
+// holds, such as a server's requests -- usable.
+void Engine::forget(std::vector<Inst*>& v) {
+    try {
+        sync();
+    } catch (std::exception&) {
+    }
+    *h_flag_ = 0;
+    flag_dirty_ = false;
+    (void)hipMemsetAsync(d_flag_, 0, sizeof(unsigned int), st_);
+    (void)hipStreamSynchronize(st_);
+    std::vector<Inst*> all;
+    for (Inst* I : v) {
+        all.push_back(I);
+        if (I->bs) all.push_back(I->bs.get());
+    }
+    for (auto& t : timed_)
+        for (Inst* I : all) t.who.erase(std::remove(t.who.begin(), t.who.end(), I), t.who.end());
+    for (Inst* I : all) {
+        drop_live(live_, I);
+        parked_.erase(std::remove(parked_.begin(), parked_.end(), I), parked_.end());
+        if (epoch_single_ == I) epoch_single_ = nullptr;
+        release_all(*I);
+        I->done = true;
+    }
+}
+
 void Engine::start(std::vector<Inst*>& all, std::vector<Inst*>& out) {
     std::vector<Inst*> act;
     for (auto* I : all) {
@@ -1372,18 +1415,19 @@ vfb_tuning current_tuning() { return g_tuning; }
 
 std::mutex& engine_mutex() { return g_mu; }
 
+static uint64_t g_arena_bytes = 0;   // vfb_set_arena_bytes; 0 = 85% of the free HBM
+
 Engine& engine_acquire(size_t want) {
     if (!g_engine || (want && want > g_engine->arena_size() && !engine_pinned())) {
         g_engine.reset();
         size_t fr = 0, tot = 0;
         CK(hipMemGetInfo(&fr, &tot));
-        size_t bytes = want ? std::min(want, (size_t)(fr * 0.95)) : (size_t)(fr * 0.85);
+        size_t bytes = g_arena_bytes ? std::min<size_t>(g_arena_bytes, (size_t)(fr * 0.95)) : (size_t)(fr * 0.85);
+        if (want > bytes) bytes = std::min(want, (size_t)(fr * 0.95));
         g_engine = std::make_unique<Engine>(bytes);
     }
     return *g_engine;
 }
-
-void engine_drop() { g_engine.reset(); }
 
 void dev_rows_free(void* p) { (void)hipFree(p); }
 
@@ -1487,7 +1531,9 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
             const uint32_t restarts = fresh.res->restarts + 1;
             reset_result(fresh.res);
             fresh.res->restarts = restarts;
-            std::vector<Inst*> one{&fresh};
+            fresh.prio = I->prio;
+            *I = std::move(fresh);   // in place: an error below is cleaned up with the rest
+            std::vector<Inst*> one{I};
             eng.run(one, opt);
         }
     } catch (std::exception& e) {
@@ -1499,7 +1545,10 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
                 I->res->formula = strdup("see_the_error");
             }
         }
-        g_engine.reset();
+        // a server's requests live in the engine: release only this call's
+        // instances; otherwise start over with a new engine
+        if (engine_pinned() && g_engine) g_engine->forget(run);
+        else g_engine.reset();
         return VFB_DEVICE_ERROR;
     }
     double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1550,6 +1599,15 @@ extern "C" vfb_status vfb_instance_upload(vfb_instance* inst) {
     return VFB_OK;
 }
 
+extern "C" vfb_status vfb_set_arena_bytes(uint64_t bytes) {
+    if (bytes != 0 && bytes < (64ull << 20)) return VFB_INVALID_INPUT;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (engine_pinned()) return VFB_INVALID_INPUT;
+    g_arena_bytes = bytes;
+    g_engine.reset();   // re-created with this arena at the next solve
+    return VFB_OK;
+}
+
 extern "C" void vfb_tuning_default(vfb_tuning* t) {
     if (!t) return;
     t->pass_min = 1ull << 22;
@@ -1558,6 +1616,7 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     t->resolve_blocks_per_cu = 16;
     t->store_blocks_per_cu = 8;
     t->park_min = 1ull << 24;
+    t->table_min_slots = kMinSlots;
 }
 
 extern "C" void vfb_get_tuning(vfb_tuning* t) {
@@ -1569,7 +1628,8 @@ extern "C" void vfb_get_tuning(vfb_tuning* t) {
 extern "C" vfb_status vfb_set_tuning(const vfb_tuning* t) {
     if (!t) return VFB_INVALID_INPUT;
     auto ok = [](uint32_t b) { return b >= 1 && b <= 64; };
-    if (!ok(t->expand_blocks_per_cu) || !ok(t->resolve_blocks_per_cu) || !ok(t->store_blocks_per_cu))
+    if (!ok(t->expand_blocks_per_cu) || !ok(t->resolve_blocks_per_cu) || !ok(t->store_blocks_per_cu) ||
+        t->table_min_slots > (1ull << 32))
         return VFB_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(g_mu);
     g_tuning = *t;

@@ -191,6 +191,7 @@ class Engine {
     void step(std::vector<Inst*>& act);
     void drain() { sync(); }
     void abandon(Inst& I);
+    void forget(std::vector<Inst*>& v);
     size_t arena_size() const { return arena_->size(); }
 
   private:
@@ -213,6 +214,10 @@ class Engine {
     struct RehashOp { Inst* I; const Slot* old; uint64_t old_cap; Slot* tab; uint64_t mask; uint64_t keep; };
     std::vector<FillOp> fills_;
     std::vector<RehashOp> rehashes_;
+    // old tables of queued rehashes: released to the arena after the flush
+    // that launches their rehash (then at the following sync)
+    struct RehashSrc { Inst* I; void* p; };
+    std::vector<RehashSrc> rehash_src_;
     std::vector<unsigned long long> staged_;  // upload data of copy ops (staged != SIZE_MAX)
     char* d_ops_ = nullptr;
     size_t ops_cap_ = 0;
@@ -267,7 +272,6 @@ class Engine {
 std::mutex& engine_mutex();
 vfb_tuning current_tuning();   // caller holds engine_mutex()
 Engine& engine_acquire(size_t want_bytes);
-void engine_drop();
 bool engine_pinned();   // true while a server holds requests in the engine
 
 }  // namespace vfb

@@ -63,6 +63,7 @@ struct vfb_server {
     uint64_t peak_active = 0;
     uint64_t bytes_active = 0;
     bool stop = false;
+    int waiters = 0;            // callers blocked in vfb_server_wait
     std::thread th;
     void loop();
 };
@@ -102,7 +103,14 @@ void vfb_server::loop() {
                 if (!act.empty()) eng.step(act);
             } catch (std::exception& e) {
                 std::fprintf(stderr, "vfb server: device error: %s\n", e.what());
-                engine_drop();
+                // other servers' requests may live in the engine too: release
+                // only this server's running and admitted requests
+                std::vector<Inst*> mine;
+                for (Req* r : touched) mine.push_back(&r->st);
+                try {
+                    engine_acquire(0).forget(mine);
+                } catch (std::exception&) {
+                }
                 for (Req* r : touched) {
                     r->st.done = true;
                     r->res.status = VFB_DEVICE_ERROR;
@@ -182,16 +190,28 @@ extern "C" vfb_status vfb_server_submit(vfb_server* s, const vfb_instance* inst,
     return VFB_OK;
 }
 
+// Hands a finished request's result to its caller and releases the ticket.
+// A ticket is consumed once: a second caller waiting on the same ticket, or
+// one still waiting when the server stops, gets VFB_INVALID_INPUT.  The
+// request is looked up again after every wake-up, since another caller may
+// have consumed (and freed) it meanwhile.
 static vfb_status take(vfb_server* s, uint64_t ticket, vfb_result* out, bool block, int* done) {
     std::unique_lock<std::mutex> lk(s->mu);
+    if (done) *done = 0;
+    if (s->reqs.find(ticket) == s->reqs.end()) return VFB_INVALID_INPUT;
+    if (block) {
+        ++s->waiters;
+        s->cv_done.wait(lk, [&] {
+            auto w = s->reqs.find(ticket);
+            return s->stop || w == s->reqs.end() || w->second->done;
+        });
+        --s->waiters;
+        if (s->stop) s->cv_done.notify_all();   // vfb_server_destroy waits for the waiters to leave
+    }
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return VFB_INVALID_INPUT;
     Req* r = it->second.get();
-    if (block) s->cv_done.wait(lk, [&] { return r->done; });
-    if (!r->done) {
-        if (done) *done = 0;
-        return VFB_OK;
-    }
+    if (!r->done) return block ? VFB_INVALID_INPUT : VFB_OK;   // blocked, and the server stopped
     if (done) *done = 1;
     if (out) {
         *out = r->res;
@@ -231,6 +251,13 @@ extern "C" void vfb_server_destroy(vfb_server* s) {
     }
     s->cv_work.notify_all();
     if (s->th.joinable()) s->th.join();
+    {
+        // callers blocked in vfb_server_wait return VFB_INVALID_INPUT before
+        // the server (its mutex and condition variables) is freed
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.notify_all();
+        s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+    }
     {
         std::lock_guard<std::mutex> g(engine_mutex());
         // release the device memory of requests still running

@@ -30,6 +30,7 @@ const Field kFields[] = {
     {"resolve_blocks_per_cu", false, offsetof(vfb_tuning, resolve_blocks_per_cu)},
     {"store_blocks_per_cu", false, offsetof(vfb_tuning, store_blocks_per_cu)},
     {"park_min", true, offsetof(vfb_tuning, park_min)},
+    {"table_min_slots", true, offsetof(vfb_tuning, table_min_slots)},
 };
 
 uint64_t get(const vfb_tuning& t, const Field& f) {
@@ -108,7 +109,8 @@ extern "C" vfb_status vfb_tuning_load(const char* path, vfb_tuning* t) {
     }
     std::fclose(f);
     auto ok = [](uint32_t b) { return b >= 1 && b <= 64; };
-    if (!ok(v.expand_blocks_per_cu) || !ok(v.resolve_blocks_per_cu) || !ok(v.store_blocks_per_cu))
+    if (!ok(v.expand_blocks_per_cu) || !ok(v.resolve_blocks_per_cu) || !ok(v.store_blocks_per_cu) ||
+        v.table_min_slots > (1ull << 32))
         return VFB_INVALID_INPUT;
     *t = v;
     return VFB_OK;
