@@ -174,10 +174,14 @@ void Engine::dalloc(Inst& I, Buf& b, size_t bytes) {
                 p = arena_->alloc(rb);
                 continue;
             }
-            // a batch solves this instance again alone, as the plain solve
-            // would (its arena holds none of the others' holes); a server
-            // lets the oldest request go on under the plain limit
-            if (defer_alone_) throw CapacityError("device arena exhausted", true);
+            // solved again alone, as the plain solve would be (its arena then
+            // holds none of the others' holes): by the batch after the others,
+            // by the server with nothing else admitted.  Already alone: it
+            // goes on under the plain limit
+            if (defer_alone_) {
+                I.need_alone = true;
+                throw CapacityError("device arena exhausted", true);
+            }
             break;
         }
         sync();

@@ -137,6 +137,7 @@ struct Inst {
     bool done = false;
     bool deferred = false;      // batch/server: evicted for memory, restarted later
     bool parked = false;        // batch: waiting, state kept, to go on alone after the others
+    bool need_alone = false;    // deferred because the arena could not hold it beside the others
     uint64_t prio = 0;          // admission order: older instances evict younger ones
     bool check_only = false;    // this level is the onTheFly last round
     bool next_check_only = false;
@@ -192,6 +193,9 @@ class Engine {
     void drain() { sync(); }
     void abandon(Inst& I);
     void forget(std::vector<Inst*>& v);
+    // members that cannot get arena memory while others hold it are deferred
+    // (need_alone) instead of going on under the plain limit
+    void set_defer_alone(bool b) { defer_alone_ = b; }
     size_t arena_size() const { return arena_->size(); }
 
   private:

@@ -57,6 +57,10 @@ struct vfb_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Req*> queue;
+    // requests that could not get arena memory beside the others: each runs
+    // alone, after the running ones finish, as its plain solve would
+    std::deque<Req*> solo;
+    Req* solo_run = nullptr;
     std::map<uint64_t, std::unique_ptr<Req>> reqs;
     std::vector<Req*> active;
     uint64_t next_ticket = 1;
@@ -75,11 +79,17 @@ void vfb_server::loop() {
         std::vector<Req*> admit;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+            cv_work.wait(lk, [&] { return stop || !queue.empty() || !active.empty() || !solo.empty(); });
             if (stop) return;
             uint64_t used = 0;
             for (Req* r : active) used += r->st.used;
-            while (!queue.empty() && (int)(active.size() + admit.size()) < max_active && used < budget) {
+            if (!solo_run && !solo.empty() && active.empty()) {
+                solo_run = solo.front();   // the arena is empty now: it runs alone
+                solo.pop_front();
+                admit.push_back(solo_run);
+            }
+            while (!solo_run && solo.empty() && !queue.empty() && (int)(active.size() + admit.size()) < max_active &&
+                   used < budget) {
                 admit.push_back(queue.front());
                 queue.pop_front();
                 used += 1 << 20;   // a new request starts with about this much (atoms, tables)
@@ -93,6 +103,7 @@ void vfb_server::loop() {
             try {
                 Engine& eng = engine_acquire(0);
                 eng.begin(opt, true);
+                eng.set_defer_alone(solo_run == nullptr);
                 std::vector<Inst*> fresh, act;
                 for (Req* r : admit) {
                     reset_req(*r, opt, per_req);
@@ -123,10 +134,12 @@ void vfb_server::loop() {
             active.clear();
             const auto now = Clock::now();
             for (Req* r : touched) {
+                if (r == solo_run && r->st.done) solo_run = nullptr;
                 if (!r->st.done) {
                     active.push_back(r);
                 } else if (r->st.deferred) {
-                    queue.push_front(r);   // restarted once memory is free
+                    if (r->st.need_alone) solo.push_back(r);   // restarted alone
+                    else queue.push_front(r);                  // restarted once memory is free
                 } else {
                     r->res.ms_total = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
                     if (!r->res.formula) r->res.formula = strdup("Not found !");
@@ -237,7 +250,7 @@ extern "C" void vfb_server_stats(vfb_server* s, uint64_t* queued, uint64_t* acti
                                  uint64_t* peak_active) {
     if (!s) return;
     std::lock_guard<std::mutex> lk(s->mu);
-    if (queued) *queued = s->queue.size();
+    if (queued) *queued = s->queue.size() + s->solo.size();
     if (active) *active = s->active.size();
     if (bytes) *bytes = s->bytes_active;
     if (peak_active) *peak_active = s->peak_active;
