@@ -191,6 +191,26 @@ void vfb_options_default(vfb_options* opt);
 /* Plain single-instance solve: the drop-in for LTLI (LTL_BS.cu:1069-1532). */
 vfb_status vfb_solve(const vfb_instance* inst, const vfb_options* opt, vfb_result* out);
 
+/* Device scratch of vfb_solve_noalloc, per GPU (256 MiB). */
+#define VFB_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+/* vfb_solve without any device or pinned-memory allocation, for callers
+   that cannot allocate (serving loops that forbid hipMalloc, harnesses that
+   own all device memory).  No counterpart in the reference.  Every device
+   buffer of the solve -- level rows, hash set, fresh lists, winner maps,
+   descriptors -- comes from one static device array of
+   VFB_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from its first
+   use on that GPU until the process ends; host staging is static too.  The
+   caller still passes the instance and the result.  Results equal
+   vfb_solve's with the same options; a solve that needs more device memory
+   than the scratch returns VFB_CAPACITY (formula "see_the_error", a message
+   on stderr) instead of being cut short, and search = VFB_SEARCH_BRUTE is
+   refused (VFB_INVALID_INPUT).  The timing option is ignored.  The array
+   keeps the previous call's contents, and the solve clears what must start
+   empty; calls are serialised by the library's lock (two calls overlapping
+   on one GPU would otherwise race on the one array). */
+vfb_status vfb_solve_noalloc(const vfb_instance* inst, const vfb_options* opt, vfb_result* out);
+
 /* Many independent instances stepped together, one level per step, in shared
    launches.  out[i] is identical to vfb_solve(insts[i]).  No reference
    counterpart (the reference solves one instance per process). */

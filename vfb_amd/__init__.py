@@ -20,6 +20,7 @@ MAX_SIZE = 32
 MAX_TRACES = 128
 MAX_ATOMS = 64
 MAX_LEN = 4096
+NOALLOC_SCRATCH_BYTES = 256 << 20
 OK, NOT_FOUND, INVALID_INPUT, CAPACITY, DEVICE_ERROR, IO_ERROR = range(6)
 OP_NAMES = ["~", "&", "|", "X", "F", "G", "U", "atom"]
 
@@ -73,7 +74,7 @@ EXPORTS = [
     "vfb_server_create", "vfb_server_submit", "vfb_server_poll", "vfb_server_wait",
     "vfb_server_stats", "vfb_server_destroy",
     "vfb_tuning_default", "vfb_get_tuning", "vfb_set_tuning", "vfb_tuning_save",
-    "vfb_tuning_load", "vfb_autotune", "vfb_set_arena_bytes",
+    "vfb_tuning_load", "vfb_autotune", "vfb_set_arena_bytes", "vfb_solve_noalloc",
 ]
 
 _lib = None
@@ -112,6 +113,7 @@ def lib():
     L.vfb_options_default.argtypes = [C.POINTER(Options)]
     L.vfb_options_default.restype = None
     L.vfb_solve.argtypes = [vp, C.POINTER(Options), C.POINTER(Result)]
+    L.vfb_solve_noalloc.argtypes = [vp, C.POINTER(Options), C.POINTER(Result)]
     L.vfb_solve_batch.argtypes = [C.POINTER(vp), C.c_size_t, C.POINTER(Options), C.POINTER(Result)]
     L.vfb_result_free.argtypes = [C.POINTER(Result)]
     L.vfb_result_free.restype = None
@@ -323,6 +325,20 @@ def solve(inst, max_size=MAX_SIZE, relaxed_type=3, timing=False, budget=0, verbo
     o = _opts(max_size, relaxed_type, timing, budget, verbose, max_rows, search, bs_max_size, costs, neg_type)
     r = Result()
     lib().vfb_solve(inst.handle, C.byref(o), C.byref(r))
+    try:
+        return _todict(r)
+    finally:
+        lib().vfb_result_free(C.byref(r))
+
+
+def solve_noalloc(inst, max_size=MAX_SIZE, relaxed_type=3, budget=0, max_rows=0, bs_max_size=0, costs=None,
+                  neg_type=1):
+    """solve() through vfb_solve_noalloc: no device allocation, everything in
+    the library's static device scratch (NOALLOC_SCRATCH_BYTES per GPU); a
+    solve that needs more returns status CAPACITY."""
+    o = _opts(max_size, relaxed_type, False, budget, False, max_rows, SEARCH_CS, bs_max_size, costs, neg_type)
+    r = Result()
+    lib().vfb_solve_noalloc(inst.handle, C.byref(o), C.byref(r))
     try:
         return _todict(r)
     finally:

@@ -40,15 +40,60 @@ Engine::Engine(size_t arena_bytes) {
     arena_ = std::make_unique<Arena>(arena_bytes);
     // probe-limit failures of rehash / atom insertion: a device word the
     // kernels set, copied back at the next stream sync after such a launch
-    CK(hipMalloc((void**)&d_flag_, sizeof(unsigned int)));
+    CK(hipMalloc((void**)&d_flag_, 256));
     CK(hipMemset(d_flag_, 0, sizeof(unsigned int)));
     CK(hipHostMalloc((void**)&h_flag_, sizeof(unsigned int), hipHostMallocDefault));
     *h_flag_ = 0;
     tun_ = current_tuning();
 }
 
+// Host side of a fixed-memory engine: static (pageable) staging for uploads,
+// counters and the failure flag.  All fixed-memory engines share them; calls
+// are serialised by the library's lock and every call ends with a stream
+// sync, after which nothing in them is in flight.
+static constexpr size_t kFixedStage = 4u << 20;
+static constexpr size_t kFixedStep = 1u << 20;      // descriptor area carved from the scratch
+static constexpr size_t kFixedCtrs = 64;
+alignas(256) static char g_fixed_stage[kFixedStage];
+static Ctr g_fixed_ctr[kFixedCtrs];
+static unsigned int g_fixed_flag;
+
+Engine::Engine(const Fixed& m) {
+    fixed_ = true;
+    vfb_options_default(&opt_);
+    CK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    int dev = 0;
+    CK(hipGetDevice(&dev));
+    CK(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
+    // [0, 256): failure flag; [256, 256 + kFixedStep): descriptors; the rest: arena
+    const size_t head = 256 + kFixedStep;
+    if (m.bytes < head + (16u << 20)) throw DeviceError("fixed engine memory too small");
+    char* base = (char*)m.dev;
+    d_flag_ = (unsigned int*)base;
+    d_step_ = base + 256;
+    step_cap_ = kFixedStep;
+    arena_ = std::make_unique<Arena>(base + head, m.bytes - head);
+    h_flag_ = &g_fixed_flag;
+    *h_flag_ = 0;
+    h_ctr_ = g_fixed_ctr;
+    h_ctr_cap_ = kFixedCtrs;
+    pin_ = g_fixed_stage;
+    pin_cap_ = kFixedStage;
+    // the scratch keeps the previous process state's contents: clear the flag
+    CK(launch_fill((unsigned long long*)d_flag_, 1, 0, 1, st_));
+    CK(hipStreamSynchronize(st_));
+    tun_ = current_tuning();
+}
+
 Engine::~Engine() {
     (void)hipStreamSynchronize(st_);
+    if (fixed_) {   // nothing of it was allocated but the stream and events
+        for (auto e : ev_pool_) (void)hipEventDestroy(e);
+        for (auto& t : timed_) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+        arena_.reset();
+        if (st_) (void)hipStreamDestroy(st_);
+        return;
+    }
     if (d_step_) (void)hipFree(d_step_);
     if (d_ops_) (void)hipFree(d_ops_);
     if (h_flag_) (void)hipHostFree(h_flag_);
@@ -84,6 +129,23 @@ void Engine::release_after_sync(Inst& I, Buf& b) {
 
 void Engine::h2d(void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
+    if (fixed_) {
+        // static staging: copies larger than it go in pieces, each after
+        // the stream has drained the staging
+        const char* s = (const char*)src;
+        char* d = (char*)dst;
+        while (bytes) {
+            if (pin_used_ + 256 > pin_cap_) sync();
+            const size_t n = std::min(bytes, pin_cap_ - pin_used_);
+            std::memcpy(pin_ + pin_used_, s, n);
+            CK(hipMemcpyAsync(d, pin_ + pin_used_, n, hipMemcpyHostToDevice, st_));
+            pin_used_ += align256(n);
+            s += n;
+            d += n;
+            bytes -= n;
+        }
+        return;
+    }
     const size_t need = align256(bytes);
     if (pin_used_ + need > pin_cap_) {
         sync();
@@ -103,6 +165,7 @@ void Engine::h2d(void* dst, const void* src, size_t bytes) {
 // so a later phase may overwrite it; growing it waits for the stream first.
 char* Engine::step_buf(size_t bytes) {
     if (bytes > step_cap_) {
+        if (fixed_) throw ScratchExhausted("descriptor area of the fixed scratch");
         sync();
         if (d_step_) CK(hipFree(d_step_));
         step_cap_ = std::max(bytes, step_cap_ * 2);
@@ -155,6 +218,7 @@ void Engine::dalloc(Inst& I, Buf& b, size_t bytes) {
     size_t rb = std::max<size_t>(align256(bytes), 256);
     if (I.used + rb > I.budget) throw CapacityError("per-solve HBM budget exhausted", false);
     void* p = arena_->alloc(rb);
+    if (!p && fixed_) throw ScratchExhausted("device scratch");
     while (!p && batch_) {
         // Shared arena full: evict the youngest other instance that holds
         // memory, so the oldest always progresses as it would alone.  The
@@ -917,13 +981,15 @@ void Engine::epoch(std::vector<Inst*>& v) {
     // host-coherent counters: when every instance's epoch ends with a scan,
     // the scan writes them there and no read-back copy is needed
     if (n > h_ctr_cap_) {
+        if (fixed_) throw ScratchExhausted("counter staging of the fixed engine");
         sync();
         if (h_ctr_) CK(hipHostFree(h_ctr_));
         h_ctr_ = nullptr;
         h_ctr_cap_ = std::max<size_t>(n, std::max<size_t>(2 * h_ctr_cap_, 64));
         CK(hipHostMalloc((void**)&h_ctr_, h_ctr_cap_ * sizeof(Ctr), hipHostMallocCoherent));
     }
-    bool all_scan = true;
+    // (a fixed engine's counters are pageable host memory: always copied back)
+    bool all_scan = !fixed_;
     for (auto* I : v) all_scan &= !checks_only(*I);
     std::vector<char> host(bytes, 0);
     InstDesc* hd = (InstDesc*)host.data();
@@ -1092,7 +1158,7 @@ void Engine::forget(std::vector<Inst*>& v) {
     }
     *h_flag_ = 0;
     flag_dirty_ = false;
-    (void)hipMemsetAsync(d_flag_, 0, sizeof(unsigned int), st_);
+    (void)launch_fill((unsigned long long*)d_flag_, 1, 0, 1, st_);
     (void)hipStreamSynchronize(st_);
     std::vector<Inst*> all;
     for (Inst* I : v) {
@@ -1342,6 +1408,9 @@ void Engine::step(std::vector<Inst*>& act) {
             uint64_t keep = W;
             if (I.max_rows) keep = std::min(keep, I.max_rows > I.stored ? I.max_rows - I.stored : 0);
             uint64_t avail = I.budget > I.used ? I.budget - I.used : 0;
+            if (fixed_ && align256(std::min<uint64_t>(keep, avail > 256 ? (avail - 256) / row_bytes : 0) *
+                                   row_bytes) > arena_->largest_free())
+                throw ScratchExhausted("device scratch");   // not the budget: the scratch
             if (!batch_) avail = std::min<uint64_t>(avail, arena_->largest_free());
             keep = std::min<uint64_t>(keep, avail > 256 ? (avail - 256) / row_bytes : 0);
             keep = std::min<uint64_t>(keep, 0xffffffffull);
@@ -1464,11 +1533,45 @@ static void set_costs(Inst& I, const vfb_options& o) {
     I.neg_type = o.neg_type == 2 ? 2 : 1;
 }
 
-static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const vfb_options* opt_in, vfb_result* out) {
+// ---------------------------------------------------------------------------
+// vfb_solve_noalloc: every device buffer of the solve lives in this array.
+// Host code never passes its name to a kernel or copy: the array's device
+// address is looked up once per device (hipGetSymbolAddress, with that device
+// current) and the fixed engine of that device carves everything from it.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[VFB_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kNoAllocDevices = 64;
+struct NoAllocDevice {
+    void* ptr = nullptr;             // device address of `scratch` on this device
+    std::unique_ptr<Engine> eng;     // fixed engine over it
+};
+static NoAllocDevice g_noalloc[kNoAllocDevices];   // under g_mu
+
+// set-up on first use per device: the symbol's address, then the engine
+static Engine& noalloc_engine() {
+    int dev = 0;
+    CK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) throw DeviceError("device ordinal out of range");
+    NoAllocDevice& d = g_noalloc[dev];
+    if (!d.ptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw DeviceError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        d.ptr = p;
+    }
+    if (!d.eng) d.eng = std::make_unique<Engine>(Engine::Fixed{d.ptr, (size_t)VFB_NOALLOC_SCRATCH_BYTES});
+    return *d.eng;
+}
+
+static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const vfb_options* opt_in, vfb_result* out,
+                             bool noalloc = false) {
     vfb_options opt;
     vfb_options_default(&opt);
     if (opt_in) opt = *opt_in;
     if (opt.relaxed_type < 1 || opt.relaxed_type > 3) opt.relaxed_type = 3;
+    if (noalloc) opt.timing = 0;   // no events to create
     for (size_t k = 0; k < n; ++k) {
         std::memset(&out[k], 0, sizeof(vfb_result));
         out[k].status = VFB_NOT_FOUND;
@@ -1516,8 +1619,10 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
         if (worst == VFB_INVALID_INPUT && n == 1) return VFB_INVALID_INPUT;
         return n == 1 ? out[0].status : VFB_OK;
     }
+    Engine* used = nullptr;
     try {
-        Engine& eng = engine_acquire(opt.hbm_budget);
+        Engine& eng = noalloc ? noalloc_engine() : engine_acquire(opt.hbm_budget);
+        used = &eng;
         uint64_t per = opt.hbm_budget ? std::min<uint64_t>(opt.hbm_budget, eng.arena_size()) : eng.arena_size();
         for (auto* I : run) I->budget = per;
         eng.run(run, opt);
@@ -1540,6 +1645,17 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
             std::vector<Inst*> one{I};
             eng.run(one, opt);
         }
+    } catch (ScratchExhausted& e) {
+        // only a fixed engine throws this: the solve needs more than its scratch
+        std::fprintf(stderr, "vfb_solve_noalloc: the solve needs more than the %llu-byte device scratch (%s)\n",
+                     (unsigned long long)VFB_NOALLOC_SCRATCH_BYTES, e.what());
+        used->forget(run);
+        for (auto* I : run) {
+            reset_result(I->res);
+            I->res->status = VFB_CAPACITY;
+            I->res->formula = strdup("see_the_error");
+        }
+        return VFB_CAPACITY;
     } catch (std::exception& e) {
         std::fprintf(stderr, "vfb: device error: %s\n", e.what());
         for (auto* I : run) {
@@ -1551,8 +1667,11 @@ static vfb_status solve_many(const vfb_instance* const* insts, size_t n, const v
         }
         // a server's requests live in the engine: release only this call's
         // instances; otherwise start over with a new engine
-        if (engine_pinned() && g_engine) g_engine->forget(run);
-        else g_engine.reset();
+        if (noalloc || (engine_pinned() && g_engine)) {
+            if (used) used->forget(run);
+        } else {
+            g_engine.reset();
+        }
         return VFB_DEVICE_ERROR;
     }
     double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1571,6 +1690,18 @@ extern "C" vfb_status vfb_solve(const vfb_instance* inst, const vfb_options* opt
     if (!out) return VFB_INVALID_INPUT;
     const vfb_instance* one[1] = {inst};
     return solve_many(one, 1, opt, out);
+}
+
+extern "C" vfb_status vfb_solve_noalloc(const vfb_instance* inst, const vfb_options* opt, vfb_result* out) {
+    if (!out) return VFB_INVALID_INPUT;
+    if (opt && opt->search != VFB_SEARCH_CS) {   // the brute-force search allocates its own buffers
+        std::memset(out, 0, sizeof *out);
+        out->status = VFB_INVALID_INPUT;
+        out->formula = strdup("see_the_error");
+        return VFB_INVALID_INPUT;
+    }
+    const vfb_instance* one[1] = {inst};
+    return solve_many(one, 1, opt, out, true);
 }
 
 extern "C" vfb_status vfb_solve_batch(const vfb_instance* const* insts, size_t n, const vfb_options* opt,

@@ -66,11 +66,15 @@ static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 // ---------------------------------------------------------------------------
 class Arena {
   public:
-    explicit Arena(size_t bytes) : size_(bytes) {
+    explicit Arena(size_t bytes) : size_(bytes), owned_(true) {
         CK(hipMalloc(&base_, bytes));
         free_[0] = bytes;
     }
-    ~Arena() { (void)hipFree(base_); }
+    // over device memory the caller owns (the no-allocation entry point's scratch)
+    Arena(void* base, size_t bytes) : base_(base), size_(bytes), owned_(false) { free_[0] = bytes; }
+    ~Arena() {
+        if (owned_) (void)hipFree(base_);
+    }
     void* alloc(size_t bytes) {
         bytes = std::max<size_t>(align256(bytes), 256);
         for (auto it = free_.begin(); it != free_.end(); ++it) {
@@ -109,6 +113,7 @@ class Arena {
   private:
     void* base_ = nullptr;
     size_t size_;
+    bool owned_;
     std::map<size_t, size_t> free_;
     std::map<size_t, size_t> used_;
 };
@@ -123,6 +128,13 @@ struct Buf {
 struct CapacityError : std::runtime_error {
     bool arena;
     CapacityError(const std::string& s, bool a) : std::runtime_error(s), arena(a) {}
+};
+
+// an engine over fixed memory (vfb_solve_noalloc) needs more than it holds:
+// the solve fails instead of switching to the onTheFly round, so that every
+// result it returns equals vfb_solve's
+struct ScratchExhausted : std::runtime_error {
+    explicit ScratchExhausted(const std::string& s) : std::runtime_error(s) {}
 };
 
 // ---------------------------------------------------------------------------
@@ -182,6 +194,14 @@ struct Inst {
 class Engine {
   public:
     explicit Engine(size_t arena_bytes);
+    // Over fixed device memory: no HIP allocation after construction (the
+    // arena, descriptors and flag are carved from `dev`; host staging is
+    // static), plain solves only, no timing events
+    struct Fixed {
+        void* dev;
+        size_t bytes;
+    };
+    explicit Engine(const Fixed& m);
     ~Engine();
     void run(std::vector<Inst*>& insts, const vfb_options& opt);
     // incremental use (server): begin() sets the options; start() sets up new
@@ -200,6 +220,7 @@ class Engine {
 
   private:
     vfb_options opt_;
+    bool fixed_ = false;          // built over fixed memory (Engine(const Fixed&))
     bool batch_ = false;
     hipStream_t st_ = nullptr;
     int cus_ = 256;
