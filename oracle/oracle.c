@@ -33,9 +33,10 @@
  *     whole level was deduplicated and checked (the reference stops deduping
  *     mid-level at a device-dependent point); the next level is check-only.
  *
- * Parity unpinned: the folding of a type-3 half-key equal to -1 onto -2
- * (at the end of ora_key, hash_set.cuh:198-204).  It happens with probability
- * 2^-64 per key, so no test input reaches it; it is a restatement only.
+ * The folding of a type-3 half-key equal to -1 onto -2 (at the end of
+ * ora_key, hash_set.cuh:198-204) is pinned by constructed instances whose
+ * trace words invert SplitMix64 so that one CS's half-key is -1 and
+ * another's is -2 (tests/fold_instance.py, tests/test_oracle.py).
  */
 #define _POSIX_C_SOURCE 200809L
 #include "oracle.h"
