@@ -1,18 +1,30 @@
 """Benchmark of the CS enumeration hot path on one MI355X.
 
-Workload (the dedup-stress configuration): instances 0..M-1 of synthetic
-config 2 -- 5 atomic propositions, 40 traces (20 positive / 20 negative),
-lengths uniform in [16, 32], bits Bernoulli(0.3), labels from a planted
-size-9 formula containing Until, max_formula_size 14, hashed 128-bit keys
-(total trace length > 126 bits).  One step = solving all M instances, each
-as a plain single-instance solve (the drop-in path).  Inputs are resident:
-the instances are generated and their atom rows uploaded to HBM
-(vfb_instance_upload) before the timed region.
+Headline workload (the largest single-GPU configuration): the config-4
+batched mix -- 512 independent synthetic instances with APs uniform in [2,8],
+traces in [8,63] with a random positive/negative split, lengths in [8,64],
+bits Bernoulli(p), p uniform in [0.2,0.6], labels from planted random
+formulas of size 5-13, max_formula_size 16 -- solved together by
+vfb_solve_batch (shared launches, members that fill the GPU alone parked and
+run one at a time), with launch parameters from the library's autotuner on a
+48-instance sample.  One step = one batched solve of all 512 instances.
+Inputs are resident: the instances are generated and their atom rows
+uploaded to HBM (vfb_instance_upload) before the timed region.
 
-Prints ONE JSON line: distinct CS/s over the timed steps, plus batched vs
-plain instances/s (checked bit-exact), p50/p99 solve latency, the roofline
-of the combine+key+dedup kernel (k_expand) from HIP events, and the CPU
-baseline (the single-threaded oracle on a bounded sample).
+value = distinct CS per second over the timed steps, where distinct CS is
+the reference's StoredLTLs count (`distinct[]`: the CS stored per completed
+level, and at the level of the returned formula the survivors ordered before
+it), summed over the 512 instances.  Beside it: the same instances solved one
+after another by the plain path (instances/s, batched vs plain, checked
+bit-exact), the combine+key+dedup kernel's roofline from HIP events over the
+timed region, and the CPU baseline (OpenMP over instances, on a bounded
+sample of the mix).
+
+Sections: config 2 (dedup stress, 32 plain solves), config 3 (limits,
+instances 4 and 10), multi-word traces, and the served stream (config 5:
+Poisson arrivals of config-4-distributed requests, 200 req/s for 60 s).
+
+Prints ONE JSON line.
 """
 import argparse
 import json
@@ -25,6 +37,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MIX_N, MIX_MAX = 512, 16
+GIANT_MS = 10.0            # a plain solve this long fills the GPU alone
+SERVE_REQUEST_BUDGET = 96 << 30
+FIELDS = ("status", "min_size", "formula", "distinct", "candidates", "level_winners", "otf_level", "otf_stored",
+          "found_ordinal")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def pct(xs, q):
@@ -37,159 +58,201 @@ def pct(xs, q):
     return xs[f] + (xs[c] - xs[f]) * (k - f)
 
 
-def profiled_traffic():
-    """HBM bytes per k_expand launch of this workload, from the committed
+def profiled_traffic(workload):
+    """HBM bytes per k_expand launch of a workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by the copy-kernel
-    calibration (profiles/k_expand_config2_main.md); None if absent."""
-    path = os.path.join(ROOT, "profiles", "k_expand_traffic.json")
+    calibration (profiles/k_expand_traffic.json); None if absent."""
     try:
-        with open(path) as f:
-            return json.load(f)["traffic_bytes_per_launch"]
+        with open(os.path.join(ROOT, "profiles", "k_expand_traffic.json")) as f:
+            return json.load(f)[workload]
     except (OSError, ValueError, KeyError):
         return None
 
 
-def cpu_baseline(insts, max_size, budget_s, gpu_results=None):
-    """Multi-core CPU build of the same enumeration (oracle/cpu_omp.c,
-    OpenMP over this host's cores) on the same instances, timed around each
-    solve alone; stops after budget_s seconds of CPU work.  Its results are
-    compared with the GPU's."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    done, distinct, t_total, same = 0, 0, 0.0, True
-    for k, inst in enumerate(insts):
-        names, lens, atoms, P = inst.arrays()
-        r, dt = O.enumerate_omp(O.Instance(names, lens, atoms, P), max_size, 3)
-        t_total += dt
-        distinct += sum(r["distinct"])
-        done += 1
-        if gpu_results is not None:
-            g = gpu_results[k]
-            same &= all(r[f] == g[f] for f in ("status", "min_size", "formula", "distinct", "candidates"))
-        if t_total > budget_s:
-            break
-    return {"value": distinct / t_total if t_total else None, "unit": "distinct CS/s",
-            "cores": O.omp_threads(), "kind": "port",
-            "sample": f"config-2 instances 0..{done - 1} at max_formula_size {max_size} "
-                      f"({distinct} distinct CS in {t_total:.2f} s, OpenMP build of the same "
-                      f"enumeration, {O.omp_threads()} threads)",
-            "bit_exact_vs_gpu": same}
+def roofline(ks, traffic):
+    """The combine+key+dedup kernel (k_expand, dedup passes) over the
+    counters of vfb_kernel_stats: algorithmic bytes / HIP-event time."""
+    ms, by, n = ks["ms_expand"], ks["bytes_expand"], ks["expand_launches"]
+    ach = by / (ms * 1e-3) / 1e9 if ms else None
+    out = {"bound": "hbm", "kernel": "k_expand", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": ach / HBM_PEAK_GBS if ach else None,
+           "traffic": traffic["bytes_per_launch"] if traffic else None,
+           "launches": n, "ms_per_launch": ms / n if n else None,
+           "alg_bytes_per_launch": by / n if n else None}
+    if traffic and ms and n:
+        # physical: the profiled HBM bytes of an average launch over the
+        # average launch time measured here
+        phys = traffic["bytes_per_launch"] / (ms / n * 1e-3) / 1e9
+        out.update({"physical_GBps": phys, "physical_frac": phys / HBM_PEAK_GBS,
+                    "traffic_source": traffic.get("source")})
+    return out
 
 
-def batched_mix(V, n, max_size, autotune=True):
-    """Config 4: n uneven instances, batched vs sequential plain solves.
+def _same(a, b):
+    return all(a[f] == b[f] for f in FIELDS)
 
-    The library's autotuner first picks launch / pass parameters on a sample
-    of the mix (instances 0..47, batched); both legs then run with them.  Also
-    reported: the same comparison without the instances that keep the GPU
-    busy alone (plain solve over 10 ms), where batching has idle GPU to fill."""
-    insts = [V.Instance.generate(4, k) for k in range(n)]
+
+def headline(V, steps, warmup, autotune):
+    log("config-4 mix: generating and uploading 512 instances")
+    insts = [V.Instance.generate(4, k).upload() for k in range(MIX_N)]
     tuned = None
     if autotune:
-        sample = insts[:min(48, n)]
-        tuned = V.autotune(sample, max_size=max_size, batched=True, reps=2)
-    V.solve(insts[0], max_size=max_size)
+        log("autotuning on instances 0..47 (batched)")
+        tuned = V.autotune(insts[:48], max_size=MIX_MAX, batched=True, reps=2)
+        log(f"tuned {tuned}")
+    for w in range(warmup):
+        t0 = time.perf_counter()
+        V.solve_batch(insts, max_size=MIX_MAX, timing=2)
+        log(f"warmup {w} {time.perf_counter() - t0:.3f}s")
+    V.kernel_stats(reset=True)
+    times, runs = [], []
+    for s in range(steps):
+        t0 = time.perf_counter()
+        res = V.solve_batch(insts, max_size=MIX_MAX, timing=2)
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        runs.append(res)
+        log(f"step {s} {dt:.3f}s")
+    ks = V.kernel_stats()
+    T = sum(times)
+    distinct = sum(sum(r["distinct"]) for res in runs for r in res)
+    level_distinct = sum(sum(r["level_winners"]) for res in runs for r in res)
+    cands = sum(sum(r["level_candidates"]) for res in runs for r in res)
+    steady = all(_same(a, b) for res in runs[1:] for a, b in zip(runs[0], res))
+    log("plain sequential solves of the same 512")
     t0 = time.perf_counter()
-    plain = [V.solve(i, max_size=max_size) for i in insts]
+    plain = [V.solve(i, max_size=MIX_MAX) for i in insts]
     t_plain = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    batch = V.solve_batch(insts, max_size=max_size)
-    t_batch = time.perf_counter() - t0
-    fields = ("status", "min_size", "formula", "distinct", "candidates", "level_winners", "otf_level", "otf_stored")
-    exact = all(a[f] == b[f] for a, b in zip(plain, batch) for f in fields)
+    exact = all(_same(a, b) for a, b in zip(plain, runs[-1]))
     lat = [r["ms_total"] for r in plain]
-    small = [k for k in range(n) if plain[k]["ms_total"] < 10.0]
+    small = [k for k in range(MIX_N) if plain[k]["ms_total"] < GIANT_MS]
     t_small_plain = sum(plain[k]["ms_total"] for k in small) / 1e3
     t0 = time.perf_counter()
-    sb = V.solve_batch([insts[k] for k in small], max_size=max_size)
+    sb = V.solve_batch([insts[k] for k in small], max_size=MIX_MAX)
     t_small_batch = time.perf_counter() - t0
-    exact_small = all(plain[k][f] == b[f] for k, b in zip(small, sb) for f in fields)
-    return {"instances": n, "max_formula_size": max_size,
-            "instances_per_s_plain": n / t_plain, "instances_per_s_batched": n / t_batch,
-            "batched_vs_plain": t_plain / t_batch, "bit_exact": exact,
+    exact_small = all(_same(plain[k], b) for k, b in zip(small, sb))
+    statuses = {}
+    for r in runs[-1]:
+        statuses[V.status_string(r["status"])] = statuses.get(V.status_string(r["status"]), 0) + 1
+    return {
+        "insts": insts, "plain": plain, "small": small, "t_small_batch": t_small_batch,
+        "out": {
+            "value": distinct / T, "ms_per_step": 1e3 * T / steps,
+            "level_distinct_per_s": level_distinct / T, "candidates_per_s": cands / T,
+            "instances_per_s_batched": MIX_N * steps / T, "instances_per_s_plain": MIX_N / t_plain,
+            "batched_vs_plain": t_plain / (T / steps), "bit_exact_vs_plain": exact and steady,
             "plain_latency_ms_p50": pct(lat, 0.5), "plain_latency_ms_p99": pct(lat, 0.99),
-            "distinct_cs": sum(sum(r["level_winners"]) for r in plain),
-            "batch_restarts": sum(r["restarts"] for r in batch), "batch_parked": sum(r["parked"] for r in batch),
-            "gpu_saturating_instances": n - len(small),
+            "batch_restarts": sum(r["restarts"] for r in runs[-1]),
+            "batch_parked": sum(r["parked"] for r in runs[-1]),
+            "statuses": statuses,
+            "gpu_saturating_instances": MIX_N - len(small),
             "gpu_saturating_share_of_plain_time": 1.0 - t_small_plain / t_plain,
             "others": {"instances": len(small), "batched_vs_plain": t_small_plain / t_small_batch,
                        "bit_exact": exact_small},
-            "autotuned": tuned}
+            "kernel_ms": {"expand": ks["ms_expand"]},
+            "roofline": roofline(ks, profiled_traffic("config4_mix")),
+            "autotuned": tuned,
+        },
+    }
 
 
-def served(V, rate, seconds, max_size, max_active, check):
-    """Config 5: open-loop Poisson arrivals of config-4-distributed requests."""
-    import random
-    import threading
-    rng = random.Random(5)
-    arrivals, t = [], 0.0
-    while True:
-        t += rng.expovariate(rate)
-        if t > seconds:
-            break
-        arrivals.append(t)
-    insts = [V.Instance.generate(5, k) for k in range(len(arrivals))]
-    results = [None] * len(insts)
-    with V.Server(max_size=max_size, max_active=max_active) as srv:
-        tickets = [None] * len(insts)
-        submitted = threading.Semaphore(0)
+def cpu_baseline_mix(V, H):
+    """OpenMP over instances (oracle/cpu_omp.c, each instance by the
+    sequential enumeration, dynamic scheduling) on the mix's instances whose
+    plain GPU solve takes under GIANT_MS -- the ones that fill the GPU alone
+    take minutes each on the host -- compared bit-exact with the GPU."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    small = H["small"]
+    oi = []
+    for k in small:
+        names, lens, atoms, P = H["insts"][k].arrays()
+        oi.append(O.Instance(names, lens, atoms, P))
+    res, dt = O.enumerate_many_omp(oi, MIX_MAX, 3)
+    same = all(all(r[f] == H["plain"][k][f] for f in ("status", "min_size", "formula", "distinct", "candidates",
+                                                         "found_ordinal"))
+               for r, k in zip(res, small))
+    distinct = sum(sum(r["distinct"]) for r in res)
+    gpu_distinct = sum(sum(H["plain"][k]["distinct"]) for k in small)
+    return {"value": distinct / dt, "unit": "distinct CS/s", "cores": O.omp_threads(), "kind": "port",
+            "sample": f"the {len(small)} instances of the 512-instance config-4 mix whose plain GPU solve takes "
+                      f"under {GIANT_MS:g} ms ({distinct} distinct CS in {dt:.2f} s; OpenMP over instances, "
+                      f"{O.omp_threads()} threads, each instance by the sequential enumeration); the "
+                      f"{MIX_N - len(small)} left out fill the GPU alone and take minutes each on the host",
+            "gpu_batched_same_sample_distinct_per_s": gpu_distinct / H["t_small_batch"],
+            "bit_exact_vs_gpu": same}
 
-        def collect():
-            for k in range(len(insts)):
-                submitted.acquire()
-                results[k] = srv.wait(tickets[k])
-        th = threading.Thread(target=collect)
-        th.start()
+
+def config2(V, n, max_size, steps, cpu_seconds):
+    """Config 2 (dedup stress): n plain single-instance solves per step."""
+    insts = [V.Instance.generate(2, k).upload() for k in range(n)]
+    for i in insts:
+        V.solve(i, max_size=max_size)
+    V.kernel_stats(reset=True)
+    times, runs = [], []
+    for s in range(steps):
         t0 = time.perf_counter()
+        runs.append([V.solve(i, max_size=max_size, timing=2) for i in insts])
+        times.append(time.perf_counter() - t0)
+    ks = V.kernel_stats()
+    T = sum(times)
+    plain = runs[-1]
+    V.solve_batch(insts, max_size=max_size)
+    t0 = time.perf_counter()
+    batch = V.solve_batch(insts, max_size=max_size)
+    t_batch = time.perf_counter() - t0
+    lat = [r["ms_total"] for res in runs for r in res]
+    out = {"workload": f"config 2 dedup stress: instances 0..{n - 1} (5 APs, 40 traces 20+/20-, len U[16,32], "
+                       f"p=0.3, planted size 9 with U), plain single-instance solves, max_formula_size {max_size}",
+           "distinct_per_s": sum(sum(r["distinct"]) for res in runs for r in res) / T,
+           "ms_per_step": 1e3 * T / steps,
+           "instances_per_s_plain": n * steps / T, "instances_per_s_batched": n / t_batch,
+           "batched_vs_plain": (T / steps) / t_batch,
+           "batched_bit_exact": all(_same(a, b) for a, b in zip(plain, batch)),
+           "latency_ms_p50": pct(lat, 0.5), "latency_ms_p99": pct(lat, 0.99),
+           "roofline": roofline(ks, profiled_traffic("config2_main"))}
+    if cpu_seconds:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        done, distinct, t_total, same = 0, 0, 0.0, True
+        for k, inst in enumerate(insts):
+            names, lens, atoms, P = inst.arrays()
+            r, dt = O.enumerate_omp(O.Instance(names, lens, atoms, P), max_size, 3)
+            t_total += dt
+            distinct += sum(r["distinct"])
+            done += 1
+            same &= all(r[f] == plain[k][f] for f in ("status", "min_size", "formula", "distinct", "candidates"))
+            if t_total > cpu_seconds:
+                break
+        out["cpu_baseline"] = {"value": distinct / t_total if t_total else None, "unit": "distinct CS/s",
+                               "cores": O.omp_threads(), "kind": "port",
+                               "sample": f"config-2 instances 0..{done - 1} at max_formula_size {max_size}, "
+                                         f"OpenMP within each solve",
+                               "bit_exact_vs_gpu": same}
+    return out
 
-        def monitor():
-            while th.is_alive():
-                time.sleep(5)
-                done = sum(r is not None for r in results)
-                print(f"[bench] served {time.perf_counter() - t0:.0f}s: {done}/{len(insts)} done, "
-                      f"{srv.stats()}", file=sys.stderr, flush=True)
-        threading.Thread(target=monitor, daemon=True).start()
-        for k, a in enumerate(arrivals):
-            d = t0 + a - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            tickets[k] = srv.submit(insts[k])
-            submitted.release()
-        th.join()
-        elapsed = time.perf_counter() - t0
-        stats = srv.stats()
-    lat = [r["ms_total"] for r in results]
-    sample = list(range(0, len(insts), max(1, len(insts) // check)))[:check]
-    plain_t0 = time.perf_counter()
-    plain = {k: V.solve(insts[k], max_size=max_size) for k in sample}
-    plain_rate = len(sample) / (time.perf_counter() - plain_t0)
-    fields = ("status", "min_size", "formula", "distinct", "candidates", "level_winners")
-    exact = all(plain[k][f] == results[k][f] for k in sample for f in fields)
-    return {"rate_req_s": rate, "seconds": seconds, "requests": len(insts), "max_active": max_active,
-            "instances_per_s": len(insts) / elapsed, "latency_ms_p50": pct(lat, 0.5),
-            "latency_ms_p99": pct(lat, 0.99), "peak_active": stats["peak_active"],
-            "plain_sequential_instances_per_s": plain_rate, "bit_exact_vs_plain_sample": exact,
-            "checked": len(sample)}
 
-
-def limits(V, ks, max_size):
-    """Config 3 (10 APs, 63 traces of length 48-64, max size 18): large levels,
-    where the combine+dedup kernel runs at its bandwidth roof."""
+def limits(V, ks_list, max_size):
+    """Config 3 (10 APs, 63 traces of 48-64 steps, max size 18), a fixed set
+    of instances: large levels, sets grown toward 2^28 slots."""
     out = []
-    for k in ks:
-        inst = V.Instance.generate(3, k)
+    for k in ks_list:
+        inst = V.Instance.generate(3, k).upload()
         V.solve(inst, max_size=max_size)            # warm
+        V.kernel_stats(reset=True)
         t0 = time.perf_counter()
-        r = V.solve(inst, max_size=max_size, timing=True)
+        r = V.solve(inst, max_size=max_size, timing=2)
         dt = time.perf_counter() - t0
-        ach = r["bytes_expand"] / (r["ms_expand"] * 1e-3) / 1e9 if r["ms_expand"] else None
+        ks = V.kernel_stats()
+        rf = roofline(ks, profiled_traffic(f"config3_{k}"))
         out.append({"instance": k, "status": V.status_string(r["status"]), "min_size": r["min_size"],
                     "seconds": dt, "distinct_cs": sum(r["distinct"]),
                     "distinct_per_s": sum(r["distinct"]) / dt,
                     "candidates_per_s": sum(r["level_candidates"]) / dt,
-                    "onthefly_level": r["otf_level"],
-                    "k_expand_ms": r["ms_expand"], "k_expand_launches": r["expand_launches"],
-                    "k_expand_alg_GBps": ach, "k_expand_frac_of_hbm": ach / HBM_PEAK_GBS if ach else None})
+                    "table_slots": r["table_slots"], "onthefly_level": r["otf_level"],
+                    "k_expand_ms": ks["ms_expand"], "k_expand_launches": ks["expand_launches"],
+                    "k_expand_alg_GBps": rf["achieved"], "k_expand_frac_of_hbm": rf["frac"],
+                    "k_expand_physical_frac": rf.get("physical_frac")})
     return out
 
 
@@ -211,16 +274,15 @@ def multiword(V, n, stretch, max_size):
     for k in range(n):
         names, lens, atoms, P = V.Instance.generate(2, k).arrays()
         st = [[_stutter(w, L, stretch) for w, L in zip(row, lens)] for row in atoms]
-        inst = V.Instance.from_bits(names, [L * stretch for L in lens], st, P)
+        inst = V.Instance.from_bits(names, [L * stretch for L in lens], st, P).upload()
         insts.append(inst)
         words.append(V.lib().vfb_instance_words(inst.handle))
     V.solve(insts[0], max_size=max_size)          # warm
+    V.kernel_stats(reset=True)
     t0 = time.perf_counter()
     rs = [V.solve(i, max_size=max_size, timing=2) for i in insts]
     dt = time.perf_counter() - t0
-    ms = sum(r["ms_expand"] for r in rs)
-    by = sum(r["bytes_expand"] for r in rs)
-    ach = by / (ms * 1e-3) / 1e9 if ms else None
+    rf = roofline(V.kernel_stats(), None)
     st = {}
     for r in rs:
         st[V.status_string(r["status"])] = st.get(V.status_string(r["status"]), 0) + 1
@@ -230,8 +292,80 @@ def multiword(V, n, stretch, max_size):
             "distinct_per_s": sum(sum(r["distinct"]) for r in rs) / dt,
             "candidates_per_s": sum(sum(r["level_candidates"]) for r in rs) / dt,
             "statuses": st, "min_sizes": [r["min_size"] for r in rs],
-            "k_expand_mw_ms": ms, "k_expand_mw_launches": sum(r["expand_launches"] for r in rs),
-            "k_expand_mw_alg_GBps": ach, "k_expand_mw_frac_of_hbm": ach / HBM_PEAK_GBS if ach else None}
+            "k_expand_mw_ms": rf["launches"] and rf["ms_per_launch"] * rf["launches"],
+            "k_expand_mw_launches": rf["launches"],
+            "k_expand_mw_alg_GBps": rf["achieved"], "k_expand_mw_frac_of_hbm": rf["frac"]}
+
+
+def served(V, rate, seconds, max_size, max_active, check, request_budget):
+    """Config 5: open-loop Poisson arrivals of config-4-distributed requests.
+    Every request, and every plain solve it is checked against, runs under
+    the same per-request HBM cap (below the arena), so that a request that
+    needs more is cut the same way in both and the arena keeps room to admit
+    small requests while a giant runs."""
+    import random
+    import threading
+    rng = random.Random(5)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.expovariate(rate)
+        if t > seconds:
+            break
+        arrivals.append(t)
+    insts = [V.Instance.generate(5, k) for k in range(len(arrivals))]
+    results = [None] * len(insts)
+    queued_seen = []
+    with V.Server(max_size=max_size, max_active=max_active, request_budget=request_budget) as srv:
+        tickets = [None] * len(insts)
+        submitted = threading.Semaphore(0)
+
+        def collect():
+            for k in range(len(insts)):
+                submitted.acquire()
+                results[k] = srv.wait(tickets[k])
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+
+        def monitor():
+            while th.is_alive():
+                time.sleep(1)
+                s = srv.stats()
+                queued_seen.append(s["queued"])
+                if len(queued_seen) % 5 == 0:
+                    done = sum(r is not None for r in results)
+                    log(f"served {time.perf_counter() - t0:.0f}s: {done}/{len(insts)} done, {s}")
+        threading.Thread(target=monitor, daemon=True).start()
+        for k, a in enumerate(arrivals):
+            d = t0 + a - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[k] = srv.submit(insts[k])
+            submitted.release()
+        th.join()
+        elapsed = time.perf_counter() - t0
+        stats = srv.stats()
+    lat = [r["ms_total"] for r in results]
+    # giants: a level of at least 2^24 candidates (the size at which a
+    # batch member fills the device alone)
+    giant = [max(r["level_candidates"]) >= (1 << 24) for r in results]
+    lat_small = [x for x, g in zip(lat, giant) if not g]
+    sample = list(range(0, len(insts), max(1, len(insts) // check)))[:check]
+    giants = [k for k, g in enumerate(giant) if g]
+    sample = sorted(set(sample) | set(giants[:4]))     # the check includes giants
+    plain_t0 = time.perf_counter()
+    plain = {k: V.solve(insts[k], max_size=max_size, budget=request_budget) for k in sample}
+    plain_rate = len(sample) / (time.perf_counter() - plain_t0)
+    exact = all(_same(plain[k], results[k]) for k in sample)
+    return {"rate_req_s": rate, "seconds": seconds, "requests": len(insts), "max_active": max_active,
+            "request_budget_bytes": request_budget,
+            "instances_per_s": len(insts) / elapsed, "latency_ms_p50": pct(lat, 0.5),
+            "latency_ms_p99": pct(lat, 0.99), "giant_requests": len(giants),
+            "non_giant_latency_ms_p50": pct(lat_small, 0.5), "non_giant_latency_ms_p99": pct(lat_small, 0.99),
+            "queued_p50": pct(queued_seen, 0.5), "queued_max": max(queued_seen) if queued_seen else None,
+            "peak_active": stats["peak_active"], "restarts": sum(r["restarts"] for r in results),
+            "plain_sequential_instances_per_s": plain_rate, "bit_exact_vs_plain_sample": exact,
+            "checked": len(sample), "checked_giants": len([k for k in sample if giant[k]])}
 
 
 def main():
@@ -239,19 +373,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--instances", type=int, default=32)
-    ap.add_argument("--max-size", type=int, default=14)
-    ap.add_argument("--cpu-max-size", type=int, default=14)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mix", type=int, default=512, help="config-4 instances for batched vs plain (0: skip)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0, help="config-5 served stream length (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-active", type=int, default=128)
+    ap.add_argument("--config2", type=int, default=32, help="config-2 instances (0: skip)")
     ap.add_argument("--limits", default="4,10", help="config-3 instances to time (empty: skip)")
     ap.add_argument("--multiword", default="8,8,12",
                     help="instances,stretch,max_size of the long-trace workload (empty: skip)")
-    ap.add_argument("--only", choices=["main", "limits", "mix", "multiword", "serve"], default=None,
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="served stream length (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-active", type=int, default=128)
+    ap.add_argument("--only", choices=["mix", "config2", "limits", "multiword", "serve"], default=None,
                     help="run one section alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -264,119 +395,64 @@ def main():
     if dev is None:
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
-    print(f"[bench] device {dev}", file=sys.stderr, flush=True)
+    log(f"device {dev}")
     if args.only == "limits":
         print(json.dumps({"limits": limits(V, [int(x) for x in args.limits.split(",")], 18)}))
         return
-    if args.only == "mix":
-        print(json.dumps({"batched_mix": batched_mix(V, args.mix, 16)}))
+    if args.only == "config2":
+        print(json.dumps({"config2": config2(V, args.config2, 14, args.steps, 0)}))
         return
     if args.only == "serve":
-        print(json.dumps({"served": served(V, args.serve_rate, args.serve_seconds, 16, args.serve_active, 256)}))
+        print(json.dumps({"served": served(V, args.serve_rate, args.serve_seconds, MIX_MAX, args.serve_active, 256,
+                                           SERVE_REQUEST_BUDGET)}))
         return
     if args.only == "multiword":
         n, k, m = (int(x) for x in args.multiword.split(","))
         print(json.dumps({"multiword": multiword(V, n, k, m)}))
         return
-    # inputs resident in HBM before the timed region: each instance's atom
-    # rows are uploaded once, and the solves read them there
-    insts = [V.Instance.generate(2, k).upload() for k in range(args.instances)]
 
-    def step():
-        res = []
-        for inst in insts:
-            # timing=2: HIP events around the expand kernel only (the roofline's)
-            res.append(V.solve(inst, max_size=args.max_size, timing=2))
-        return res
-
-    for w in range(args.warmup):
-        t0 = time.perf_counter()
-        step()
-        print(f"[bench] warmup {w} {time.perf_counter() - t0:.3f}s", file=sys.stderr, flush=True)
-
-    step_times, all_res = [], []
-    for s in range(args.steps):
-        t0 = time.perf_counter()
-        res = step()
-        dt = time.perf_counter() - t0
-        step_times.append(dt)
-        all_res.append(res)
-        print(f"[bench] step {s} {dt:.3f}s", file=sys.stderr, flush=True)
-    T = sum(step_times)
-    # the reference's count: CS stored per level, and at the level of the
-    # returned formula the survivors ordered before it (StoredLTLs)
-    distinct = sum(sum(r["distinct"]) for res in all_res for r in res)
-    level_distinct = sum(sum(r["level_winners"]) for res in all_res for r in res)
-    cands = sum(sum(r["level_candidates"]) for res in all_res for r in res)
-    lat = [r["ms_total"] for res in all_res for r in res]
-    plain = all_res[-1]
-    statuses = {}
-    for r in plain:
-        statuses[r["status"]] = statuses.get(r["status"], 0) + 1
-
-    if args.only == "main":
-        print(json.dumps({"distinct_per_s": distinct / T, "ms_per_step": 1e3 * T / args.steps}))
-        return
-    # batched: the same instances stepped together; must equal the plain runs
-    V.solve_batch(insts, max_size=args.max_size)  # warm
-    t0 = time.perf_counter()
-    batch = V.solve_batch(insts, max_size=args.max_size)
-    t_batch = time.perf_counter() - t0
-    exact = all(a[f] == b[f] for a, b in zip(plain, batch)
-                for f in ("status", "min_size", "formula", "distinct", "candidates", "level_winners"))
-    t_plain = step_times[-1]
-
-    bytes_exp = sum(r["bytes_expand"] for res in all_res for r in res)
-    ms_exp = sum(r["ms_expand"] for res in all_res for r in res)
-    launches = sum(r["expand_launches"] for res in all_res for r in res)
-    achieved = bytes_exp / (ms_exp * 1e-3) / 1e9 if ms_exp else None
-
+    H = headline(V, args.steps, args.warmup, not args.no_autotune)
+    h = H["out"]
     out = {
         "metric": "distinct CS/sec + instances/sec batched vs plain (bit-exact); p50/p99 solve latency",
-        "value": distinct / T,
+        "value": h["value"],
         "unit": "distinct CS/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * T / args.steps,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic",
-        "config": {"workload": f"config 2 dedup stress: {args.instances} instances "
-                               f"(5 APs, 40 traces 20+/20-, len U[16,32], p=0.3, planted size 9 with U), "
-                               f"plain single-instance solves, max_formula_size {args.max_size}, "
-                               f"relaxed 128-bit hashed keys",
-                   "instances": args.instances, "max_formula_size": args.max_size,
-                   "parallelism": "single GPU"},
-        "candidates_per_s": cands / T,
-        "level_distinct_per_s": level_distinct / T,
-        "instances_per_s_plain": len(insts) / t_plain,
-        "instances_per_s_batched": len(insts) / t_batch,
-        "batched_vs_plain": t_plain / t_batch,
-        "batched_bit_exact": exact,
-        "latency_ms_p50": pct(lat, 0.5), "latency_ms_p99": pct(lat, 0.99),
-        "statuses": {V.status_string(k): v for k, v in statuses.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_expand", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": profiled_traffic(),
-                     "launches": launches, "ms_per_launch": ms_exp / launches if launches else None,
-                     "alg_bytes_per_launch": bytes_exp / launches if launches else None},
-        "device": dev,
+        "config": {"workload": "config-4 batched mix: 512 instances (APs U[2,8], traces U[8,63] random +/- split, "
+                               "len U[8,64], bits Bernoulli(p) p~U[0.2,0.6], planted formulas size 5-13), "
+                               "one vfb_solve_batch per step, max_formula_size 16, autotuned launch parameters, "
+                               "relaxed 128-bit hashed keys where total trace length > 126",
+                   "instances": MIX_N, "max_formula_size": MIX_MAX, "parallelism": "single GPU"},
+        "roofline": h["roofline"],
     }
+    out.update({k: v for k, v in h.items() if k not in ("value", "ms_per_step", "roofline")})
+    out["latency_ms_p50"] = h["plain_latency_ms_p50"]
+    out["latency_ms_p99"] = h["plain_latency_ms_p99"]
+    out["device"] = dev
+    if args.only == "mix":
+        print(json.dumps(out))
+        return
+    if not args.no_cpu:
+        log("cpu baseline (OpenMP over the mix's instances)")
+        out["cpu_baseline"] = cpu_baseline_mix(V, H)
+    del H
+    if args.config2:
+        log("config 2")
+        out["config2"] = config2(V, args.config2, 14, 3, 0 if args.no_cpu else 20.0)
     if args.limits:
-        print("[bench] limits config", file=sys.stderr, flush=True)
+        log("config 3 (limits)")
         out["limits"] = limits(V, [int(x) for x in args.limits.split(",")], 18)
     if args.multiword:
-        print("[bench] multi-word traces", file=sys.stderr, flush=True)
+        log("multi-word traces")
         n, k, m = (int(x) for x in args.multiword.split(","))
         out["multiword"] = multiword(V, n, k, m)
-    if args.mix:
-        print("[bench] batched mix", file=sys.stderr, flush=True)
-        out["batched_mix"] = batched_mix(V, args.mix, 16)
     if args.serve_seconds:
-        print("[bench] served stream", file=sys.stderr, flush=True)
-        out["served"] = served(V, args.serve_rate, args.serve_seconds, 16, args.serve_active, 256)
-    if not args.no_cpu:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(insts, args.cpu_max_size, args.cpu_seconds,
-                                           plain if args.cpu_max_size == args.max_size else None)
+        log("served stream")
+        out["served"] = served(V, args.serve_rate, args.serve_seconds, MIX_MAX, args.serve_active, 256,
+                               SERVE_REQUEST_BUDGET)
     print(json.dumps(out))
 
 

@@ -100,8 +100,8 @@ int vfb_instance_max_size(const vfb_instance* inst);        /* config default, 0
 /* Copies the instance's atom rows (its level-1 CS rows, A*W words) to device
    memory once.  Later vfb_solve calls on this handle read them there instead
    of uploading them per call (the reference uploads them in every LTLI call,
-   LTL_BS.cu:1216).  Batches of several instances and the server upload
-   their own copies.
+   LTL_BS.cu:1216); so do batches (vfb_solve_batch).  The server works on
+   its own copy of a submitted instance and uploads the rows itself.
    The device copy lives until vfb_instance_free and never travels with a copy
    of the instance.  VFB_DEVICE_ERROR without a usable GPU. */
 vfb_status vfb_instance_upload(vfb_instance* inst);
@@ -315,6 +315,28 @@ vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, const vfb_o
    or below 64 MiB.  Replaces the reference's sizing of its cache from the
    device's memory in every LTLI call (LTL_BS.cu:1179-1183). */
 vfb_status vfb_set_arena_bytes(uint64_t bytes);
+
+/* Kernel time and algorithmic bytes of every solve of the process, counted
+   once per launch (the per-result ms_* fields give each instance of a
+   batch the whole time of the launches it shared).  Times come from the HIP
+   events of solves run with vfb_options.timing (2: the combine+dedup kernel
+   only); bytes and launches are counted always.  No reference counterpart. */
+typedef struct {
+    double ms_expand;            /* combine+key+dedup kernel (k_expand), dedup passes */
+    uint64_t expand_launches;
+    uint64_t bytes_expand;       /* (a*8W + 16) per candidate of those launches (probe
+                                    candidates sharing a launch counted at a*8W) */
+    double ms_probe;             /* launches that only probed (target check) */
+    uint64_t probe_launches;
+    uint64_t bytes_probe;        /* a*8W per probed candidate */
+    double ms_resolve;
+    double ms_store;
+    uint64_t store_launches;
+    uint64_t bytes_store;        /* 8W + 24 per stored row */
+} vfb_kernel_stats;
+
+/* Copies the counters into *out (if not NULL), then clears them if reset. */
+void vfb_kernel_stats_get(vfb_kernel_stats* out, int reset);
 
 /* Device name and compute-unit count, for reports.  Returns VFB_DEVICE_ERROR
    when no HIP device is usable. */

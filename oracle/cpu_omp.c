@@ -512,3 +512,16 @@ done:
 }
 
 int ora_omp_threads(void) { return omp_get_max_threads(); }
+
+/* CPU baseline of a batch: n independent instances, OpenMP dynamic
+   scheduling over instances, each solved by the sequential ora_enumerate
+   (no row cap).  Results as ora_enumerate's. */
+int ora_enumerate_many_omp(const ora_instance* const* in, const char* const* const* names, int n, int max_size,
+                           int key_mode, ora_result* out)
+{
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+    for (int i = 0; i < n; ++i)
+        bad |= ora_enumerate(in[i], max_size, key_mode, 0, names[i], &out[i]) < 0;
+    return bad ? -1 : 0;
+}

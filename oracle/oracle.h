@@ -100,6 +100,10 @@ int ora_bruteforce(const ora_instance* in, int max_size,
 int ora_enumerate_omp(const ora_instance* in, int max_size, int key_mode, uint64_t pass_min,
                       const char* const* names, ora_result* out);
 int ora_omp_threads(void);
+/* CPU baseline of a batch (cpu_omp.c): OpenMP dynamic scheduling over the n
+   instances, each by ora_enumerate (no row cap); 0, or -1 on a failure */
+int ora_enumerate_many_omp(const ora_instance* const* in, const char* const* const* names, int n, int max_size,
+                           int key_mode, ora_result* out);
 
 /* closed-form count of syntactic formulas of each size, ltl_nocache.cu:248-287 */
 void ora_num_formulas(int A, int max_size, uint64_t* out /* [(max_size+1)*8] */);

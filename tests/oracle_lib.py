@@ -48,6 +48,10 @@ def omp_lib():
         _omp.ora_enumerate_omp.argtypes = [C.POINTER(OraInstance), C.c_int, C.c_int, C.c_uint64,
                                            C.POINTER(C.c_char_p), C.POINTER(OraResult)]
         _omp.ora_omp_threads.restype = C.c_int
+        _omp.ora_enumerate_many_omp.restype = C.c_int
+        _omp.ora_enumerate_many_omp.argtypes = [C.POINTER(C.POINTER(OraInstance)),
+                                                C.POINTER(C.POINTER(C.c_char_p)), C.c_int, C.c_int, C.c_int,
+                                                C.POINTER(OraResult)]
     return _omp
 
 
@@ -174,6 +178,24 @@ def enumerate_omp(inst, max_size, key_mode=3, pass_min=1 << 22):
     t0 = time.perf_counter()
     L.ora_enumerate_omp(C.byref(ci), max_size, key_mode, pass_min, names, C.byref(r))
     return _result(r), time.perf_counter() - t0
+
+
+def enumerate_many_omp(insts, max_size, key_mode=3):
+    """Batch CPU baseline: OpenMP over instances, each sequential; returns
+    (results, seconds of the whole call)."""
+    import time
+    n = len(insts)
+    conv = [i._c() for i in insts]
+    cis = (C.POINTER(OraInstance) * n)(*[C.pointer(c[0]) for c in conv])
+    nms = (C.POINTER(C.c_char_p) * n)(*[C.cast(c[2], C.POINTER(C.c_char_p)) for c in conv])
+    out = (OraResult * n)()
+    L = omp_lib()
+    t0 = time.perf_counter()
+    rc = L.ora_enumerate_many_omp(cis, nms, n, max_size, key_mode, out)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError("ora_enumerate_many_omp failed")
+    return [_result(out[k]) for k in range(n)], dt
 
 
 def omp_threads():

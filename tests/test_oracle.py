@@ -518,3 +518,13 @@ def test_fold_drops_one_distinct_cs_at_level_two(half):
     assert full["distinct"][2] == new
     assert hashed["distinct"][2] == full["distinct"][2] - 1
     assert hashed["candidates"] == full["candidates"]
+
+
+def test_omp_batch_baseline_equals_sequential_oracle():
+    """The batch CPU baseline (OpenMP over instances) returns each instance's
+    sequential oracle result."""
+    rng = random.Random(4242)
+    insts = [_rand_inst(rng, rng.randint(1, 4), rng.randint(2, 12), 1, 16) for _ in range(24)]
+    got, _ = O.enumerate_many_omp(insts, 7)
+    for k, inst in enumerate(insts):
+        assert got[k] == O.enumerate_cs(inst, 7, 3), k

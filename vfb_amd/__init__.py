@@ -64,6 +64,13 @@ class Tuning(C.Structure):
 TUNING_FIELDS = [f for f, _ in Tuning._fields_]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("ms_expand", C.c_double), ("expand_launches", C.c_uint64), ("bytes_expand", C.c_uint64),
+                ("ms_probe", C.c_double), ("probe_launches", C.c_uint64), ("bytes_probe", C.c_uint64),
+                ("ms_resolve", C.c_double), ("ms_store", C.c_double), ("store_launches", C.c_uint64),
+                ("bytes_store", C.c_uint64)]
+
+
 # every symbol declared in include/vfb.h
 EXPORTS = [
     "vfb_instance_load_json", "vfb_instance_load_text", "vfb_instance_from_bits", "vfb_instance_from_words",
@@ -74,7 +81,7 @@ EXPORTS = [
     "vfb_server_create", "vfb_server_submit", "vfb_server_poll", "vfb_server_wait",
     "vfb_server_stats", "vfb_server_destroy",
     "vfb_tuning_default", "vfb_get_tuning", "vfb_set_tuning", "vfb_tuning_save",
-    "vfb_tuning_load", "vfb_autotune", "vfb_set_arena_bytes", "vfb_solve_noalloc",
+    "vfb_tuning_load", "vfb_autotune", "vfb_set_arena_bytes", "vfb_solve_noalloc", "vfb_kernel_stats_get",
 ]
 
 _lib = None
@@ -137,6 +144,8 @@ def lib():
     L.vfb_tuning_save.argtypes = [TP, C.c_char_p]
     L.vfb_tuning_load.argtypes = [C.c_char_p, TP]
     L.vfb_set_arena_bytes.argtypes = [C.c_uint64]
+    L.vfb_kernel_stats_get.argtypes = [C.POINTER(KernelStats), C.c_int]
+    L.vfb_kernel_stats_get.restype = None
     L.vfb_autotune.argtypes = [C.POINTER(vp), C.c_size_t, C.POINTER(Options), C.c_int, C.c_int,
                                C.c_char_p, TP, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     _lib = L
@@ -413,6 +422,14 @@ def autotune(insts, max_size=MAX_SIZE, batched=True, reps=2, cache=None, relaxed
     if st != OK:
         raise VfbError(f"autotune: {status_string(st)}")
     return {"tuning": _tdict(best), "ms_default": t0.value, "ms_best": t1.value}
+
+
+def kernel_stats(reset=False):
+    """Process-wide kernel time (HIP events of solves run with timing) and
+    algorithmic bytes, counted once per launch."""
+    s = KernelStats()
+    lib().vfb_kernel_stats_get(C.byref(s), 1 if reset else 0)
+    return {f: getattr(s, f) for f, _ in KernelStats._fields_}
 
 
 def set_arena_bytes(nbytes):

@@ -27,6 +27,10 @@
 
 namespace vfb {
 
+// process-wide kernel counters (vfb_kernel_stats_get); updated under the
+// library's lock, like everything the engines do
+static vfb_kernel_stats g_kstats{};
+
 static void drop_live(std::vector<Inst*>& live, Inst* I) {
     live.erase(std::remove(live.begin(), live.end(), I), live.end());
 }
@@ -201,6 +205,8 @@ void Engine::collect_times() {
     for (auto& t : timed_) {
         float ms = 0.f;
         CK(hipEventElapsedTime(&ms, t.a, t.b));
+        (t.kind == 0 ? g_kstats.ms_expand : t.kind == 1 ? g_kstats.ms_resolve
+                                          : t.kind == 2 ? g_kstats.ms_store : g_kstats.ms_probe) += ms;
         for (Inst* I : t.who) {
             double* acc = t.kind == 0   ? &I->res->ms_expand
                           : t.kind == 1 ? &I->res->ms_resolve
@@ -531,7 +537,7 @@ void Engine::setup(Inst& I) {
         I.h.cinfo = (const ColInfo*)I.cols.p;
         I.h.tcol = (const int32_t*)((const uint64_t*)I.cols.p + n_ci);
     }
-    if (!batch_ && in.dev.p) {
+    if (in.dev.p) {
         // resident rows (vfb_instance_upload): read in place, never written
         I.rows[c1] = Buf{in.dev.p, 0, true};
     } else {
@@ -1028,7 +1034,9 @@ void Engine::epoch(std::vector<Inst*>& v) {
         CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, mw ? 1 : 0, grid, st_));
         time_end(ev, ekind, v);
         for (auto* I : v) I->res->expand_launches++;
+        (ekind == 0 ? g_kstats.expand_launches : g_kstats.probe_launches)++;
     }
+    last_epoch_probe_only_ = all_probe;
     bool any_dedup = false;
     for (auto* I : v) any_dedup |= !checks_only(*I);
     if (any_dedup) {
@@ -1090,6 +1098,7 @@ void Engine::store(std::vector<Inst*>& v) {
         time_begin(&ev, 2);
         CK(launch_store((const InstDesc*)d_step_, nullptr, 0, tot, p, small ? 1 : 0, grid, st_));
         time_end(ev, 2, v);
+        g_kstats.store_launches++;
         return;
     }
     const size_t o_seg = align256(n * sizeof(InstDesc));
@@ -1104,6 +1113,7 @@ void Engine::store(std::vector<Inst*>& v) {
     CK(launch_store((const InstDesc*)d, (const Seg*)(d + o_seg), (int)segs.size(), tot, StorePatch{}, small ? 1 : 0, grid,
                     st_));
     time_end(ev, 2, v);
+    g_kstats.store_launches++;
 }
 
 void Engine::begin(const vfb_options& opt, bool batch) {
@@ -1320,6 +1330,8 @@ void Engine::step(std::vector<Inst*>& act) {
                     // no match anywhere in the level: it cannot hold the
                     // answer, one fused pass; else the blocks up to the match
                     I.res->probe_candidates += c;
+                    // bytes counted with the kernel whose time they share
+                    (last_epoch_probe_only_ ? g_kstats.bytes_probe : g_kstats.bytes_expand) += b;
                     uint8_t us = SEL_ALL, bs = SEL_ALL;
                     uint64_t pre = 0;
                     if (I.ctr.found != kEmpty && first_pass_for(I, I.ctr.found, &us, &bs, &pre)) {
@@ -1335,6 +1347,7 @@ void Engine::step(std::vector<Inst*>& act) {
                 }
                 I.evaluated += c;
                 I.res->bytes_expand += b;
+                g_kstats.bytes_expand += b;
                 if (I.pass == P_FIRST && I.ctr.found == kEmpty) {
                     // the probe's match was no winner (its relaxed key was
                     // taken by a lower ordinal): the rest of the level
@@ -1440,6 +1453,7 @@ void Engine::step(std::vector<Inst*>& act) {
                 R.otf_stored = keep;
             }
             R.bytes_store += keep * (8 * (uint64_t)I.h.C + 24);
+            g_kstats.bytes_store += keep * (8 * (uint64_t)I.h.C + 24);
             I.nrows[s] = (uint32_t)keep;
             I.h.out_n = keep;
             if (keep) {
@@ -1741,6 +1755,12 @@ extern "C" vfb_status vfb_set_arena_bytes(uint64_t bytes) {
     g_arena_bytes = bytes;
     g_engine.reset();   // re-created with this arena at the next solve
     return VFB_OK;
+}
+
+extern "C" void vfb_kernel_stats_get(vfb_kernel_stats* out, int reset) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (out) *out = g_kstats;
+    if (reset) g_kstats = vfb_kernel_stats{};
 }
 
 extern "C" void vfb_tuning_default(vfb_tuning* t) {

@@ -257,6 +257,7 @@ class Engine {
     std::vector<Timed> timed_;
     std::vector<Inst*> live_;     // started, not finished: eviction candidates
     Inst* epoch_single_ = nullptr; // the one instance whose descriptor is at d_step_[0]
+    bool last_epoch_probe_only_ = false;   // the last epoch's expansion was timed as a probe
     vfb_tuning tun_{};            // launch / pass parameters, copied at begin()
     bool defer_alone_ = false;    // batch: an instance that cannot get arena memory is deferred
     bool park_ = false;           // batch: members reaching a level of tun_.park_min candidates wait
