@@ -88,13 +88,30 @@ def roofline(ks, traffic):
     return out
 
 
+def generate(V, config, n):
+    """Instances 0..n-1 of a configuration, generated on 16 host threads
+    (the generator's rejection sampling takes ~60 ms per mix instance; the
+    library call releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    out = [None] * n
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        futs = {ex.submit(V.Instance.generate, config, k): k for k in range(n)}
+        done = 0
+        for f in futs:
+            out[futs[f]] = f.result()
+            done += 1
+            if done % 2000 == 0:
+                log(f"generated {done}/{n} config-{config} instances")
+    return out
+
+
 def _same(a, b):
     return all(a[f] == b[f] for f in FIELDS)
 
 
 def headline(V, steps, warmup, autotune):
     log("config-4 mix: generating and uploading 512 instances")
-    insts = [V.Instance.generate(4, k).upload() for k in range(MIX_N)]
+    insts = [i.upload() for i in generate(V, 4, MIX_N)]
     tuned = None
     if autotune:
         log("autotuning on instances 0..47 (batched)")
@@ -312,7 +329,8 @@ def served(V, rate, seconds, max_size, max_active, check, request_budget):
         if t > seconds:
             break
         arrivals.append(t)
-    insts = [V.Instance.generate(5, k) for k in range(len(arrivals))]
+    log(f"generating {len(arrivals)} requests")
+    insts = generate(V, 5, len(arrivals))
     results = [None] * len(insts)
     queued_seen = []
     with V.Server(max_size=max_size, max_active=max_active, request_budget=request_budget) as srv:
