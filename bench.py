@@ -109,7 +109,7 @@ def _same(a, b):
     return all(a[f] == b[f] for f in FIELDS)
 
 
-def headline(V, steps, warmup, autotune):
+def headline(V, steps, warmup, autotune, batch_only=False):
     log("config-4 mix: generating and uploading 512 instances")
     insts = [i.upload() for i in generate(V, 4, MIX_N)]
     tuned = None
@@ -133,6 +133,8 @@ def headline(V, steps, warmup, autotune):
     ks = V.kernel_stats()
     T = sum(times)
     distinct = sum(sum(r["distinct"]) for res in runs for r in res)
+    if batch_only:        # the batched steps alone (profiler runs)
+        return {"out": {"value": distinct / T, "ms_per_step": 1e3 * T / steps, "kernel_stats": ks}}
     level_distinct = sum(sum(r["level_winners"]) for res in runs for r in res)
     cands = sum(sum(r["level_candidates"]) for res in runs for r in res)
     steady = all(_same(a, b) for res in runs[1:] for a, b in zip(runs[0], res))
@@ -400,7 +402,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="served stream length (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-active", type=int, default=128)
-    ap.add_argument("--only", choices=["mix", "config2", "limits", "multiword", "serve"], default=None,
+    ap.add_argument("--only", choices=["mix", "mixbatch", "config2", "limits", "multiword", "serve"], default=None,
                     help="run one section alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -429,6 +431,9 @@ def main():
         print(json.dumps({"multiword": multiword(V, n, k, m)}))
         return
 
+    if args.only == "mixbatch":
+        print(json.dumps(headline(V, args.steps, args.warmup, not args.no_autotune, True)["out"]))
+        return
     H = headline(V, args.steps, args.warmup, not args.no_autotune)
     h = H["out"]
     out = {
