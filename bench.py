@@ -109,13 +109,17 @@ def _same(a, b):
     return all(a[f] == b[f] for f in FIELDS)
 
 
-def headline(V, steps, warmup, autotune, batch_only=False):
+def headline(V, steps, warmup, autotune, batch_only=False, tuning_file=None, save_tuning=None):
     log("config-4 mix: generating and uploading 512 instances")
     insts = [i.upload() for i in generate(V, 4, MIX_N)]
     tuned = None
-    if autotune:
+    if tuning_file:
+        tuned = {"tuning": V.load_tuning(tuning_file), "from": tuning_file}
+        V.set_tuning(**tuned["tuning"])
+        log(f"tuning from {tuning_file}: {tuned['tuning']}")
+    elif autotune:
         log("autotuning on instances 0..47 (batched)")
-        tuned = V.autotune(insts[:48], max_size=MIX_MAX, batched=True, reps=2)
+        tuned = V.autotune(insts[:48], max_size=MIX_MAX, batched=True, reps=2, cache=save_tuning)
         log(f"tuned {tuned}")
     for w in range(warmup):
         t0 = time.perf_counter()
@@ -395,6 +399,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning-file", default=None, help="load the mix's launch tuning instead of autotuning")
+    ap.add_argument("--save-tuning", default=None, help="save the autotuned launch tuning here")
     ap.add_argument("--config2", type=int, default=32, help="config-2 instances (0: skip)")
     ap.add_argument("--limits", default="4,10", help="config-3 instances to time (empty: skip)")
     ap.add_argument("--multiword", default="8,8,12",
@@ -432,9 +438,10 @@ def main():
         return
 
     if args.only == "mixbatch":
-        print(json.dumps(headline(V, args.steps, args.warmup, not args.no_autotune, True)["out"]))
+        print(json.dumps(headline(V, args.steps, args.warmup, not args.no_autotune, True, args.tuning_file,
+                                  args.save_tuning)["out"]))
         return
-    H = headline(V, args.steps, args.warmup, not args.no_autotune)
+    H = headline(V, args.steps, args.warmup, not args.no_autotune, False, args.tuning_file, args.save_tuning)
     h = H["out"]
     out = {
         "metric": "distinct CS/sec + instances/sec batched vs plain (bit-exact); p50/p99 solve latency",

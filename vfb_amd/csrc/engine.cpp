@@ -658,25 +658,60 @@ void Engine::build_units(Inst& I) {
 // Passes of a level.  P_FUSED: every operator in one pass.  P_PROBE: target
 // check only (bit 0 of every candidate, no keys, no set), which finds m0, the
 // lowest matching candidate.  A level without one cannot hold the answer and
-// is built by P_FUSED.  Otherwise P_FIRST evaluates the canonical operator
-// blocks up to m0's block (the probe's match is then settled if it is a
-// winner, as it is unless a relaxed key collides), and P_REST the others.
+// is built by P_FUSED.  Otherwise P_FIRST builds the candidates with ordinals
+// 0..m0 (the probe's match is then settled if it is a winner, as it is unless
+// a relaxed key collides with a lower ordinal), and P_REST those above m0.
+// Every candidate below the returned formula is built, so counts and ranks
+// are those of a fused pass.
 enum Pass { P_FUSED = 0, P_PROBE = 1, P_FIRST = 2, P_REST = 3 };
 
 static size_t nsplit_of(const Inst& I) { return (I.units[I.level + 1].size() - 4) / 3; }
 
-// first pass for a match at m0: task masks and the end of the evaluated
-// prefix; false when m0 lies in the last block (U), for which P_FUSED is used
-static bool first_pass_for(const Inst& I, uint64_t m0, uint8_t* us, uint8_t* bs, uint64_t* prefix) {
+// operators of the first pass for a match at m0: the canonical blocks up to
+// m0's (the kernels skip what an operator mask leaves out)
+static void first_pass_for(const Inst& I, uint64_t m0, uint8_t* us, uint8_t* bs) {
     const auto& U = I.units[I.level + 1];
     const size_t ns = nsplit_of(I);
     const uint64_t b_and = U[1].base, b_or = U[ns + 1].base, b_x = U[2 * ns + 1].base, b_u = U[2 * ns + 4].base;
-    if (m0 < b_and) { *us = SEL_NOT; *bs = 0; *prefix = b_and; }
-    else if (m0 < b_or) { *us = SEL_NOT; *bs = SEL_AND; *prefix = b_or; }
-    else if (m0 < b_x) { *us = SEL_NOT; *bs = SEL_ANDOR; *prefix = b_x; }
-    else if (m0 < b_u) { *us = SEL_ALL; *bs = SEL_ANDOR; *prefix = b_u; }
-    else return false;
-    return true;
+    if (m0 < b_and) { *us = SEL_NOT; *bs = 0; }
+    else if (m0 < b_or) { *us = SEL_NOT; *bs = SEL_AND; }
+    else if (m0 < b_x) { *us = SEL_NOT; *bs = SEL_ANDOR; }
+    else if (m0 < b_u) { *us = SEL_ALL; *bs = SEL_ANDOR; }
+    else { *us = SEL_ALL; *bs = SEL_ALL; }
+}
+
+// level-local ordinal window [lo, hi] of the current pass
+static void pass_window(const Inst& I, uint64_t* lo, uint64_t* hi) {
+    *lo = 0;
+    *hi = kEmpty;
+    if (I.pass == P_FIRST) *hi = I.m0;
+    else if (I.pass == P_REST) *lo = I.m0 + 1;
+}
+
+// waves (of W candidates, nchunk set for W) of task t that hold one of its
+// selected candidates (t.part) with an ordinal at most hi: a prefix, since a
+// task's ordinals grow with its rows (unary) or left-major pairs (binary)
+static uint64_t waves_upto(const Task& t, uint64_t hi, uint64_t W) {
+    bool any = false;
+    uint64_t pmax = 0;
+    auto upd = [&](uint64_t base, uint64_t per) {
+        if (hi < base) return;
+        const uint64_t p = (hi - base) / per;
+        if (!any || p > pmax) pmax = p;
+        any = true;
+    };
+    if (t.kind == 0) {
+        for (int q = 0; q < 4; ++q)
+            if ((t.part >> q) & 1u) upd(t.base[q], 1);
+        if (!any) return 0;
+        return std::min<uint64_t>(pmax, (uint64_t)t.nl - 1) / W + 1;
+    }
+    if (t.part & 1u) upd(t.base[0], 1);
+    if (t.part & 2u) upd(t.base[1], 1);
+    if (t.part & 0xcu) upd(t.base[2], 2);   // l U r and r U l of pair p are 2p, 2p + 1
+    if (!any) return 0;
+    const uint64_t p = std::min<uint64_t>(pmax, (uint64_t)t.nl * t.nr - 1);
+    return (p / t.nr) * t.nchunk + (p % t.nr) / W + 1;
 }
 
 // whether unit q of the level is computed with unary / binary task masks us / bs
@@ -697,13 +732,19 @@ static void pass_work(const Inst& I, uint64_t* cand, uint64_t* bytes) {
     const auto& U = I.units[I.level + 1];
     const size_t ns = nsplit_of(I);
     const bool all = I.pass == P_FUSED || I.pass == P_PROBE;
+    uint64_t lo, hi;
+    pass_window(I, &lo, &hi);
     *cand = *bytes = 0;
     for (size_t q = 0; q < U.size(); ++q) {
         if (!all && !unit_selected(q, ns, I.usel, I.bsel)) continue;
         const Unit& u = U[q];
+        // the unit's ordinals [base, base + count) inside [lo, hi]
+        const uint64_t s = std::max(u.base, lo);
+        const uint64_t e = hi == kEmpty ? u.base + u.count : std::min(u.base + u.count, hi + 1);
+        const uint64_t n = e > s ? e - s : 0;
         const uint64_t a = (u.op == VFB_OP_AND || u.op == VFB_OP_OR) ? 2 : 1;
-        *cand += u.count;
-        *bytes += u.count * (a * 8 * (uint64_t)I.h.C + (checks_only(I) ? 0 : 16));
+        *cand += n;
+        *bytes += n * (a * 8 * (uint64_t)I.h.C + (checks_only(I) ? 0 : 16));
     }
 }
 
@@ -950,10 +991,16 @@ void Engine::epoch(std::vector<Inst*>& v) {
         t.part = (uint8_t)(sel & t.ops);
         return t.part != 0;
     };
+    // waves of a task at W candidates per wave; a first pass stops at m0
+    auto task_waves = [](const Inst& I, Task& t, uint64_t W) -> uint64_t {
+        t.nchunk = (uint32_t)((t.nr + W - 1) / W);
+        const uint64_t full = t.kind == 0 ? (t.nl + W - 1) / W : (uint64_t)t.nl * t.nchunk;
+        return I.pass == P_FIRST ? std::min(full, waves_upto(t, I.m0, W)) : full;
+    };
     uint64_t waves1 = 0;
     for (size_t k = 0; k < n; ++k)
         for (Task t : v[k]->tasks)
-            if (in_pass(*v[k], t)) waves1 += t.kind == 0 ? (t.nl + 63) / 64 : (uint64_t)t.nl * ((t.nr + 63) / 64);
+            if (in_pass(*v[k], t)) waves1 += task_waves(*v[k], t, 64);
     const int wide = waves1 < tun_.wide_max_waves ? 1 : 0;
     const uint64_t per_wave = wide ? 64 / kWideLanes : 64;
     std::vector<Task> tasks;
@@ -961,10 +1008,11 @@ void Engine::epoch(std::vector<Inst*>& v) {
     for (size_t k = 0; k < n; ++k)
         for (Task t : v[k]->tasks) {
             if (!in_pass(*v[k], t)) continue;
+            const uint64_t tw = task_waves(*v[k], t, per_wave);
+            if (!tw) continue;
             t.inst = (uint32_t)k;
             t.wave_begin = waves;
-            t.nchunk = (uint32_t)((t.nr + per_wave - 1) / per_wave);
-            waves += t.kind == 0 ? (t.nl + per_wave - 1) / per_wave : (uint64_t)t.nl * t.nchunk;
+            waves += tw;
             tasks.push_back(t);
         }
     std::vector<uint32_t> ids;
@@ -1004,7 +1052,8 @@ void Engine::epoch(std::vector<Inst*>& v) {
         Inst& I = *v[k];
         I.h.ctr = (Ctr*)(d + o_ctr) + k;
         I.h.mark = 1;
-        I.h.found_below = I.pass == P_FIRST ? I.first_prefix : ~0ull;
+        I.h.found_below = I.pass == P_FIRST ? I.m0 + 1 : ~0ull;
+        pass_window(I, &I.h.ord_lo, &I.h.ord_hi);
         hd[k] = I.h;
         hd[k].ctr_host = all_scan ? h_ctr_ + k : nullptr;
         hd[k].check_only = checks_only(I) ? 1 : 0;
@@ -1023,20 +1072,23 @@ void Engine::epoch(std::vector<Inst*>& v) {
     const InstDesc* D = (const InstDesc*)d;
 
     hipEvent_t ev = nullptr;
-    bool all_probe = true;
-    for (auto* I : v) all_probe &= I->pass == P_PROBE;
-    const int ekind = all_probe ? 3 : 0;    // probes are timed apart from the dedup expansion
+    // an epoch of target checks only (probes, onTheFly levels) runs k_check,
+    // timed apart from the dedup expansion
+    bool all_check = true;
+    for (auto* I : v) all_check &= checks_only(*I);
+    const int ekind = all_check ? 3 : 0;
     if (waves) {
         time_begin(&ev, ekind);
         int grid = (int)std::min<uint64_t>((waves + 3) / 4, (uint64_t)cus_ * tun_.expand_blocks_per_cu);
         bool mw = false;
         for (auto* I : v) mw |= I->h.mw != 0;
-        CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, mw ? 1 : 0, grid, st_));
+        CK(launch_expand(D, (const Task*)(d + o_task), (int)tasks.size(), waves, wide, all_check ? -1 : mw ? 1 : 0,
+                         grid, st_));
         time_end(ev, ekind, v);
         for (auto* I : v) I->res->expand_launches++;
         (ekind == 0 ? g_kstats.expand_launches : g_kstats.probe_launches)++;
     }
-    last_epoch_probe_only_ = all_probe;
+    last_epoch_probe_only_ = all_check;
     bool any_dedup = false;
     for (auto* I : v) any_dedup |= !checks_only(*I);
     if (any_dedup) {
@@ -1332,13 +1384,13 @@ void Engine::step(std::vector<Inst*>& act) {
                     I.res->probe_candidates += c;
                     // bytes counted with the kernel whose time they share
                     (last_epoch_probe_only_ ? g_kstats.bytes_probe : g_kstats.bytes_expand) += b;
-                    uint8_t us = SEL_ALL, bs = SEL_ALL;
-                    uint64_t pre = 0;
-                    if (I.ctr.found != kEmpty && first_pass_for(I, I.ctr.found, &us, &bs, &pre)) {
+                    if (I.ctr.found != kEmpty) {
+                        uint8_t us = SEL_ALL, bs = SEL_ALL;
+                        first_pass_for(I, I.ctr.found, &us, &bs);
                         I.pass = P_FIRST;
+                        I.m0 = I.ctr.found;
                         I.usel = I.first_usel = us;
                         I.bsel = I.first_bsel = bs;
-                        I.first_prefix = pre;
                     } else {
                         I.pass = P_FUSED;
                     }
@@ -1346,14 +1398,20 @@ void Engine::step(std::vector<Inst*>& act) {
                     continue;
                 }
                 I.evaluated += c;
-                I.res->bytes_expand += b;
-                g_kstats.bytes_expand += b;
+                if (I.check_only && last_epoch_probe_only_) {
+                    // an onTheFly level ran in k_check, like a probe
+                    I.res->probe_candidates += c;
+                    g_kstats.bytes_probe += b;
+                } else {
+                    I.res->bytes_expand += b;
+                    g_kstats.bytes_expand += b;
+                }
                 if (I.pass == P_FIRST && I.ctr.found == kEmpty) {
                     // the probe's match was no winner (its relaxed key was
-                    // taken by a lower ordinal): the rest of the level
+                    // taken by a lower ordinal): the candidates above it
                     I.pass = P_REST;
-                    I.usel = (uint8_t)(SEL_ALL & ~I.usel);
-                    I.bsel = (uint8_t)(SEL_ALL & ~I.bsel);
+                    I.usel = SEL_ALL;
+                    I.bsel = SEL_ALL;
                     again.push_back(&I);
                 }
             }

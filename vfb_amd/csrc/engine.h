@@ -187,7 +187,7 @@ struct Inst {
     int pass = 0;               // next pass of the level (enum Pass in engine.cpp)
     uint8_t usel = 0xf, bsel = 0xf;     // operators of the pass: unary / binary task masks
     uint8_t first_usel = 0xf, first_bsel = 0xf;
-    uint64_t first_prefix = 0;  // every candidate below it is evaluated by the first pass
+    uint64_t m0 = 0;            // the probe's lowest match: the first pass builds ordinals 0..m0
     uint64_t evaluated = 0;     // candidates of this level evaluated so far
 };
 

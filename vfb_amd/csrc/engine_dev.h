@@ -103,6 +103,7 @@ struct InstDesc {
     uint64_t claim_limit;       // claims allowed this epoch before the set must grow
     uint64_t epoch_base;        // global ordinal of the level's first candidate
     uint64_t found_below;       // resolve: only matches below this level-local ordinal are reported
+    uint64_t ord_lo, ord_hi;    // dedup passes build only candidates with level-local ordinals in [ord_lo, ord_hi]
     uint64_t bs_target;         // BS stage: the positive-traces mask
     unsigned long long* fresh_ord;
     uint32_t* fresh_slot;

@@ -343,6 +343,15 @@ __device__ __forceinline__ void finish_one(const InstDesc& d, const Key& key, ui
     }
 }
 
+// candidates q (of the four) whose level-local ordinal lies in the pass's
+// window [ord_lo, ord_hi]
+__device__ __forceinline__ uint32_t in_window(const InstDesc& d, const uint64_t* ord) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m |= (ord[q] >= d.ord_lo && ord[q] <= d.ord_hi ? 1u : 0u) << q;
+    return m;
+}
+
 // Combines the per-lane partial keys / mismatch bits of the G lanes that
 // share a candidate (G > 1: each lane folded every G-th trace), then probes
 // (or, in a check-only level, checks) the four candidates.
@@ -438,7 +447,8 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
     }
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
     // SEL_ALL also serves any other mask (the cost function's): tk.part enables
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part) : 0u, lane);
+    const uint32_t win = CHECK ? 0xfu : in_window(d, ord);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part & win) : 0u, lane);
 }
 
 template <int KEY, int G>
@@ -506,7 +516,8 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     const uint64_t pair = l * tk.nr + r;
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part) : 0u, lane);
+    const uint32_t win = CHECK ? 0xfu : in_window(d, ord);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part & win) : 0u, lane);
 }
 
 template <int KEY, int G>
@@ -638,7 +649,8 @@ __device__ __forceinline__ void expand_unary_mw(const InstDesc& d, const Task& t
         }
     }
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part) : 0u, lane);
+    const uint32_t win = CHECK ? 0xfu : in_window(d, ord);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part & win) : 0u, lane);
 }
 
 // &, |, l U r, r U l of one pair over multi-word traces
@@ -688,7 +700,8 @@ __device__ __forceinline__ void expand_binary_mw(const InstDesc& d, const Task& 
     const uint64_t pair = l * tk.nr + r;
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part) : 0u, lane);
+    const uint32_t win = CHECK ? 0xfu : in_window(d, ord);
+    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part & win) : 0u, lane);
 }
 
 template <int KEY, int G>
@@ -752,6 +765,57 @@ __global__ __launch_bounds__(256) void k_expand_mw(const InstDesc* __restrict__ 
             if (d.key_kind == KEY_MIX) expand_binary_mw_part<KEY_MIX, G>(d, tk, wl, lane);
             else if (d.key_kind == KEY_FIELD) expand_binary_mw_part<KEY_FIELD, G>(d, tk, wl, lane);
             else expand_binary_mw_part<KEY_STRIDE, G>(d, tk, wl, lane);
+        }
+    }
+}
+
+// Lowest level-local ordinal among the candidates of wave wl of task tk
+// (W = candidates per wave): ~ X F G of rows x0..; & | of pairs p0.., whose
+// Until candidates are 2p and 2p + 1 of their block.
+__device__ __forceinline__ uint64_t wave_min_ordinal(const Task& tk, uint64_t wl, uint64_t W) {
+    uint64_t m = kEmpty;
+    if (tk.kind == 0) {
+        const uint64_t x0 = wl * W;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (((tk.part >> q) & 1u) && tk.base[q] + x0 < m) m = tk.base[q] + x0;
+    } else {
+        const uint64_t l = wl / tk.nchunk;
+        const uint64_t p0 = l * tk.nr + (wl - l * tk.nchunk) * W;
+        if ((tk.part & 1u) && tk.base[0] + p0 < m) m = tk.base[0] + p0;
+        if ((tk.part & 2u) && tk.base[1] + p0 < m) m = tk.base[1] + p0;
+        if ((tk.part & 0xcu) && tk.base[2] + 2 * p0 < m) m = tk.base[2] + 2 * p0;
+    }
+    return m;
+}
+
+// Target check only (the probe of a large level, or an onTheFly level): the
+// lowest matching ordinal is all it finds, so a wave whose candidates all lie
+// above a match already found skips (a stale read of `found` only skips less).
+template <int G>
+__global__ __launch_bounds__(256) void k_check(const InstDesc* __restrict__ D, const Task* __restrict__ tasks,
+                                               int ntasks, uint64_t total_waves) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t wpb = blockDim.x >> 6;
+    const uint64_t stride = (uint64_t)gridDim.x * wpb;
+    for (uint64_t w = (uint64_t)blockIdx.x * wpb + wib; w < total_waves; w += stride) {
+        int lo = 0, hi = ntasks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tasks[mid].wave_begin <= w) lo = mid; else hi = mid - 1;
+        }
+        const Task& tk = tasks[lo];
+        const InstDesc& d = D[tk.inst];
+        const uint64_t wl = w - tk.wave_begin;
+        const uint64_t found = __hip_atomic_load(&d.ctr->found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (found <= wave_min_ordinal(tk, wl, 64 / G)) continue;
+        if (d.mw) {
+            if (tk.kind == 0) expand_unary_mw<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
+            else expand_binary_mw<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
+        } else {
+            if (tk.kind == 0) expand_unary<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
+            else expand_binary<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
         }
     }
 }
@@ -1235,6 +1299,13 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
 
 hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int wide, int mw,
                          int grid, hipStream_t st) {
+    if (mw < 0) {   // every instance of the epoch checks only
+        if (wide)
+            hipLaunchKernelGGL(k_check<kWideLanes>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
+        else
+            hipLaunchKernelGGL(k_check<1>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
+        return hipGetLastError();
+    }
     if (mw && wide)
         hipLaunchKernelGGL(k_expand_mw<kWideLanes>, dim3(grid), dim3(256), 0, st, D, tasks, ntasks, total_waves);
     else if (mw)
