@@ -612,7 +612,7 @@ void Engine::build_units(Inst& I) {
             t->nr = 1;
             t->nchunk = 1;
         }
-        t->ops |= 1u << k;
+        t->ops = (uint16_t)(t->ops | (1u << k));
         t->base[k] = u.base;
     }
     std::vector<Task> bin;
@@ -632,7 +632,7 @@ void Engine::build_units(Inst& I) {
             t->nr = u.nr;
             t->nchunk = (t->nr + 63) / 64;
         }
-        t->ops |= bits;
+        t->ops = (uint16_t)(t->ops | bits);
         t->base[slot] = u.base;
     };
     for (size_t q : q_and) bin_add(q, 0x1, 0);
@@ -991,10 +991,16 @@ void Engine::epoch(std::vector<Inst*>& v) {
         t.part = (uint8_t)(sel & t.ops);
         return t.part != 0;
     };
-    // waves of a task at W candidates per wave; a first pass stops at m0
+    // an epoch of target checks only (probes, onTheFly levels) runs k_check,
+    // timed apart from the dedup expansion
+    bool all_check = true;
+    for (auto* I : v) all_check &= checks_only(*I);
+    // waves of a task at W candidates per wave; a first pass stops at m0; a
+    // binary task of k_check takes t.lrows left rows per wave
     auto task_waves = [](const Inst& I, Task& t, uint64_t W) -> uint64_t {
         t.nchunk = (uint32_t)((t.nr + W - 1) / W);
-        const uint64_t full = t.kind == 0 ? (t.nl + W - 1) / W : (uint64_t)t.nl * t.nchunk;
+        const uint64_t lr = t.lrows > 1 ? t.lrows : 1;
+        const uint64_t full = t.kind == 0 ? (t.nl + W - 1) / W : (uint64_t)((t.nl + lr - 1) / lr) * t.nchunk;
         return I.pass == P_FIRST ? std::min(full, waves_upto(t, I.m0, W)) : full;
     };
     uint64_t waves1 = 0;
@@ -1005,9 +1011,12 @@ void Engine::epoch(std::vector<Inst*>& v) {
     const uint64_t per_wave = wide ? 64 / kWideLanes : 64;
     std::vector<Task> tasks;
     uint64_t waves = 0;
+    // one right-operand word for kCheckRows left rows: target checks of
+    // one-word rows at one lane per candidate
     for (size_t k = 0; k < n; ++k)
         for (Task t : v[k]->tasks) {
             if (!in_pass(*v[k], t)) continue;
+            t.lrows = (uint8_t)(all_check && !wide && t.kind == 1 && !v[k]->h.mw && !v[k]->h.bs_mode ? kCheckRows : 1);
             const uint64_t tw = task_waves(*v[k], t, per_wave);
             if (!tw) continue;
             t.inst = (uint32_t)k;
@@ -1072,10 +1081,6 @@ void Engine::epoch(std::vector<Inst*>& v) {
     const InstDesc* D = (const InstDesc*)d;
 
     hipEvent_t ev = nullptr;
-    // an epoch of target checks only (probes, onTheFly levels) runs k_check,
-    // timed apart from the dedup expansion
-    bool all_check = true;
-    for (auto* I : v) all_check &= checks_only(*I);
     const int ekind = all_check ? 3 : 0;
     if (waves) {
         time_begin(&ev, ekind);

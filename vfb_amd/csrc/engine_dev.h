@@ -38,6 +38,7 @@ constexpr uint64_t kStoreSmallBlk = 16;  // this many chunks (more, shorter wave
 constexpr int kWideLanes = 8;         // lanes per candidate in k_expand's small-level mode
 constexpr uint64_t kWideMaxWaves = 512;  // levels below this many wavefronts use it (default)
 constexpr int kTraceGroup = 8;        // traces whose operand loads k_expand issues together
+constexpr int kCheckRows = 8;         // left rows per wavefront of k_check's binary tasks
 constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are counted in k_scan
 constexpr uint64_t kSmallFresh = 4096;  // fresh lists up to this long are resolved in the scan's workgroup
 constexpr uint64_t kMinWinWords = 1024; // winner maps are allocated for at least 64K candidates
@@ -137,7 +138,10 @@ struct Task {
     uint8_t kind, lvl_l, lvl_r, part;
     uint32_t nl, nr;
     uint32_t nchunk;            // ceil(nr / candidates per wavefront)
-    uint32_t ops;               // candidates of base[] that exist at this level (cost function)
+    uint16_t ops;               // candidates of base[] that exist at this level (cost function)
+    uint8_t lrows;              // k_check, binary: left rows per wavefront (one right word load
+                                // serves them all); 0 or 1 otherwise
+    uint8_t pad8;
     uint64_t base[4];
     uint64_t wave_begin;        // prefix over tasks, in wavefronts
 };

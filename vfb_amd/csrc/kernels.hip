@@ -59,12 +59,11 @@ __device__ __forceinline__ uint64_t apply_op(int op, uint64_t a, uint64_t b, uin
     }
 }
 
-// bit 0 of l U r: some k with r_k and l_j for all j < k; the first set bit
-// of r is the best witness (the doubling of until_op evaluated at time 0)
+// bit 0 of l U r: some k with r_k and l_j for all j < k.  l ^ (l + 1) is the
+// mask of bits 0..k0, k0 the first step where l fails (all bits when l holds
+// everywhere), so r must hold somewhere in it (until_op evaluated at time 0)
 __device__ __forceinline__ bool until_bit0(uint64_t l, uint64_t r) {
-    if (r == 0) return false;
-    const uint64_t below = (r & (0ull - r)) - 1ull;   // bits under the lowest set bit of r
-    return (l & below) == below;
+    return (r & (l ^ (l + 1ull))) != 0ull;
 }
 
 __device__ __forceinline__ uint32_t bit0_mismatch(uint64_t b0, uint64_t b1, bool b2, bool b3, uint64_t want) {
@@ -769,6 +768,60 @@ __global__ __launch_bounds__(256) void k_expand_mw(const InstDesc* __restrict__ 
     }
 }
 
+// Target check of tk.lrows (= K) left rows against one right row per lane,
+// one-word rows: each right word is loaded once for the K left rows (scalar
+// loads of consecutive words), so the larger operand level is streamed
+// ceil(nl / K) times instead of nl.  The lowest matching ordinal of the wave
+// goes to ctr->found.
+template <int K>
+__device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    const uint64_t lg = wl / tk.nchunk;
+    const uint64_t rc = wl - lg * tk.nchunk;
+    const uint64_t l0 = lg * K;
+    const uint64_t r = rc * 64 + (uint64_t)lane;
+    const bool active = r < tk.nr;
+    const uint64_t rr = active ? r : 0;
+    const uint64_t nl = tk.nl;
+    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
+    const unsigned long long* __restrict__ R = d.lvl[tk.lvl_r];
+    uint32_t mm[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) mm[k] = 0;
+    const int T = d.T;
+    for (int t0 = 0; t0 < T; t0 += kTraceGroup) {
+        uint64_t bv[kTraceGroup];
+#pragma unroll
+        for (int j = 0; j < kTraceGroup; ++j)
+            if (t0 + j < T) bv[j] = R[(uint64_t)(t0 + j) * tk.nr + rr];
+#pragma unroll
+        for (int j = 0; j < kTraceGroup; ++j) {
+            const int t = t0 + j;
+            if (t >= T) break;
+            const uint64_t b = bv[j];
+            const uint64_t want = t < d.P ? 1ull : 0ull;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                // rows past the level repeat its last row (masked below)
+                const uint64_t l = l0 + k < nl ? l0 + k : nl - 1;
+                const uint64_t a = L[(uint64_t)t * nl + l];
+                mm[k] |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
+            }
+        }
+    }
+    uint64_t best = kEmpty;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!active || l0 + k >= nl) continue;
+        const uint64_t pair = (l0 + k) * tk.nr + r;
+        const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
+                                 tk.base[2] + 2 * pair + 1};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (((tk.part >> q) & 1u) && !((mm[k] >> q) & 1u) && ord[q] < best) best = ord[q];
+    }
+    if (__ballot(best != kEmpty) && best != kEmpty) atomicMin(&d.ctr->found, (unsigned long long)best);
+}
+
 // Lowest level-local ordinal among the candidates of wave wl of task tk
 // (W = candidates per wave): ~ X F G of rows x0..; & | of pairs p0.., whose
 // Until candidates are 2p and 2p + 1 of their block.
@@ -780,8 +833,8 @@ __device__ __forceinline__ uint64_t wave_min_ordinal(const Task& tk, uint64_t wl
         for (int q = 0; q < 4; ++q)
             if (((tk.part >> q) & 1u) && tk.base[q] + x0 < m) m = tk.base[q] + x0;
     } else {
-        const uint64_t l = wl / tk.nchunk;
-        const uint64_t p0 = l * tk.nr + (wl - l * tk.nchunk) * W;
+        const uint64_t lg = wl / tk.nchunk;
+        const uint64_t p0 = lg * (tk.lrows > 1 ? tk.lrows : 1) * tk.nr + (wl - lg * tk.nchunk) * W;
         if ((tk.part & 1u) && tk.base[0] + p0 < m) m = tk.base[0] + p0;
         if ((tk.part & 2u) && tk.base[1] + p0 < m) m = tk.base[1] + p0;
         if ((tk.part & 0xcu) && tk.base[2] + 2 * p0 < m) m = tk.base[2] + 2 * p0;
@@ -810,7 +863,9 @@ __global__ __launch_bounds__(256) void k_check(const InstDesc* __restrict__ D, c
         const uint64_t wl = w - tk.wave_begin;
         const uint64_t found = __hip_atomic_load(&d.ctr->found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (found <= wave_min_ordinal(tk, wl, 64 / G)) continue;
-        if (d.mw) {
+        if (G == 1 && tk.kind == 1 && tk.lrows == kCheckRows) {
+            check_binary_rows<kCheckRows>(d, tk, wl, lane);
+        } else if (d.mw) {
             if (tk.kind == 0) expand_unary_mw<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
             else expand_binary_mw<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
         } else {
