@@ -36,15 +36,19 @@ __device__ __forceinline__ uint64_t suffix_or(uint64_t x) {
     return x;
 }
 
+// l U r (the reference doubles the carry over shifts, LTL_BS.cu:222-233).
+// Here one 64-bit add: with the words bit-reversed the future of a step is
+// the lower bits, and "r holds, or l holds and l U r holds one step later"
+// is the carry chain of an adder with generate g = rev(r) and propagate
+// rev(l): the carries of g + (g | rev(l)), taken out of each position, are
+// rev(l U r).  Steps past a trace are 0 in both words, so they stay 0.
 __device__ __forceinline__ uint64_t until_op(uint64_t l, uint64_t r) {
-    // bit-parallel carry doubling (LTL_BS.cu:222-233)
-    r |= l & (r >> 1);  l &= l >> 1;
-    r |= l & (r >> 2);  l &= l >> 2;
-    r |= l & (r >> 4);  l &= l >> 4;
-    r |= l & (r >> 8);  l &= l >> 8;
-    r |= l & (r >> 16); l &= l >> 16;
-    r |= l & (r >> 32);
-    return r;
+    const uint64_t g = __builtin_bitreverse64(r);
+    const uint64_t b = g | __builtin_bitreverse64(l);
+    const uint64_t s = g + b;
+    const uint64_t carry_out = s < g ? 1ull : 0ull;   // out of the top bit
+    const uint64_t c = s ^ g ^ b;                      // carries into each bit
+    return __builtin_bitreverse64((c >> 1) | (carry_out << 63));
 }
 
 __device__ __forceinline__ uint64_t apply_op(int op, uint64_t a, uint64_t b, uint64_t m) {
