@@ -23,6 +23,41 @@
 
 namespace vfb {
 
+// Device memory reached through descriptors is global memory.  These casts
+// let the compiler emit global loads, stores and atomics (a scalar base and a
+// 32-bit offset) instead of flat ones, which need 64-bit addresses in vector
+// registers and also count against the scalar-load counter, so that waiting
+// on a scalar load waited on them too.
+#define GLB __attribute__((address_space(1)))
+typedef GLB unsigned long long gu64;
+typedef GLB unsigned int gu32;
+template <class T>
+__device__ __forceinline__ GLB T* glob(T* p) {
+    return (GLB T*)p;
+}
+template <class T>
+__device__ __forceinline__ const GLB T* glob(const T* p) {
+    return (const GLB T*)p;
+}
+
+// relaxed atomics at agent scope (those of atomicCAS, atomicMin, atomicAdd
+// and atomicOr), on global memory
+__device__ __forceinline__ uint64_t cas64(gu64* p, uint64_t expect, uint64_t desired) {
+    unsigned long long e = expect;
+    __hip_atomic_compare_exchange_strong(p, &e, (unsigned long long)desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return e;   // the value found (= expect when the exchange happened)
+}
+__device__ __forceinline__ uint64_t min64(gu64* p, uint64_t v) {
+    return __hip_atomic_fetch_min(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t add64(gu64* p, uint64_t v) {
+    return __hip_atomic_fetch_add(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void or32(gu32* p, uint32_t v) {
+    (void)__hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ uint64_t dmix64(uint64_t x) {
     // SplitMix64 finaliser (LTL_BS.cu:80-83)
     x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -125,13 +160,13 @@ __device__ __forceinline__ uint64_t slot_hash(uint64_t hi, uint64_t lo) {
     return dmix64(hi ^ dmix64(lo + 0x9e3779b97f4a7c15ull));
 }
 
-__device__ __forceinline__ uint64_t ld_any(const unsigned long long* p) {
+__device__ __forceinline__ uint64_t ld_any(const gu64* p) {
     // may return a stale value; only EMPTY can be stale because every field
     // of a slot is written once (hi, lo) or only decreases (ord)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
-__device__ __forceinline__ uint64_t ld_dev(const unsigned long long* p) {
+__device__ __forceinline__ uint64_t ld_dev(const gu64* p) {
     // device-coherent read of a slot's ordinal: a candidate that sees a smaller
     // ordinal of this epoch skips its atomicMin (hot keys stay uncontended)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -146,24 +181,24 @@ enum { PR_DUP = 0, PR_FRESH = 1, PR_FAIL = 2 };
 // The first slot's fields (th, tl, to) are passed in, loaded ahead by the
 // caller.  PR_FRESH only for a candidate that may still hold the key's lowest
 // ordinal: one that sees a smaller ordinal of this epoch has certainly lost.
-__device__ __forceinline__ int probe_from(Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo, uint64_t gord,
+__device__ __forceinline__ int probe_from(GLB Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo, uint64_t gord,
                                           uint64_t epoch_base, uint64_t i, uint64_t th, uint64_t tl,
                                           uint64_t to, uint32_t* slot_out, int* claimed) {
     for (int n = 0; n < kProbeLimit; ++n) {
-        Slot* s = tab + i;
+        GLB Slot* s = tab + i;
         if (th == kEmpty) {
-            th = atomicCAS(&s->hi, kEmpty, hi);
+            th = cas64(&s->hi, kEmpty, hi);
             if (th == kEmpty) { th = hi; *claimed += 1; }
         }
         if (th == hi) {
             if (tl == kEmpty) {
-                tl = atomicCAS(&s->lo, kEmpty, lo);
+                tl = cas64(&s->lo, kEmpty, lo);
                 if (tl == kEmpty) tl = lo;
             }
             if (tl == lo) {
                 *slot_out = (uint32_t)i;
                 if (to < gord) return PR_DUP;      // older key, or a smaller ordinal of this epoch
-                const uint64_t old = atomicMin(&s->ord, (unsigned long long)gord);
+                const uint64_t old = min64(&s->ord, gord);
                 return old < gord ? PR_DUP : PR_FRESH;
             }
         }
@@ -176,11 +211,11 @@ __device__ __forceinline__ int probe_from(Slot* tab, uint64_t mask, uint64_t hi,
     return PR_FAIL;
 }
 
-__device__ __forceinline__ int probe_insert(Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo,
+__device__ __forceinline__ int probe_insert(GLB Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo,
                                             uint64_t gord, uint64_t epoch_base,
                                             uint32_t* slot_out, int* claimed) {
     const uint64_t i = slot_hash(hi, lo) & mask;
-    Slot* s = tab + i;
+    GLB Slot* s = tab + i;
     return probe_from(tab, mask, hi, lo, gord, epoch_base, i, ld_any(&s->hi), ld_any(&s->lo), ld_any(&s->ord),
                       slot_out, claimed);
 }
@@ -209,7 +244,7 @@ __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t* tot
 // per wavefront.  Per-candidate semantics are those of probe_from().
 __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* key, const uint64_t* ord,
                                                   uint32_t mismatch, uint32_t enable, int lane) {
-    Slot* tab = d.table;
+    GLB Slot* tab = glob(d.table);
     const uint64_t mask = d.table_mask;
     uint64_t idx[4], th[4], tl[4], to[4];
     uint32_t act = enable, fresh = 0, claimed = 0;
@@ -227,7 +262,7 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
         uint64_t r[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (((act >> q) & 1u) && th[q] == kEmpty) r[q] = atomicCAS(&tab[idx[q]].hi, kEmpty, key[q].hi);
+            if (((act >> q) & 1u) && th[q] == kEmpty) r[q] = cas64(&tab[idx[q]].hi, kEmpty, key[q].hi);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if (((act >> q) & 1u) && th[q] == kEmpty) {
@@ -236,7 +271,7 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if (((act >> q) & 1u) && th[q] == key[q].hi && tl[q] == kEmpty)
-                r[q] = atomicCAS(&tab[idx[q]].lo, kEmpty, key[q].lo);
+                r[q] = cas64(&tab[idx[q]].lo, kEmpty, key[q].lo);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if (((act >> q) & 1u) && th[q] == key[q].hi && tl[q] == kEmpty) tl[q] = r[q] == kEmpty ? key[q].lo : r[q];
@@ -251,7 +286,7 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if ((need_min >> q) & 1u) r[q] = atomicMin(&tab[idx[q]].ord, (unsigned long long)(d.epoch_base + ord[q]));
+            if ((need_min >> q) & 1u) r[q] = min64(&tab[idx[q]].ord, d.epoch_base + ord[q]);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if ((need_min >> q) & 1u) {
@@ -275,12 +310,12 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
     if (ftot | ctot | (fail ? 1u : 0u)) {
         unsigned long long base = 0;
         if (lane == 0) {
-            if (ftot) base = atomicAdd(&d.ctr->fresh_count, (unsigned long long)ftot);
+            if (ftot) base = add64(glob(&d.ctr->fresh_count), ftot);
             if (ctot) {
-                const unsigned long long before = atomicAdd(&d.ctr->claims, (unsigned long long)ctot);
-                if (before + ctot > d.claim_limit) atomicOr(&d.ctr->overflow, 4u);
+                const unsigned long long before = add64(glob(&d.ctr->claims), ctot);
+                if (before + ctot > d.claim_limit) or32(glob(&d.ctr->overflow), 4u);
             }
-            if (fail) atomicOr(&d.ctr->overflow, 1u);
+            if (fail) or32(glob(&d.ctr->overflow), 1u);
         }
         base = bcast64(base, 0) + fbase;
         uint32_t k = 0;
@@ -290,10 +325,10 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
                 const uint64_t e = base + k++;
                 if (e < d.fresh_cap) {
                     const uint64_t flag = ((mismatch >> q) & 1u) ? 0ull : kMatchBit;
-                    d.fresh_ord[e] = (d.epoch_base + ord[q]) | flag;
-                    d.fresh_slot[e] = (uint32_t)idx[q];
+                    glob(d.fresh_ord)[e] = (d.epoch_base + ord[q]) | flag;
+                    glob(d.fresh_slot)[e] = (uint32_t)idx[q];
                 } else {
-                    atomicOr(&d.ctr->overflow, 2u);
+                    or32(glob(&d.ctr->overflow), 2u);
                 }
             }
         }
@@ -308,7 +343,7 @@ __device__ __forceinline__ void check_candidates(const InstDesc& d, const uint64
 #pragma unroll
     for (int q = 0; q < 4; ++q)
         if (((enable >> q) & 1u) && !((mismatch >> q) & 1u) && ord[q] < best) best = ord[q];
-    if (__ballot(best != kEmpty) && best != kEmpty) atomicMin(&d.ctr->found, (unsigned long long)best);
+    if (__ballot(best != kEmpty) && best != kEmpty) min64(glob(&d.ctr->found), best);
 }
 
 // One candidate per lane (small-level mode): probe, then the fresh-list
@@ -318,7 +353,7 @@ __device__ __forceinline__ void finish_one(const InstDesc& d, const Key& key, ui
     uint32_t slot = 0xffffffffu;
     int claimed = 0;
     int r = PR_DUP;
-    if (en) r = probe_insert(d.table, d.table_mask, key.hi, key.lo, d.epoch_base + ord, d.epoch_base, &slot, &claimed);
+    if (en) r = probe_insert(glob(d.table), d.table_mask, key.hi, key.lo, d.epoch_base + ord, d.epoch_base, &slot, &claimed);
     const bool fresh = r == PR_FRESH;
     uint32_t ftot = 0, ctot = 0;
     const uint32_t fpos = wave_excl_scan_u32(fresh ? 1u : 0u, &ftot);
@@ -327,20 +362,20 @@ __device__ __forceinline__ void finish_one(const InstDesc& d, const Key& key, ui
     if (ftot | ctot | (fail ? 1u : 0u)) {
         unsigned long long base = 0;
         if (lane == 0) {
-            if (ftot) base = atomicAdd(&d.ctr->fresh_count, (unsigned long long)ftot);
+            if (ftot) base = add64(glob(&d.ctr->fresh_count), ftot);
             if (ctot) {
-                const unsigned long long before = atomicAdd(&d.ctr->claims, (unsigned long long)ctot);
-                if (before + ctot > d.claim_limit) atomicOr(&d.ctr->overflow, 4u);
+                const unsigned long long before = add64(glob(&d.ctr->claims), ctot);
+                if (before + ctot > d.claim_limit) or32(glob(&d.ctr->overflow), 4u);
             }
-            if (fail) atomicOr(&d.ctr->overflow, 1u);
+            if (fail) or32(glob(&d.ctr->overflow), 1u);
         }
         base = bcast64(base, 0) + fpos;
         if (fresh) {
             if (base < d.fresh_cap) {
-                d.fresh_ord[base] = (d.epoch_base + ord) | (mismatch ? 0ull : kMatchBit);
-                d.fresh_slot[base] = slot;
+                glob(d.fresh_ord)[base] = (d.epoch_base + ord) | (mismatch ? 0ull : kMatchBit);
+                glob(d.fresh_slot)[base] = slot;
             } else {
-                atomicOr(&d.ctr->overflow, 2u);
+                or32(glob(&d.ctr->overflow), 2u);
             }
         }
     }
@@ -410,7 +445,7 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
     const uint64_t x = wl * W + (uint64_t)(lane / G);
     const bool active = x < tk.nl;
     const uint64_t xr = active ? x : 0;
-    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
+    const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
     Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     uint32_t mm = 0;
     const int T = d.T;
@@ -472,8 +507,8 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     const uint64_t r = rc * W + (uint64_t)(lane / G);
     const bool active = r < tk.nr;
     const uint64_t rr = active ? r : 0;
-    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
-    const unsigned long long* __restrict__ R = d.lvl[tk.lvl_r];
+    const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
+    const gu64* __restrict__ R = glob(d.lvl[tk.lvl_r]);
     Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     uint32_t mm = 0;
     const int T = d.T;
@@ -619,7 +654,7 @@ __device__ __forceinline__ void expand_unary_mw(const InstDesc& d, const Task& t
     const uint64_t x = wl * W + (uint64_t)(lane / G);
     const bool active = x < tk.nl;
     const uint64_t xr = active ? x : 0;
-    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
+    const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
     Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     uint32_t mm = 0;
     for (int t = sub; t < d.T; t += G) {
@@ -666,8 +701,8 @@ __device__ __forceinline__ void expand_binary_mw(const InstDesc& d, const Task& 
     const uint64_t r = rc * W + (uint64_t)(lane / G);
     const bool active = r < tk.nr;
     const uint64_t rr = active ? r : 0;
-    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
-    const unsigned long long* __restrict__ R = d.lvl[tk.lvl_r];
+    const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
+    const gu64* __restrict__ R = glob(d.lvl[tk.lvl_r]);
     Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     uint32_t mm = 0;
     constexpr bool kU = CHECK || (SEL & 0xc) != 0;
@@ -786,8 +821,8 @@ __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task&
     const bool active = r < tk.nr;
     const uint64_t rr = active ? r : 0;
     const uint64_t nl = tk.nl;
-    const unsigned long long* __restrict__ L = d.lvl[tk.lvl_l];
-    const unsigned long long* __restrict__ R = d.lvl[tk.lvl_r];
+    const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
+    const gu64* __restrict__ R = glob(d.lvl[tk.lvl_r]);
     uint32_t mm[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) mm[k] = 0;
@@ -823,7 +858,7 @@ __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task&
         for (int q = 0; q < 4; ++q)
             if (((tk.part >> q) & 1u) && !((mm[k] >> q) & 1u) && ord[q] < best) best = ord[q];
     }
-    if (__ballot(best != kEmpty) && best != kEmpty) atomicMin(&d.ctr->found, (unsigned long long)best);
+    if (__ballot(best != kEmpty) && best != kEmpty) min64(glob(&d.ctr->found), best);
 }
 
 // Lowest level-local ordinal among the candidates of wave wl of task tk
@@ -865,7 +900,7 @@ __global__ __launch_bounds__(256) void k_check(const InstDesc* __restrict__ D, c
         const Task& tk = tasks[lo];
         const InstDesc& d = D[tk.inst];
         const uint64_t wl = w - tk.wave_begin;
-        const uint64_t found = __hip_atomic_load(&d.ctr->found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t found = __hip_atomic_load(glob(&d.ctr->found), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (found <= wave_min_ordinal(tk, wl, 64 / G)) continue;
         if (G == 1 && tk.kind == 1 && tk.lrows == kCheckRows) {
             check_binary_rows<kCheckRows>(d, tk, wl, lane);
@@ -894,7 +929,7 @@ __device__ __forceinline__ int seg_of(const S* segs, int n, uint64_t g) {
 // levels deduplicate against them (hashSetsInitialisation, LTL_BS.cu:136-160).
 template <int KEY>
 __device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned int* fail) {
-    const unsigned long long* L = d.lvl[1];
+    const gu64* L = glob(d.lvl[1]);
     Key k{0, 0};
     if (d.mw) {
         for (int c = 0; c < d.C; ++c) key_acc_col<KEY>(k, L[(uint64_t)c * n + x], d.cinfo[c], d);
@@ -914,7 +949,7 @@ __device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned i
     key_fin(k);
     uint32_t slot;
     int claimed = 0;
-    if (probe_insert(d.table, d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed) == PR_FAIL) atomicOr(fail, 1u);
+    if (probe_insert(glob(d.table), d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed) == PR_FAIL) or32(glob(fail), 1u);
 }
 
 // BS stage: bit 0 of every trace of each stored row, packed into one word
@@ -936,7 +971,7 @@ __global__ void k_insert_keys(Slot* tab, uint64_t mask, const unsigned long long
     if (i >= n) return;
     uint32_t slot;
     int claimed = 0;
-    if (probe_insert(tab, mask, 0ull, keys[i], i, 0, &slot, &claimed) == PR_FAIL) atomicOr(fail, 1u);
+    if (probe_insert(glob(tab), mask, 0ull, keys[i], i, 0, &slot, &claimed) == PR_FAIL) or32(glob(fail), 1u);
 }
 
 // the atoms of many instances in one launch (segment k = instance k's atoms)
@@ -966,26 +1001,26 @@ __global__ void k_insert_atoms(const InstDesc* __restrict__ D, uint32_t inst, ui
 // The keys are distinct, so a key takes the first slot whose hi it claims
 // from EMPTY, alone (one CAS); lo and ord are plain stores, read only by
 // later kernels.  Lookups stay valid: they skip slots whose (hi, lo) differ.
-__device__ __forceinline__ void rehash_slot(const Slot& s, Slot* tab, uint64_t mask, uint64_t keep_below,
+__device__ __forceinline__ void rehash_slot(const Slot& s, GLB Slot* tab, uint64_t mask, uint64_t keep_below,
                                             unsigned int* fail) {
     if (s.hi == kEmpty || s.lo == kEmpty || s.ord >= keep_below) return;
     uint64_t j = slot_hash(s.hi, s.lo) & mask;
     for (int n = 0; n < kProbeLimit; ++n, j = (j + 1) & mask) {
         if (ld_any(&tab[j].hi) != kEmpty) continue;
-        if (atomicCAS(&tab[j].hi, kEmpty, s.hi) == kEmpty) {
+        if (cas64(&tab[j].hi, kEmpty, s.hi) == kEmpty) {
             tab[j].lo = s.lo;
             tab[j].ord = s.ord;
             return;
         }
     }
-    atomicOr(fail, 1u);
+    or32(glob(fail), 1u);
 }
 
 __global__ void k_rehash(const Slot* __restrict__ old, uint64_t old_cap, Slot* tab, uint64_t mask,
                          uint64_t keep_below, unsigned int* fail) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < old_cap; i += stride)
-        rehash_slot(old[i], tab, mask, keep_below, fail);
+        rehash_slot(old[i], glob(tab), mask, keep_below, fail);
 }
 
 // the rehashes of many instances' tables in one launch
@@ -993,7 +1028,8 @@ __global__ void k_rehash_multi(const RehashSeg* __restrict__ segs, int n, uint64
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
         const RehashSeg& sg = segs[seg_of(segs, n, g)];
-        rehash_slot(sg.old[g - sg.begin], sg.tab, sg.mask, sg.keep_below, fail);
+        const GLB Slot& os = glob(sg.old)[g - sg.begin];
+        rehash_slot(Slot{os.hi, os.lo, os.ord, 0}, glob(sg.tab), sg.mask, sg.keep_below, fail);
     }
 }
 
@@ -1003,7 +1039,7 @@ __global__ void k_fill_multi(const OpSeg* __restrict__ segs, int n, uint64_t tot
     for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
         const OpSeg& sg = segs[seg_of(segs, n, g)];
         const uint64_t i = g - sg.begin;
-        sg.dst[i] = sg.src ? sg.src[i] : sg.value;
+        glob(sg.dst)[i] = sg.src ? glob(sg.src)[i] : sg.value;
     }
 }
 
@@ -1042,17 +1078,17 @@ __global__ void k_fresh_prefix(const InstDesc* __restrict__ D, Seg* segs, int n,
 // lowest ordinal.  The last pass of a level marks it in the byte map; every
 // pass reports a matching winner below found_below.
 __device__ __forceinline__ void resolve_entry(const InstDesc& d, uint64_t e) {
-    const uint64_t of = d.fresh_ord[e];
+    const uint64_t of = glob(d.fresh_ord)[e];
     // a pass before the last only looks for a settled match
     if (!d.mark && !(of & kMatchBit)) return;
-    const uint32_t slot = d.fresh_slot[e];
+    const uint32_t slot = glob(d.fresh_slot)[e];
     const uint64_t o = of & ~kMatchBit;
-    if (d.table[slot].ord == o) {
+    if (glob(d.table)[slot].ord == o) {
         const uint64_t local = o - d.epoch_base;
         // each ordinal appears at most once in the fresh list: a plain byte
         // store, packed to bits (and cleared) by the count / scan kernels
-        if (d.mark) d.winbyte[local] = 1;
-        if ((of & kMatchBit) && local < d.found_below) atomicMin(&d.ctr->found, (unsigned long long)local);
+        if (d.mark) glob(d.winbyte)[local] = 1;
+        if ((of & kMatchBit) && local < d.found_below) min64(glob(&d.ctr->found), local);
     }
 }
 
@@ -1085,19 +1121,19 @@ __device__ __forceinline__ uint64_t gather8(uint64_t x) {
 // returns the word's popcount
 __device__ __forceinline__ uint32_t pack_word(const InstDesc& d, uint64_t w) {
     if (w >= d.nwords) return 0u;
-    uint4* src = reinterpret_cast<uint4*>(d.winbyte + w * 64);
+    typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+    GLB u64x2* src = (GLB u64x2*)(d.winbyte + w * 64);
     uint64_t bits = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const uint4 v = src[q];
-        const uint64_t a = ((uint64_t)v.y << 32) | v.x, b = ((uint64_t)v.w << 32) | v.z;
-        bits |= gather8(a) << (16 * q);
-        bits |= gather8(b) << (16 * q + 8);
+        const u64x2 v = src[q];
+        bits |= gather8(v.x) << (16 * q);
+        bits |= gather8(v.y) << (16 * q + 8);
     }
     // the map starts every level cleared
 #pragma unroll
-    for (int q = 0; q < 4; ++q) src[q] = make_uint4(0u, 0u, 0u, 0u);
-    d.winbits[w] = bits;
+    for (int q = 0; q < 4; ++q) src[q] = u64x2{0ull, 0ull};
+    glob(d.winbits)[w] = bits;
     return (uint32_t)__popcll(bits);
 }
 
@@ -1115,7 +1151,7 @@ __global__ __launch_bounds__(256) void k_count(const InstDesc* __restrict__ D, c
         const uint64_t c = g - sg.begin;
         uint32_t v = pack_word(d, c * kChunkWords + lane);
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) d.blkcount[c] = v;
+        if (lane == 0) glob(d.blkcount)[c] = v;
     }
 }
 
@@ -1206,8 +1242,8 @@ __global__ __launch_bounds__(256) void k_resolve_scan1(const InstDesc* __restric
 // operands of the candidate with level-local ordinal o
 struct Cand {
     int op;
-    const unsigned long long* L;
-    const unsigned long long* R;
+    const gu64* L;
+    const gu64* R;
     uint64_t nl, nr, l, r;
     bool swap;
 };
@@ -1224,8 +1260,8 @@ __device__ __forceinline__ int unit_of(const InstDesc& d, uint64_t o) {
 __device__ __forceinline__ Cand decode_unit(const InstDesc& d, const Unit& u, uint64_t o) {
     Cand c;
     c.op = u.op;
-    c.L = d.lvl[u.lvl_l];
-    c.R = d.lvl[u.lvl_r];
+    c.L = glob(d.lvl[u.lvl_l]);
+    c.R = glob(d.lvl[u.lvl_r]);
     c.nl = u.nl;
     c.nr = u.nr;
     c.swap = false;
@@ -1258,7 +1294,7 @@ __device__ __forceinline__ uint64_t cand_word(const Cand& c, int t, uint64_t m) 
 }
 
 // trace t of candidate c over its words (multi-word rows), last word first
-__device__ __forceinline__ void store_trace_mw(const InstDesc& d, const Cand& cd, int t, unsigned long long* out,
+__device__ __forceinline__ void store_trace_mw(const InstDesc& d, const Cand& cd, int t, gu64* out,
                                                uint64_t out_n, uint64_t row) {
     const int c0 = d.tcol[t];
     const bool unary = cd.op == VFB_OP_NOT || cd.op == VFB_OP_NEXT || cd.op == VFB_OP_FINALLY ||
@@ -1310,8 +1346,8 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         const int si = nseg ? seg_find(segs, nseg, g) : 0;
         const InstDesc& d = D[nseg ? segs[si].inst : 0];
         const uint64_t item = g - (nseg ? segs[si].begin : 0);
-        unsigned long long* const out_rows = nseg ? d.out_rows : patch.out_rows;
-        unsigned long long* const out_par = nseg ? d.out_par : patch.out_par;
+        gu64* const out_rows = glob(nseg ? d.out_rows : patch.out_rows);
+        gu64* const out_par = glob(nseg ? d.out_par : patch.out_par);
         const uint64_t out_n = nseg ? d.out_n : patch.out_n;
         const uint32_t nsl = (uint32_t)(d.T + kStoreTraces - 1) / kStoreTraces;
         const uint64_t c = item / (nsl * SP);
@@ -1319,11 +1355,11 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         const uint32_t part = rem / nsl;
         const int t0 = (int)(rem - part * nsl) * kStoreTraces;
         const int t1 = t0 + kStoreTraces < d.T ? t0 + kStoreTraces : d.T;
-        const uint32_t n = d.blkcount[c];
-        const uint64_t rank0 = d.blkprefix[c];
+        const uint32_t n = glob(d.blkcount)[c];
+        const uint64_t rank0 = glob(d.blkprefix)[c];
         if (n == 0 || rank0 >= out_n) continue;
         const uint64_t w = c * kChunkWords + lane;
-        uint64_t bits = w < d.nwords ? d.winbits[w] : 0ull;
+        uint64_t bits = w < d.nwords ? glob(d.winbits)[w] : 0ull;
         const uint64_t pc = __popcll(bits);
         uint32_t pos = (uint32_t)(wave_incl_scan(pc) - pc);
         while (bits) {
