@@ -283,6 +283,12 @@ typedef struct {
     uint64_t table_min_slots;      /* hash sets start at least this large (rounded up to a power
                                       of two, at most 2^32; default 2^17 = 4 MB); a set is
                                       otherwise grown by rehash as its keys need */
+    uint32_t max_load_pct;         /* hash-set load (percent of slots) at which a set is grown,
+                                      50..90 (default 70) */
+    uint32_t store_traces;         /* traces per survivor-storage work item, 1..128 (default 8) */
+    uint32_t check_rows;           /* left rows per wavefront in target-check launches
+                                      (1, 4, 8 or 16; default 8) */
+    uint32_t pad_tuning;
 } vfb_tuning;
 
 void vfb_tuning_default(vfb_tuning* t);

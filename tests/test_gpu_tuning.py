@@ -24,6 +24,13 @@ SETTINGS = {
                                    resolve_blocks_per_cu=1, store_blocks_per_cu=1),
     "big_grids": dict(pass_min=1 << 16, expand_blocks_per_cu=64, resolve_blocks_per_cu=64,
                       store_blocks_per_cu=64),
+    # hash sets at half or nine tenths load, one or 64 traces per store item,
+    # target checks one or 16 left rows per wave
+    "low_load_thin_store": dict(pass_min=1, max_load_pct=50, store_traces=1, check_rows=16,
+                                wide_max_waves=0),
+    "high_load_wide_store": dict(pass_min=1, max_load_pct=90, store_traces=64, check_rows=4,
+                                 wide_max_waves=0, table_min_slots=1 << 20),
+    "check_one_row": dict(pass_min=1, check_rows=1, wide_max_waves=0),
 }
 
 FIELDS = ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "otf_level", "otf_stored")

@@ -653,6 +653,7 @@ void Engine::build_units(Inst& I) {
     h.epoch_base = I.gbase;
     h.found_below = ~0ull;
     h.mark = 1;
+    h.store_traces = (int)tun_.store_traces;
 }
 
 // Passes of a level.  P_FUSED: every operator in one pass.  P_PROBE: target
@@ -791,18 +792,19 @@ void Engine::prepare_scratch(Inst& I) {
     ensure_table(I, I.committed + est_new, false);
 }
 
-// (re)builds the hash set for want_keys keys at load <= 0.7; keeps only keys
+// (re)builds the hash set for want_keys keys at load <= max_load_pct; keeps only keys
 // of completed epochs (ord < epoch_base)
 void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     // at least kMinSlots (or the tuned start size): the small levels of a
     // solve then share one set, with no fill and rehash as it grows
     const uint64_t start = std::max<uint64_t>(kMinSlots, tun_.table_min_slots);
-    uint64_t need = next_pow2(std::max<uint64_t>(start, (uint64_t)(want_keys / 0.7) + 1));
+    const uint64_t pct = tun_.max_load_pct;
+    uint64_t need = next_pow2(std::max<uint64_t>(start, want_keys * 100 / pct + 1));
     // no headroom: plain, batched and served solves size their sets alike, so
     // their HBM-budget decisions (and hence their results) agree
     if (need > (1ull << 32)) need = 1ull << 32;
     if (I.table.p && need <= I.cap && !force_rebuild) {
-        I.h.claim_limit = I.cap * 7 / 10 > I.committed ? I.cap * 7 / 10 - I.committed : 0;
+        I.h.claim_limit = I.cap * pct / 100 > I.committed ? I.cap * pct / 100 - I.committed : 0;
         return;
     }
     if (force_rebuild && need <= I.cap) need = I.cap;
@@ -829,7 +831,7 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     I.cap = need;
     I.h.table = (Slot*)nt.p;
     I.h.table_mask = need - 1;
-    I.h.claim_limit = need * 7 / 10 > I.committed ? need * 7 / 10 - I.committed : 0;
+    I.h.claim_limit = need * pct / 100 > I.committed ? need * pct / 100 - I.committed : 0;
     I.res->table_slots = need;
 }
 
@@ -1016,7 +1018,7 @@ void Engine::epoch(std::vector<Inst*>& v) {
     for (size_t k = 0; k < n; ++k)
         for (Task t : v[k]->tasks) {
             if (!in_pass(*v[k], t)) continue;
-            t.lrows = (uint8_t)(all_check && !wide && t.kind == 1 && !v[k]->h.mw && !v[k]->h.bs_mode ? kCheckRows : 1);
+            t.lrows = (uint8_t)(all_check && !wide && t.kind == 1 && !v[k]->h.mw && !v[k]->h.bs_mode ? tun_.check_rows : 1);
             const uint64_t tw = task_waves(*v[k], t, per_wave);
             if (!tw) continue;
             t.inst = (uint32_t)k;
@@ -1143,7 +1145,7 @@ void Engine::store(std::vector<Inst*>& v) {
         Inst& I = *v[k];
         if (!I.h.out_n || !I.h.nblk) continue;
         segs.push_back(Seg{(uint32_t)k, 0, tot});
-        tot += I.h.nblk * parts * (uint64_t)((I.h.T + kStoreTraces - 1) / kStoreTraces);
+        tot += I.h.nblk * parts * (uint64_t)((I.h.T + I.h.store_traces - 1) / I.h.store_traces);
     }
     if (!tot) return;
     hipEvent_t ev = nullptr;
@@ -1362,7 +1364,7 @@ void Engine::step(std::vector<Inst*>& act) {
                             I.h.fresh_slot = (uint32_t*)I.fresh_slot.p;
                             I.h.fresh_cap = fcap;
                         }
-                        uint64_t keys = (uint64_t)(I.cap * 0.7);
+                        uint64_t keys = I.cap * tun_.max_load_pct / 100;
                         ensure_table(I, (I.ctr.overflow & 5) ? keys * 4 : keys, true);
                         // the level is built again from its first dedup pass
                         if (I.pass == P_REST) {
@@ -1835,6 +1837,18 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     t->store_blocks_per_cu = 8;
     t->park_min = 1ull << 24;
     t->table_min_slots = kMinSlots;
+    t->max_load_pct = 70;
+    t->store_traces = kStoreTraces;
+    t->check_rows = kCheckRows;
+    t->pad_tuning = 0;
+}
+
+// the ranges vfb_set_tuning accepts (vfb_tuning_load applies the same)
+bool vfb::tuning_valid(const vfb_tuning& t) {
+    auto ok = [](uint32_t b) { return b >= 1 && b <= 64; };
+    return ok(t.expand_blocks_per_cu) && ok(t.resolve_blocks_per_cu) && ok(t.store_blocks_per_cu) &&
+           t.table_min_slots <= (1ull << 32) && t.max_load_pct >= 50 && t.max_load_pct <= 90 && t.store_traces >= 1 &&
+           t.store_traces <= 128 && (t.check_rows == 1 || t.check_rows == 4 || t.check_rows == 8 || t.check_rows == 16);
 }
 
 extern "C" void vfb_get_tuning(vfb_tuning* t) {
@@ -1844,11 +1858,7 @@ extern "C" void vfb_get_tuning(vfb_tuning* t) {
 }
 
 extern "C" vfb_status vfb_set_tuning(const vfb_tuning* t) {
-    if (!t) return VFB_INVALID_INPUT;
-    auto ok = [](uint32_t b) { return b >= 1 && b <= 64; };
-    if (!ok(t->expand_blocks_per_cu) || !ok(t->resolve_blocks_per_cu) || !ok(t->store_blocks_per_cu) ||
-        t->table_min_slots > (1ull << 32))
-        return VFB_INVALID_INPUT;
+    if (!t || !tuning_valid(*t)) return VFB_INVALID_INPUT;
     std::lock_guard<std::mutex> lk(g_mu);
     g_tuning = *t;
     return VFB_OK;

@@ -297,6 +297,7 @@ class Engine {
 // or re-creates it with a larger arena when want_bytes exceeds the current one.
 std::mutex& engine_mutex();
 vfb_tuning current_tuning();   // caller holds engine_mutex()
+bool tuning_valid(const vfb_tuning& t);
 Engine& engine_acquire(size_t want_bytes);
 bool engine_pinned();   // true while a server holds requests in the engine
 

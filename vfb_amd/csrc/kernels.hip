@@ -902,8 +902,12 @@ __global__ __launch_bounds__(256) void k_check(const InstDesc* __restrict__ D, c
         const uint64_t wl = w - tk.wave_begin;
         const uint64_t found = __hip_atomic_load(glob(&d.ctr->found), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (found <= wave_min_ordinal(tk, wl, 64 / G)) continue;
-        if (G == 1 && tk.kind == 1 && tk.lrows == kCheckRows) {
-            check_binary_rows<kCheckRows>(d, tk, wl, lane);
+        if (G == 1 && tk.kind == 1 && tk.lrows == 8) {
+            check_binary_rows<8>(d, tk, wl, lane);
+        } else if (G == 1 && tk.kind == 1 && tk.lrows == 4) {
+            check_binary_rows<4>(d, tk, wl, lane);
+        } else if (G == 1 && tk.kind == 1 && tk.lrows == 16) {
+            check_binary_rows<16>(d, tk, wl, lane);
         } else if (d.mw) {
             if (tk.kind == 0) expand_unary_mw<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
             else expand_binary_mw<KEY_MIX, true, G, SEL_ALL>(d, tk, wl, lane);
@@ -1349,12 +1353,13 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
         gu64* const out_rows = glob(nseg ? d.out_rows : patch.out_rows);
         gu64* const out_par = glob(nseg ? d.out_par : patch.out_par);
         const uint64_t out_n = nseg ? d.out_n : patch.out_n;
-        const uint32_t nsl = (uint32_t)(d.T + kStoreTraces - 1) / kStoreTraces;
+        const int st = d.store_traces;
+        const uint32_t nsl = (uint32_t)((d.T + st - 1) / st);
         const uint64_t c = item / (nsl * SP);
         const uint32_t rem = (uint32_t)(item - c * nsl * SP);
         const uint32_t part = rem / nsl;
-        const int t0 = (int)(rem - part * nsl) * kStoreTraces;
-        const int t1 = t0 + kStoreTraces < d.T ? t0 + kStoreTraces : d.T;
+        const int t0 = (int)(rem - part * nsl) * st;
+        const int t1 = t0 + st < d.T ? t0 + st : d.T;
         const uint32_t n = glob(d.blkcount)[c];
         const uint64_t rank0 = glob(d.blkprefix)[c];
         if (n == 0 || rank0 >= out_n) continue;

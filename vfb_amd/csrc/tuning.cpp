@@ -8,6 +8,10 @@
 
 #include "../../include/vfb.h"
 
+namespace vfb {
+bool tuning_valid(const vfb_tuning& t);   // engine.cpp
+}
+
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -31,6 +35,9 @@ const Field kFields[] = {
     {"store_blocks_per_cu", false, offsetof(vfb_tuning, store_blocks_per_cu)},
     {"park_min", true, offsetof(vfb_tuning, park_min)},
     {"table_min_slots", true, offsetof(vfb_tuning, table_min_slots)},
+    {"max_load_pct", false, offsetof(vfb_tuning, max_load_pct)},
+    {"store_traces", false, offsetof(vfb_tuning, store_traces)},
+    {"check_rows", false, offsetof(vfb_tuning, check_rows)},
 };
 
 uint64_t get(const vfb_tuning& t, const Field& f) {
@@ -108,10 +115,7 @@ extern "C" vfb_status vfb_tuning_load(const char* path, vfb_tuning* t) {
             if (std::strcmp(key, fl.key) == 0) put(v, fl, val);
     }
     std::fclose(f);
-    auto ok = [](uint32_t b) { return b >= 1 && b <= 64; };
-    if (!ok(v.expand_blocks_per_cu) || !ok(v.resolve_blocks_per_cu) || !ok(v.store_blocks_per_cu) ||
-        v.table_min_slots > (1ull << 32))
-        return VFB_INVALID_INPUT;
+    if (!vfb::tuning_valid(v)) return VFB_INVALID_INPUT;
     *t = v;
     return VFB_OK;
 }
@@ -155,18 +159,27 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     static const uint64_t cand_resolve[] = {8, 16, 32};
     static const uint64_t cand_store[] = {4, 8, 16};
     static const uint64_t cand_park[] = {0, 1ull << 22, 1ull << 24, 1ull << 26};
+    static const uint64_t cand_tmin[] = {1ull << 17, 1ull << 20, 1ull << 22};
+    static const uint64_t cand_load[] = {50, 60, 70, 80, 90};
+    static const uint64_t cand_st[] = {4, 8, 16, 32, 64};
+    static const uint64_t cand_rows[] = {1, 4, 8, 16};
     struct Axis {
         const Field* f;
         const uint64_t* v;
         size_t nv;
     } axes[] = {{&kFields[0], cand_pass, 5}, {&kFields[1], cand_wide, 7}, {&kFields[2], cand_expand, 4},
-                {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}, {&kFields[5], cand_park, 4}};
+                {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}, {&kFields[5], cand_park, 4},
+                {&kFields[6], cand_tmin, 3},  {&kFields[7], cand_load, 5}, {&kFields[8], cand_st, 5},
+                {&kFields[9], cand_rows, 4}};
+    // every setting timed, for the cache file: "# tried <key> <value> <ms>"
+    std::string log = "# tried start - " + std::to_string(t_default) + "\n";
     for (const Axis& ax : axes) {
         for (size_t i = 0; i < ax.nv && err == VFB_OK; ++i) {
             if (get(cur, *ax.f) == ax.v[i]) continue;
             vfb_tuning t = cur;
             put(t, *ax.f, ax.v[i]);
             const double ms = time_setting(t);
+            log += std::string("# tried ") + ax.f->key + " " + std::to_string(ax.v[i]) + " " + std::to_string(ms) + "\n";
             if (ms < t_best * 0.98) {   // a clear win only: timings of equal settings jitter
                 t_best = ms;
                 cur = t;
@@ -180,6 +193,13 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     if (best) *best = cur;
     if (ms_default) *ms_default = t_default;
     if (ms_best) *ms_best = t_best;
-    if (cache_path) return vfb_tuning_save(&cur, cache_path);
+    if (cache_path) {
+        const vfb_status s = vfb_tuning_save(&cur, cache_path);
+        if (s != VFB_OK) return s;
+        FILE* f = std::fopen(cache_path, "a");
+        if (!f) return VFB_IO_ERROR;
+        std::fputs(log.c_str(), f);
+        return std::fclose(f) == 0 ? VFB_OK : VFB_IO_ERROR;
+    }
     return VFB_OK;
 }
