@@ -105,6 +105,23 @@ def generate(V, config, n):
     return out
 
 
+def autotune_axes(path, ms_default):
+    """Per-axis outcome of an autotuner run, from the '# tried <key> <value>
+    <ms>' lines it wrote: the best time over each axis's values (the search
+    is coordinate-wise, so each axis starts from the best setting so far)."""
+    axes = {}
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 5 and p[:2] == ["#", "tried"] and p[2] != "start":
+                ms = float(p[4])
+                if p[2] not in axes or ms < axes[p[2]]["best_ms"]:
+                    axes[p[2]] = {"best_value": int(p[3]), "best_ms": ms}
+    for a in axes.values():
+        a["gain_vs_default"] = ms_default / a["best_ms"] if a["best_ms"] else None
+    return axes
+
+
 def _same(a, b):
     return all(a[f] == b[f] for f in FIELDS)
 
@@ -118,8 +135,14 @@ def headline(V, steps, warmup, autotune, batch_only=False, tuning_file=None, sav
         V.set_tuning(**tuned["tuning"])
         log(f"tuning from {tuning_file}: {tuned['tuning']}")
     elif autotune:
-        log("autotuning on instances 0..47 (batched)")
-        tuned = V.autotune(insts[:48], max_size=MIX_MAX, batched=True, reps=2, cache=save_tuning)
+        # a sample of the mix: its first 48 instances and two whose largest
+        # levels reach 10^7 candidates (the large levels set the mix's pace)
+        sample = insts[:48] + [insts[219], insts[456]]
+        log("autotuning on instances 0..47, 219 and 456 of the mix (batched)")
+        cache = save_tuning or os.path.join(ROOT, "gpurun_out", "mix_tuning_autotune.txt")
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        tuned = V.autotune(sample, max_size=MIX_MAX, batched=True, reps=2, cache=cache)
+        tuned["per_axis"] = autotune_axes(cache, tuned["ms_default"])
         log(f"tuned {tuned}")
     for w in range(warmup):
         t0 = time.perf_counter()
