@@ -260,7 +260,11 @@ void vfb_server_destroy(vfb_server* srv);
 /* ---- tuning and the autotuner ------------------------------------------ */
 
 /* Launch and scheduling parameters of the process-wide engine.  They change
-   speed only: every result is identical under every valid setting.  No
+   speed only -- every result is identical under every valid setting -- except
+   table_min_slots and max_load_pct, which size the hash sets: like hbm_budget,
+   they take part in deciding whether a level's set and scratch fit the HBM
+   budget (a level that does not fit is checked only, the onTheFly round), so
+   a solve that reaches the budget can be cut at another level.  No
    reference counterpart (the reference hard-codes 1024-thread blocks and one
    launch per temp-buffer chunk, LTL_BS.cu:383-446, 1226-1493). */
 typedef struct {

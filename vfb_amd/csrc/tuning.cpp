@@ -135,6 +135,8 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
         return st;
     }
     vfb_status err = VFB_OK;
+    const char* axis_key = "start";
+    uint64_t axis_val = 0;
     auto time_setting = [&](const vfb_tuning& t) -> double {
         if (vfb_set_tuning(&t) != VFB_OK) return 1e300;
         double b = 1e300;
@@ -143,7 +145,8 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
             if (st == VFB_DEVICE_ERROR) { err = st; break; }
             for (size_t k = 0; k < n; ++k)
                 if (!same_result(ref[k], got[k])) {
-                    std::fprintf(stderr, "vfb_autotune: instance %zu differs under a tuning setting\n", k);
+                    std::fprintf(stderr, "vfb_autotune: instance %zu differs under %s = %llu\n", k, axis_key,
+                                 (unsigned long long)axis_val);
                     err = VFB_DEVICE_ERROR;
                     break;
                 }
@@ -159,8 +162,9 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     static const uint64_t cand_resolve[] = {8, 16, 32};
     static const uint64_t cand_store[] = {4, 8, 16};
     static const uint64_t cand_park[] = {0, 1ull << 22, 1ull << 24, 1ull << 26};
-    static const uint64_t cand_tmin[] = {1ull << 17, 1ull << 20, 1ull << 22};
-    static const uint64_t cand_load[] = {50, 60, 70, 80, 90};
+    // (table_min_slots and max_load_pct are not searched: they size the sets,
+    // and a level whose set and scratch do not fit the HBM budget is checked
+    // only, so they can change where a budget-limited solve is cut)
     static const uint64_t cand_st[] = {4, 8, 16, 32, 64};
     static const uint64_t cand_rows[] = {1, 4, 8, 16};
     struct Axis {
@@ -169,8 +173,7 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
         size_t nv;
     } axes[] = {{&kFields[0], cand_pass, 5}, {&kFields[1], cand_wide, 7}, {&kFields[2], cand_expand, 4},
                 {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}, {&kFields[5], cand_park, 4},
-                {&kFields[6], cand_tmin, 3},  {&kFields[7], cand_load, 5}, {&kFields[8], cand_st, 5},
-                {&kFields[9], cand_rows, 4}};
+                {&kFields[8], cand_st, 5},      {&kFields[9], cand_rows, 4}};
     // every setting timed, for the cache file: "# tried <key> <value> <ms>"
     std::string log = "# tried start - " + std::to_string(t_default) + "\n";
     for (const Axis& ax : axes) {
@@ -178,6 +181,8 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
             if (get(cur, *ax.f) == ax.v[i]) continue;
             vfb_tuning t = cur;
             put(t, *ax.f, ax.v[i]);
+            axis_key = ax.f->key;
+            axis_val = ax.v[i];
             const double ms = time_setting(t);
             log += std::string("# tried ") + ax.f->key + " " + std::to_string(ax.v[i]) + " " + std::to_string(ms) + "\n";
             if (ms < t_best * 0.98) {   // a clear win only: timings of equal settings jitter
