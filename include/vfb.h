@@ -289,7 +289,8 @@ typedef struct {
                                       otherwise grown by rehash as its keys need */
     uint32_t max_load_pct;         /* hash-set load (percent of slots) at which a set is grown,
                                       50..90 (default 70) */
-    uint32_t store_traces;         /* traces per survivor-storage work item, 1..128 (default 8) */
+    uint32_t store_traces;         /* traces per survivor-storage work item on levels of at least
+                                      2^22 candidates, 1..128 (default 64; smaller levels take 8) */
     uint32_t check_rows;           /* left rows per wavefront in target-check launches
                                       (1, 4, 8 or 16; default 8) */
     uint32_t pad_tuning;

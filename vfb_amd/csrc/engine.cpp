@@ -653,7 +653,9 @@ void Engine::build_units(Inst& I) {
     h.epoch_base = I.gbase;
     h.found_below = ~0ull;
     h.mark = 1;
-    h.store_traces = (int)tun_.store_traces;
+    // large levels: many items anyway, so each survivor is decoded once for
+    // more traces; smaller levels keep more, shorter items
+    h.store_traces = I.N >= kStoreLargeLevel ? (int)tun_.store_traces : kStoreTraces;
 }
 
 // Passes of a level.  P_FUSED: every operator in one pass.  P_PROBE: target
@@ -1832,13 +1834,13 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     if (!t) return;
     t->pass_min = 1ull << 22;
     t->wide_max_waves = kWideMaxWaves;
-    t->expand_blocks_per_cu = 8;
+    t->expand_blocks_per_cu = 32;
     t->resolve_blocks_per_cu = 16;
     t->store_blocks_per_cu = 8;
     t->park_min = 1ull << 24;
     t->table_min_slots = kMinSlots;
     t->max_load_pct = 70;
-    t->store_traces = kStoreTraces;
+    t->store_traces = 64;
     t->check_rows = kCheckRows;
     t->pad_tuning = 0;
 }

@@ -31,7 +31,8 @@ constexpr uint64_t kEmpty = ~0ull;
 constexpr uint64_t kMatchBit = 1ull << 63;
 constexpr int kChunkWords = 64;        // bitmap words per scan chunk (4096 ordinals)
 constexpr int kProbeLimit = 4096;
-constexpr int kStoreTraces = 8;       // traces per k_store work item
+constexpr int kStoreTraces = 8;       // traces per k_store work item (levels below kStoreLargeLevel)
+constexpr uint64_t kStoreLargeLevel = 1ull << 22;  // levels of this many candidates take vfb_tuning.store_traces
 constexpr int kStoreParts = 4;        // survivor subsets per chunk in k_store
 constexpr int kStorePartsSmall = 16;  // the same when every level of the launch has at most
 constexpr uint64_t kStoreSmallBlk = 16;  // this many chunks (more, shorter wavefronts)
