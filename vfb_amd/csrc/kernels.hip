@@ -505,7 +505,13 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     const uint64_t l = wl / tk.nchunk;
     const uint64_t rc = wl - l * tk.nchunk;
     const uint64_t r = rc * W + (uint64_t)(lane / G);
-    const bool active = r < tk.nr;
+    // both operands from one level: pair (r, l), r > l, repeats the CS of
+    // pair (l, r) at a higher ordinal (& | commute, and its l U r / r U l are
+    // (l, r)'s r U l / l U r), and (l, l) gives l itself, an older key; none
+    // of them can be a winner or the lowest match, so they are skipped
+    const bool diag = tk.lvl_l == tk.lvl_r;
+    if (diag && rc * W + W - 1 <= l) return;
+    const bool active = r < tk.nr && (!diag || r > l);
     const uint64_t rr = active ? r : 0;
     const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
     const gu64* __restrict__ R = glob(d.lvl[tk.lvl_r]);
@@ -699,7 +705,13 @@ __device__ __forceinline__ void expand_binary_mw(const InstDesc& d, const Task& 
     const uint64_t l = wl / tk.nchunk;
     const uint64_t rc = wl - l * tk.nchunk;
     const uint64_t r = rc * W + (uint64_t)(lane / G);
-    const bool active = r < tk.nr;
+    // both operands from one level: pair (r, l), r > l, repeats the CS of
+    // pair (l, r) at a higher ordinal (& | commute, and its l U r / r U l are
+    // (l, r)'s r U l / l U r), and (l, l) gives l itself, an older key; none
+    // of them can be a winner or the lowest match, so they are skipped
+    const bool diag = tk.lvl_l == tk.lvl_r;
+    if (diag && rc * W + W - 1 <= l) return;
+    const bool active = r < tk.nr && (!diag || r > l);
     const uint64_t rr = active ? r : 0;
     const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
     const gu64* __restrict__ R = glob(d.lvl[tk.lvl_r]);
@@ -818,6 +830,10 @@ __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task&
     const uint64_t rc = wl - lg * tk.nchunk;
     const uint64_t l0 = lg * K;
     const uint64_t r = rc * 64 + (uint64_t)lane;
+    // one operand level: pairs (l, r) with l >= r repeat lower ordinals (as
+    // in expand_binary) and are skipped
+    const bool diag = tk.lvl_l == tk.lvl_r;
+    if (diag && rc * 64 + 63 <= l0) return;
     const bool active = r < tk.nr;
     const uint64_t rr = active ? r : 0;
     const uint64_t nl = tk.nl;
@@ -850,7 +866,7 @@ __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task&
     uint64_t best = kEmpty;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        if (!active || l0 + k >= nl) continue;
+        if (!active || l0 + k >= nl || (diag && r <= l0 + k)) continue;
         const uint64_t pair = (l0 + k) * tk.nr + r;
         const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                                  tk.base[2] + 2 * pair + 1};
