@@ -10,6 +10,10 @@ tests/test_oracle.py).
   (LTL_BS.cu:1179-1214); results must not depend on capacity.
 - Config 2 (5 APs, 40 traces of 16-32 steps) instances 0..31 at
   max_formula_size 14, the dedup-stress workload.
+- Config-4 mix instances 219 and 456 at max_formula_size 16 (the batch's
+  max size): levels of 10^7-10^8 candidates with probes, found levels built
+  up to the probe's match only, target checks over 8 left rows per wavefront
+  and diagonal splits without their repeated pairs.
 
 Bit-exact: status, minimal size, formula text, found ordinal and the
 per-level distinct / candidate counts.
@@ -63,3 +67,16 @@ def test_config2_instances_max14():
             assert got[f] == want[f], (k, f, got[f], want[f])
         if got["status"] == V.OK:
             assert separates(got["formula"], *arrays)
+
+
+@pytest.mark.parametrize("k", [219, 456])
+def test_config4_large_instances_max16(k):
+    print(f"[scale] config-4 #{k} max 16", flush=True)
+    inst = V.Instance.generate(4, k)
+    want, arrays = _omp(inst, 16)
+    got = V.solve(inst, max_size=16)
+    for f in FIELDS:
+        assert got[f] == want[f], (f, got[f], want[f])
+    assert got["status"] == V.OK and separates(got["formula"], *arrays)
+    assert max(got["level_candidates"]) > 10_000_000
+    assert got["probe_candidates"] > 0      # the found level was probed first
