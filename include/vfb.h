@@ -184,6 +184,8 @@ typedef struct {
                               and started again from its atoms (0 for vfb_solve) */
     uint32_t parked;       /* vfb_solve_batch: 1 if it waited for the others at a level of
                               vfb_tuning.park_min candidates and then went on alone */
+    uint64_t chain_levels; /* levels built inside the persistent level chain
+                              (vfb_tuning.chain_blocks), with no host round trip */
 } vfb_result;
 
 void vfb_options_default(vfb_options* opt);
@@ -293,6 +295,16 @@ typedef struct {
                                       2^22 candidates, 1..128 (default 64; smaller levels take 8) */
     uint32_t check_rows;           /* left rows per wavefront in target-check launches
                                       (1, 4, 8 or 16; default 8) */
+    uint32_t chain_blocks;         /* persistent level chain: at most this many 256-thread
+                                      workgroups per instance (a batch gives each member
+                                      what the device holds at once divided among them,
+                                      at least 1); 0 = no chain, levels return to the host
+                                      one by one; 1..64 (default 64) */
+    uint32_t chain_cands;          /* a level is built inside the chain when it has at most
+                                      chain_cands candidates per workgroup of its instance
+                                      (and at most 2^18), its solve has unit costs and
+                                      one-word traces, and its hash set need not grow;
+                                      64..65536 (default 1024) */
     uint32_t pad_tuning;
 } vfb_tuning;
 
@@ -344,6 +356,13 @@ typedef struct {
     double ms_store;
     uint64_t store_launches;
     uint64_t bytes_store;        /* 8W + 24 per stored row */
+    double ms_chain;             /* the persistent level chain (k_chain) */
+    uint64_t chain_launches;
+    uint64_t chain_levels;       /* instance levels it built */
+    uint64_t bytes_chain;        /* (a*8W + 16) per candidate of those levels */
+    uint64_t chain_solo_launches; /* of chain_launches: teams of one workgroup because the
+                                    runtime refused the cooperative launch (another process
+                                    on the device) */
 } vfb_kernel_stats;
 
 /* Copies the counters into *out (if not NULL), then clears them if reset. */

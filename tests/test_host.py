@@ -154,7 +154,8 @@ def test_tuning_set_get_and_range_checks():
         assert (got["pass_min"], got["wide_max_waves"], got["expand_blocks_per_cu"]) == (1, 0, 3)
         for bad in ({"expand_blocks_per_cu": 0}, {"resolve_blocks_per_cu": 65}, {"store_blocks_per_cu": 0},
                     {"max_load_pct": 49}, {"max_load_pct": 91}, {"store_traces": 0}, {"store_traces": 129},
-                    {"check_rows": 2}, {"check_rows": 32}, {"table_min_slots": (1 << 32) + 1}):
+                    {"check_rows": 2}, {"check_rows": 32}, {"table_min_slots": (1 << 32) + 1},
+                    {"chain_blocks": 65}, {"chain_cands": 63}, {"chain_cands": 65537}):
             with pytest.raises(V.VfbError):
                 V.set_tuning(**bad)
             assert V.get_tuning() == got

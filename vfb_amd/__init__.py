@@ -51,7 +51,7 @@ class Result(C.Structure):
                 ("bytes_store", C.c_uint64), ("fresh_total", C.c_uint64),
                 ("otf_level", C.c_int), ("bs_stage", C.c_int), ("otf_stored", C.c_uint64),
                 ("ms_probe", C.c_double), ("probe_candidates", C.c_uint64),
-                ("restarts", C.c_uint32), ("parked", C.c_uint32)]
+                ("restarts", C.c_uint32), ("parked", C.c_uint32), ("chain_levels", C.c_uint64)]
 
 
 class Tuning(C.Structure):
@@ -59,7 +59,8 @@ class Tuning(C.Structure):
                 ("expand_blocks_per_cu", C.c_uint32), ("resolve_blocks_per_cu", C.c_uint32),
                 ("store_blocks_per_cu", C.c_uint32), ("park_min", C.c_uint64),
                 ("table_min_slots", C.c_uint64), ("max_load_pct", C.c_uint32),
-                ("store_traces", C.c_uint32), ("check_rows", C.c_uint32), ("pad_tuning", C.c_uint32)]
+                ("store_traces", C.c_uint32), ("check_rows", C.c_uint32), ("chain_blocks", C.c_uint32),
+                ("chain_cands", C.c_uint32), ("pad_tuning", C.c_uint32)]
 
 
 TUNING_FIELDS = [f for f, _ in Tuning._fields_]
@@ -69,7 +70,8 @@ class KernelStats(C.Structure):
     _fields_ = [("ms_expand", C.c_double), ("expand_launches", C.c_uint64), ("bytes_expand", C.c_uint64),
                 ("ms_probe", C.c_double), ("probe_launches", C.c_uint64), ("bytes_probe", C.c_uint64),
                 ("ms_resolve", C.c_double), ("ms_store", C.c_double), ("store_launches", C.c_uint64),
-                ("bytes_store", C.c_uint64)]
+                ("bytes_store", C.c_uint64), ("ms_chain", C.c_double), ("chain_launches", C.c_uint64),
+                ("chain_levels", C.c_uint64), ("bytes_chain", C.c_uint64), ("chain_solo_launches", C.c_uint64)]
 
 
 # every symbol declared in include/vfb.h
@@ -322,7 +324,7 @@ def _todict(r):
         "bytes_store": r.bytes_store, "fresh_total": r.fresh_total,
         "otf_level": r.otf_level, "otf_stored": r.otf_stored, "bs_stage": r.bs_stage,
         "ms_probe": r.ms_probe, "probe_candidates": r.probe_candidates,
-        "restarts": r.restarts, "parked": r.parked,
+        "restarts": r.restarts, "parked": r.parked, "chain_levels": r.chain_levels,
     }
 
 

@@ -49,6 +49,7 @@ Engine::Engine(size_t arena_bytes) {
     CK(hipHostMalloc((void**)&h_flag_, sizeof(unsigned int), hipHostMallocDefault));
     *h_flag_ = 0;
     tun_ = current_tuning();
+    chain_resident_ = chain_blocks_per_cu() * cus_;
 }
 
 // Host side of a fixed-memory engine: static (pageable) staging for uploads,
@@ -104,6 +105,8 @@ Engine::~Engine() {
     if (h_ctr_) (void)hipHostFree(h_ctr_);
     if (d_flag_) (void)hipFree(d_flag_);
     if (pin_) (void)hipHostFree(pin_);
+    if (d_chain_ws_) (void)hipFree(d_chain_ws_);
+    if (h_chain_out_) (void)hipHostFree(h_chain_out_);
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     for (auto& t : timed_) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     arena_.reset();
@@ -180,7 +183,8 @@ char* Engine::step_buf(size_t bytes) {
 
 void Engine::time_begin(hipEvent_t* a, int kind) {
     *a = nullptr;
-    if (!opt_.timing || (opt_.timing == 2 && kind != 0)) return;
+    // kinds: 0 expand, 1 resolve + scan, 2 store, 3 probe, 4 level chain
+    if (!opt_.timing || (opt_.timing == 2 && kind != 0 && kind != 4)) return;
     if (ev_pool_.empty()) {
         hipEvent_t e;
         CK(hipEventCreate(&e));
@@ -205,9 +209,13 @@ void Engine::collect_times() {
     for (auto& t : timed_) {
         float ms = 0.f;
         CK(hipEventElapsedTime(&ms, t.a, t.b));
-        (t.kind == 0 ? g_kstats.ms_expand : t.kind == 1 ? g_kstats.ms_resolve
-                                          : t.kind == 2 ? g_kstats.ms_store : g_kstats.ms_probe) += ms;
+        (t.kind == 0   ? g_kstats.ms_expand
+         : t.kind == 1 ? g_kstats.ms_resolve
+         : t.kind == 2 ? g_kstats.ms_store
+         : t.kind == 3 ? g_kstats.ms_probe
+                       : g_kstats.ms_chain) += ms;
         for (Inst* I : t.who) {
+            if (t.kind == 4) break;   // the chain has no per-result time
             double* acc = t.kind == 0   ? &I->res->ms_expand
                           : t.kind == 1 ? &I->res->ms_resolve
                           : t.kind == 2 ? &I->res->ms_store
@@ -289,6 +297,8 @@ void Engine::release_all(Inst& I) {
     dfree(I, I.blkcount);
     dfree(I, I.blkprefix);
     dfree(I, I.cols);
+    for (auto& b : I.slabs) dfree(I, b);
+    I.slabs.clear();
     I.fresh_cap = I.nwords_cap = I.nblk_cap = 0;
     I.cap = 0;
 }
@@ -758,27 +768,33 @@ void Engine::prepare_scratch(Inst& I) {
     const uint64_t N = I.N;
     const uint64_t nwords = (N + 63) / 64;
     const uint64_t nblk = (nwords + kChunkWords - 1) / kChunkWords;
+    // at least what the host would hold had it built the levels the level
+    // chain built (the floors), so that memory use does not depend on the chain
     uint64_t fcap = std::min<uint64_t>(
         N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
+    fcap = std::max(fcap, I.floor_fcap);
+    const uint64_t want_nw = std::max(nwords, I.floor_nwords);
+    const uint64_t want_nb = std::max(nblk, I.floor_nblk);
+    I.floor_fcap = I.floor_nwords = I.floor_nblk = 0;
     if (fcap > I.fresh_cap) {
         ensure(I, I.fresh_ord, fcap * 8);
         ensure(I, I.fresh_slot, fcap * 4);
         I.fresh_cap = fcap;
     }
-    if (nwords > I.nwords_cap) {
+    if (want_nw > I.nwords_cap) {
         // at least 64K candidates' worth, so that a solve's small levels
         // share one clearing fill
-        const uint64_t nw = std::max<uint64_t>(nwords, kMinWinWords);
+        const uint64_t nw = std::max<uint64_t>(want_nw, kMinWinWords);
         ensure(I, I.winbits, nw * 8);
         ensure(I, I.winbyte, nw * 64);
         // cleared once here; the count / scan kernels clear what they pack
         fill(I, I.winbyte.p, nw * 8, 0);
         I.nwords_cap = nw;
     }
-    if (nblk > I.nblk_cap) {
-        ensure(I, I.blkcount, std::max<uint64_t>(nblk, 1) * 4);
-        ensure(I, I.blkprefix, std::max<uint64_t>(nblk, 1) * 8);
-        I.nblk_cap = nblk;
+    if (want_nb > I.nblk_cap) {
+        ensure(I, I.blkcount, std::max<uint64_t>(want_nb, 1) * 4);
+        ensure(I, I.blkprefix, std::max<uint64_t>(want_nb, 1) * 8);
+        I.nblk_cap = want_nb;
     }
     h.fresh_ord = (unsigned long long*)I.fresh_ord.p;
     h.fresh_slot = (uint32_t*)I.fresh_slot.p;
@@ -1177,6 +1193,245 @@ void Engine::store(std::vector<Inst*>& v) {
     g_kstats.store_launches++;
 }
 
+// Scratch and set for a level the host builds (prepare_scratch); a level
+// without the memory to deduplicate it is checked only, and the search stops.
+void Engine::prepare_or_check(Inst& I, std::vector<Inst*>& step) {
+    if (!I.check_only) {
+        try {
+            prepare_scratch(I);
+        } catch (CapacityError& e) {
+            if (on_capacity(I, e)) return;
+            if (I.bs_mode) { finish(I, VFB_CAPACITY); return; }
+            // no memory to deduplicate this level: check it only, then stop
+            I.check_only = true;
+            build_units(I);
+        }
+    }
+    step.push_back(&I);
+}
+
+// Whether the level chain may build the next level (I.N candidates, units
+// already built): a deduplicating level of a unit-cost CS search over
+// one-word traces, below max_size, of at most n_max candidates, within the
+// row cap, not one the host would probe first (vfb_tuning.pass_min), and
+// one for which the host would not grow the hash set (prepare_scratch's
+// ensure_table, the same arithmetic as in k_chain).
+bool Engine::chain_admits(const Inst& I, uint64_t n_max) const {
+    const int s = I.level + 1;
+    if (I.bs_mode || !I.unit_cost || I.h.mw || I.check_only || s >= I.max_size || s == I.chain_skip || !I.table.p)
+        return false;
+    if (I.N == 0 || I.N > n_max) return false;
+    if (I.max_rows && I.N > (I.max_rows > I.stored ? I.max_rows - I.stored : 0)) return false;
+    if (tun_.pass_min && I.N >= tun_.pass_min && nsplit_of(I) >= 1) return false;   // probed first, on the host
+    const uint64_t est = std::min<uint64_t>(
+        I.N, std::max<uint64_t>(4096, (uint64_t)(I.N * std::min(1.0, I.win_ratio * 1.3))));
+    const uint64_t start = std::max<uint64_t>(kMinSlots, tun_.table_min_slots);
+    return start <= I.cap && (I.committed + est) * 100 / tun_.max_load_pct + 1 <= I.cap;
+}
+
+// One launch of the persistent level chain over instances v (k_chain in
+// chain.hip), then one stream sync, after which each instance takes over the
+// levels the chain built as if the host had built them.  Each instance gets a
+// slab for the rows of those levels (given back beyond what they use); one
+// whose budget or the arena has no room keeps building on the host, and is
+// left in v.
+void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
+    std::vector<Inst*> ch, rest;
+    for (auto* I : v) {
+        const uint64_t row_bytes = 8 * (uint64_t)I->h.C + 8;
+        const size_t slab = align256(n_max * 3 / 2 * row_bytes) + 256 * (size_t)kMaxS;
+        const uint64_t room = I->budget > I->used ? I->budget - I->used : 0;
+        // only where the budget leaves far more than the chain's levels
+        // take, so that no budget decision could have cut one of them
+        void* p = room >= 4 * (uint64_t)slab + (64ull << 20) ? arena_->alloc(slab) : nullptr;
+        if (!p) {
+            rest.push_back(I);
+            continue;
+        }
+        I->slabs.push_back(Buf{p, slab, false});
+        I->used += slab;
+        ch.push_back(I);
+    }
+    v.swap(rest);
+    if (ch.empty()) return;
+    const size_t n = ch.size();
+    // workspace: the winner maps of all instances first, then per instance
+    // its fresh list and chunk counts
+    const uint64_t nw = (n_max + 63) / 64, nb = (nw + kChunkWords - 1) / kChunkWords;
+    const size_t wb = align256(nw * 64), fo = align256(n_max * 8), fs = align256(n_max * 4);
+    const size_t bits = align256(nw * 8), bc = align256(nb * 4), bp = align256(nb * 8);
+    const size_t per = fo + fs + bits + bc + bp;
+    const size_t wb_all = n * wb, need = wb_all + n * per;
+    if (need > chain_ws_cap_) {
+        sync();
+        if (d_chain_ws_) CK(hipFree(d_chain_ws_));
+        d_chain_ws_ = nullptr;
+        chain_ws_cap_ = std::max(need, 2 * chain_ws_cap_);
+        CK(hipMalloc((void**)&d_chain_ws_, chain_ws_cap_));
+        chain_wb_clean_ = 0;
+    }
+    if (wb_all > chain_wb_clean_)
+        CK(launch_fill((unsigned long long*)d_chain_ws_, wb_all / 8, 0, cus_ * 4, st_));
+    if (n > h_chain_out_cap_) {
+        sync();
+        if (h_chain_out_) CK(hipHostFree(h_chain_out_));
+        h_chain_out_ = nullptr;
+        h_chain_out_cap_ = std::max<size_t>(n, 2 * h_chain_out_cap_);
+        CK(hipHostMalloc((void**)&h_chain_out_, h_chain_out_cap_ * sizeof(ChainOut), hipHostMallocDefault));
+    }
+    const size_t o_out = align256(n * sizeof(ChainIn));
+    const size_t bytes = o_out + align256(n * sizeof(ChainOut));
+    char* d = step_buf(bytes);
+    std::vector<char> host(bytes, 0);
+    ChainIn* hin = (ChainIn*)host.data();
+    ChainOut* hout = (ChainOut*)(host.data() + o_out);
+    for (size_t k = 0; k < n; ++k) {
+        const Inst& I = *ch[k];
+        ChainIn& c = hin[k];
+        c.d = I.h;
+        c.slab = (unsigned long long*)I.slabs.back().p;
+        c.slab_bytes = I.slabs.back().bytes;
+        char* w = d_chain_ws_ + wb_all + k * per;
+        c.winbyte = (unsigned char*)(d_chain_ws_ + k * wb);
+        c.fresh_ord = (unsigned long long*)w;
+        c.fresh_slot = (uint32_t*)(w + fo);
+        c.winbits = (unsigned long long*)(w + fo + fs);
+        c.blkcount = (uint32_t*)(w + fo + fs + bits);
+        c.blkprefix = (unsigned long long*)(w + fo + fs + bits + bc);
+        c.n_max = n_max;
+        c.rows_left = I.max_rows ? (I.max_rows > I.stored ? I.max_rows - I.stored : 0) : ~0ull;
+        c.committed = I.committed;
+        c.cap = I.cap;
+        c.gbase = I.gbase;
+        c.fresh_cap = std::max(I.fresh_cap, I.floor_fcap);
+        c.pass_min = tun_.pass_min;
+        c.win_ratio = I.win_ratio;
+        c.fresh_ratio = I.fresh_ratio;
+        c.pct = tun_.max_load_pct;
+        c.level = I.level;
+        c.last_level = I.max_size - 1;
+        c.neg_type = I.neg_type;
+        for (auto& x : c.ctr) x.found = kEmpty;
+        hout[k].exit = CH_ERROR;
+    }
+    h2d(d, host.data(), bytes);
+    epoch_single_ = nullptr;
+    hipEvent_t ev = nullptr;
+    time_begin(&ev, 4);
+    int G_used = G;
+    CK(launch_chain((ChainIn*)d, (ChainOut*)(d + o_out), (int)n, G, st_, &G_used));
+    time_end(ev, 4, ch);
+    g_kstats.chain_launches++;
+    if (G_used != G) g_kstats.chain_solo_launches++;
+    CK(hipMemcpyAsync(h_chain_out_, d + o_out, n * sizeof(ChainOut), hipMemcpyDeviceToHost, st_));
+    chain_wb_clean_ = 0;
+    sync();
+    for (size_t k = 0; k < n; ++k)
+        if (h_chain_out_[k].err || h_chain_out_[k].exit == CH_ERROR)
+            throw DeviceError("level chain: a team barrier timed out");
+    chain_wb_clean_ = wb_all;   // every level the chain began, it packed (and so cleared)
+    for (size_t k = 0; k < n; ++k) chain_adopt(*ch[k], h_chain_out_[k]);
+}
+
+// The levels one instance's team built (o.nlev, stored in its slab) become
+// its levels, with the counters, results and host state a host-built level
+// leaves; then its exit: the found formula, or a level for the host to build
+// again (its keys in the set dropped, as after an overflow there), or the
+// next level, which the host builds.
+void Engine::chain_adopt(Inst& I, const ChainOut& o) {
+    vfb_result& R = *I.res;
+    Buf& slab = I.slabs.back();
+    auto level_work = [&]() {   // the fused pass's algorithmic bytes (pass_work)
+        I.pass = P_FUSED;
+        uint64_t c = 0, b = 0;
+        pass_work(I, &c, &b);
+        R.bytes_expand += b;
+        g_kstats.bytes_chain += b;
+        R.chain_levels++;
+        g_kstats.chain_levels++;
+    };
+    uint64_t end = 0;
+    for (int j = 0; j < o.nlev; ++j) {
+        const int s = I.level + 1;
+        if (j) build_units(I);
+        if (I.N != o.N[j]) throw DeviceError("level chain: a level's candidates differ from the host's units");
+        // the scratch prepare_scratch would have grown to for this level
+        const uint64_t N = I.N;
+        const uint64_t fcap = std::min<uint64_t>(
+            N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
+        const uint64_t nwords = (N + 63) / 64;
+        I.floor_fcap = std::max(I.floor_fcap, fcap);
+        I.floor_nwords = std::max(I.floor_nwords, nwords);
+        I.floor_nblk = std::max<uint64_t>(I.floor_nblk, (nwords + kChunkWords - 1) / kChunkWords);
+        level_work();
+        const uint64_t W = o.W[j];
+        R.level_candidates[s] = N;
+        R.level_winners[s] = W;
+        R.fresh_total += o.fresh[j];
+        R.candidates[s] = N;
+        R.distinct[s] = W;
+        R.levels_done = s;
+        R.bytes_store += W * (8 * (uint64_t)I.h.C + 24);
+        const uint64_t row_bytes = 8 * (uint64_t)I.h.C + 8;
+        const uint64_t bytes = W ? std::max<uint64_t>(256, align256(W * row_bytes)) : 0;
+        I.nrows[s] = (uint32_t)W;
+        I.rows[s] = W ? Buf{(char*)slab.p + o.off[j], bytes, true} : Buf{};
+        if (W) end = o.off[j] + bytes;
+        const uint64_t Nd = std::max<uint64_t>(N, 1);
+        I.win_ratio = (double)W / Nd;
+        I.fresh_ratio = (double)o.fresh[j] / Nd;
+        I.committed += W;
+        I.stored += W;
+        I.gbase += N;
+        I.level = s;
+    }
+    // the slab keeps the blocks of the levels built
+    if (end) {
+        arena_->shrink(slab.p, end);
+        I.used -= slab.bytes - end;
+        slab.bytes = end;
+    } else {
+        dfree(I, slab);
+        I.slabs.pop_back();
+    }
+    const int s = I.level + 1;
+    if (o.exit == CH_BEFORE) {
+        // the host builds level s; a level the chain refused outright is
+        // not offered to it again
+        if (o.nlev == 0) I.chain_skip = s;
+        return;
+    }
+    if (o.nlev) build_units(I);
+    if (I.N != o.N_exit) throw DeviceError("level chain: a level's candidates differ from the host's units");
+    if (o.exit == CH_FOUND) {
+        level_work();
+        R.level_candidates[s] = I.N;
+        R.level_winners[s] = o.W_exit;
+        R.fresh_total += o.fresh_exit;
+        R.candidates[s] = o.found + 1;
+        R.distinct[s] = o.found_rank;
+        R.min_size = s;
+        R.found_ordinal = o.found;
+        R.bs_stage = 0;
+        std::string f = cand_string(I, s, o.found, &R.top_op);
+        R.formula = strdup(f.c_str());
+        finish(I, VFB_OK);
+        return;
+    }
+    // CH_REDO: the host builds level s again; its keys in the set are dropped
+    // by rebuilding the set from the completed levels' keys, grown as the
+    // host grows it after the same overflow
+    R.retries++;
+    I.chain_skip = s;
+    try {
+        const uint64_t keys = I.cap * tun_.max_load_pct / 100;
+        ensure_table(I, (o.overflow & 5) ? keys * 4 : 0, true);
+    } catch (CapacityError& e) {
+        if (on_capacity(I, e)) return;
+        I.next_check_only = true;   // cannot rebuild: the level is checked only, as on the host
+    }
+}
+
 void Engine::begin(const vfb_options& opt, bool batch) {
     opt_ = opt;
     batch_ = batch;
@@ -1298,10 +1553,11 @@ void Engine::start(std::vector<Inst*>& all, std::vector<Inst*>& out) {
 
 void Engine::step(std::vector<Inst*>& act) {
     {
-        // ---- 1. units, tasks, scratch ----
-        std::vector<Inst*> step;
+        // ---- 1. units, tasks, scratch; small levels to the level chain ----
+        std::vector<Inst*> step, pre;
         size_t running = 0;
         for (auto* I : act) running += I->done ? 0 : 1;
+        const bool chain_on = tun_.chain_blocks > 0 && !fixed_ && chain_resident_ > 0;
         for (auto* I : act) {
             if (I->done) continue;
             I->check_only = I->next_check_only;
@@ -1314,22 +1570,34 @@ void Engine::step(std::vector<Inst*>& act) {
                 parked_.push_back(I);
                 continue;
             }
-            if (!I->check_only) {
-                try {
-                    prepare_scratch(*I);
-                } catch (CapacityError& e) {
-                    if (on_capacity(*I, e)) continue;
-                    if (I->bs_mode) { finish(*I, VFB_CAPACITY); continue; }
-                    // no memory to deduplicate this level: check it only, then stop
-                    I->check_only = true;
-                    build_units(*I);
-                }
+            if (chain_on && chain_admits(*I, kChainMaxCands)) {
+                pre.push_back(I);
+                continue;
             }
-            step.push_back(I);
+            prepare_or_check(*I, step);
+        }
+        if (!pre.empty()) {
+            // a team of G workgroups per instance, as many as the device
+            // holds at once among them; levels of up to G * chain_cands
+            const int G = (int)std::min<uint64_t>(
+                tun_.chain_blocks, std::max<uint64_t>(1, (uint64_t)chain_resident_ / pre.size()));
+            const uint64_t n_max = std::min<uint64_t>(kChainMaxCands, (uint64_t)G * tun_.chain_cands);
+            std::vector<Inst*> ch;
+            for (auto* I : pre) {
+                if (chain_admits(*I, n_max)) ch.push_back(I);
+                else prepare_or_check(*I, step);
+            }
+            if (!ch.empty()) chain(ch, G, n_max);
+            // members the chain could not take (no room for their slab)
+            for (auto* I : ch) prepare_or_check(*I, step);
         }
         step.erase(std::remove_if(step.begin(), step.end(), [](Inst* I) { return I->done; }), step.end());
         if (step.empty()) {
-            act.clear();
+            // every member is done, parked, or went on inside the level chain
+            std::vector<Inst*> nxt;
+            for (auto* I : act)
+                if (!I->done && !I->parked) nxt.push_back(I);
+            act.swap(nxt);
             return;
         }
 
@@ -1842,6 +2110,8 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     t->max_load_pct = 70;
     t->store_traces = 64;
     t->check_rows = kCheckRows;
+    t->chain_blocks = kChainMaxTeam;
+    t->chain_cands = 1024;
     t->pad_tuning = 0;
 }
 
@@ -1850,7 +2120,8 @@ bool vfb::tuning_valid(const vfb_tuning& t) {
     auto ok = [](uint32_t b) { return b >= 1 && b <= 64; };
     return ok(t.expand_blocks_per_cu) && ok(t.resolve_blocks_per_cu) && ok(t.store_blocks_per_cu) &&
            t.table_min_slots <= (1ull << 32) && t.max_load_pct >= 50 && t.max_load_pct <= 90 && t.store_traces >= 1 &&
-           t.store_traces <= 128 && (t.check_rows == 1 || t.check_rows == 4 || t.check_rows == 8 || t.check_rows == 16);
+           t.store_traces <= 128 && (t.check_rows == 1 || t.check_rows == 4 || t.check_rows == 8 || t.check_rows == 16) &&
+           t.chain_blocks <= (uint32_t)kChainMaxTeam && t.chain_cands >= 64 && t.chain_cands <= 65536;
 }
 
 extern "C" void vfb_get_tuning(vfb_tuning* t) {

@@ -42,6 +42,12 @@ hipError_t launch_rehash_multi(const RehashSeg* segs, int n, uint64_t total, uns
 hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch, int small,
                         int grid, hipStream_t st);
 
+// persistent level chain (chain.hip): teams of G workgroups per instance
+// (G > 1: a cooperative launch, else teams of one); workgroups of k_chain
+// resident per CU
+hipError_t launch_chain(ChainIn* ins, ChainOut* outs, int nteams, int G, hipStream_t st, int* G_used);
+int chain_blocks_per_cu();
+
 struct DeviceError : std::runtime_error {
     explicit DeviceError(const std::string& s) : std::runtime_error(s) {}
 };
@@ -87,6 +93,16 @@ class Arena {
             }
         }
         return nullptr;
+    }
+    // gives the tail of allocation p beyond `bytes` back to the free list
+    void shrink(void* p, size_t bytes) {
+        const size_t off = (char*)p - (char*)base_;
+        auto it = used_.find(off);
+        if (it == used_.end() || bytes >= it->second) return;
+        const size_t tail = it->second - bytes;
+        it->second = bytes;
+        used_[off + bytes] = tail;
+        release((char*)base_ + off + bytes);
     }
     void release(void* p) {
         if (!p) return;
@@ -189,6 +205,11 @@ struct Inst {
     uint8_t first_usel = 0xf, first_bsel = 0xf;
     uint64_t m0 = 0;            // the probe's lowest match: the first pass builds ordinals 0..m0
     uint64_t evaluated = 0;     // candidates of this level evaluated so far
+    std::vector<Buf> slabs;     // level blocks built by the persistent chain (rows[] borrow from them)
+    int chain_skip = 0;         // a level the chain refused or must not redo: built by the host
+    // scratch sizes the host path would have reached over the chain's levels
+    // (prepare_scratch grows to at least these at the next host level)
+    uint64_t floor_fcap = 0, floor_nwords = 0, floor_nblk = 0;
 };
 
 class Engine {
@@ -251,7 +272,7 @@ class Engine {
     std::vector<hipEvent_t> ev_pool_;
     struct Timed {
         hipEvent_t a, b;
-        int kind;                 // 0 expand, 1 resolve, 2 store
+        int kind;                 // 0 expand, 1 resolve, 2 store, 3 probe, 4 level chain
         std::vector<Inst*> who;
     };
     std::vector<Timed> timed_;
@@ -262,6 +283,14 @@ class Engine {
     bool defer_alone_ = false;    // batch: an instance that cannot get arena memory is deferred
     bool park_ = false;           // batch: members reaching a level of tun_.park_min candidates wait
     std::vector<Inst*> parked_;   // those members, resumed one at a time after the rest
+    // persistent level chain: workspace of the launch's instances (fresh
+    // lists, winner maps, chunk counts; not part of any solve's budget),
+    // laid out winner maps first; the first chain_wb_clean_ bytes are zero
+    char* d_chain_ws_ = nullptr;
+    size_t chain_ws_cap_ = 0, chain_wb_clean_ = 0;
+    ChainOut* h_chain_out_ = nullptr;   // pinned read-back
+    size_t h_chain_out_cap_ = 0;
+    int chain_resident_ = 0;      // k_chain workgroups resident on the device at once
 
     void sync();
     void h2d(void* dst, const void* src, size_t bytes);
@@ -290,6 +319,10 @@ class Engine {
     void collect_times();
     void epoch(std::vector<Inst*>& v);
     void store(std::vector<Inst*>& v);
+    void prepare_or_check(Inst& I, std::vector<Inst*>& step);
+    bool chain_admits(const Inst& I, uint64_t n_max) const;
+    void chain(std::vector<Inst*>& v, int G, uint64_t n_max);
+    void chain_adopt(Inst& I, const ChainOut& o);
 };
 
 // The process-wide engine (its arena holds all device memory of every solve)

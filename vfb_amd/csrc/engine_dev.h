@@ -183,4 +183,60 @@ struct Seg {
     uint64_t begin;             // prefix in the flat space
 };
 
+// ---------------------------------------------------------------------------
+// Persistent level chain (k_chain): one launch builds, for each instance, the
+// small levels that follow its last completed one -- expansion, resolve, scan
+// and store, separated by barriers of the instance's team of workgroups --
+// and returns to the host at the first level that is too large, that holds a
+// match, or that the host must build itself.
+// ---------------------------------------------------------------------------
+enum ChainExit : int32_t {
+    CH_BEFORE = 0,   // stopped before level+nlev+1 (too large, max size, or its set must grow)
+    CH_FOUND = 1,    // level+nlev+1 holds the returned formula (counters in ChainOut)
+    CH_REDO = 2,     // level+nlev+1 must be built again by the host (overflow bits: 1 probe
+                     // limit, 2 the host's fresh list would overflow, 4 set load)
+    CH_ERROR = 3     // a team barrier timed out
+};
+constexpr int kChainThreads = 256;
+constexpr int kChainMaxTeam = 64;               // workgroups per instance, at most
+constexpr uint64_t kChainMaxCands = 1ull << 18; // a chain level fits one workgroup's scan (64 chunks)
+constexpr int kMaxTasks = 1 + kMaxS / 2;        // ~XFG task + one per split
+
+// per instance, uploaded for one launch
+struct ChainIn {
+    InstDesc d;                 // descriptor with lvl / nrows of levels 1..level
+    unsigned long long* slab;   // rows of the chain's levels, back to back (256-byte aligned blocks)
+    uint64_t slab_bytes;
+    unsigned long long* fresh_ord;   // workspace of n_max candidates (winbyte all zero)
+    uint32_t* fresh_slot;
+    unsigned char* winbyte;
+    unsigned long long* winbits;
+    uint32_t* blkcount;
+    unsigned long long* blkprefix;
+    uint64_t n_max;             // a level enters the chain only with at most this many candidates
+    uint64_t rows_left;         // rows the solve may still store (max_rows), ~0 for no cap
+    uint64_t committed;         // keys in the set
+    uint64_t cap;               // slots of the set
+    uint64_t gbase;             // global ordinal of the next level's first candidate
+    uint64_t fresh_cap;         // the instance's own fresh list (as the host sizes it level by level)
+    uint64_t pass_min;          // levels the host probes first (vfb_tuning.pass_min) are left to it
+    double win_ratio;           // survivors / candidates of the last completed level
+    double fresh_ratio;         // fresh entries / candidates of the last completed level
+    uint32_t pct;               // set load (percent) at which the host grows the set
+    int32_t level;              // last completed level
+    int32_t last_level;         // the chain builds levels up to this one
+    int32_t neg_type;
+    Ctr ctr[2];                 // counters of even / odd levels
+    unsigned long long bar;     // team barrier arrivals
+};
+
+// per instance, read back after the launch
+struct ChainOut {
+    int32_t exit, nlev;         // ChainExit; levels built and stored
+    uint32_t overflow, err;     // CH_REDO's overflow bits; 1 if a barrier timed out
+    uint64_t found, found_rank; // CH_FOUND: level-local ordinal, survivors ordered before it
+    uint64_t N_exit, W_exit, fresh_exit;   // candidates / survivors / fresh entries of the exit level
+    uint64_t N[kMaxS], W[kMaxS], fresh[kMaxS], off[kMaxS];   // chain level k is level + 1 + k
+};
+
 }  // namespace vfb

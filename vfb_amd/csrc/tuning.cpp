@@ -38,6 +38,8 @@ const Field kFields[] = {
     {"max_load_pct", false, offsetof(vfb_tuning, max_load_pct)},
     {"store_traces", false, offsetof(vfb_tuning, store_traces)},
     {"check_rows", false, offsetof(vfb_tuning, check_rows)},
+    {"chain_blocks", false, offsetof(vfb_tuning, chain_blocks)},
+    {"chain_cands", false, offsetof(vfb_tuning, chain_cands)},
 };
 
 uint64_t get(const vfb_tuning& t, const Field& f) {
