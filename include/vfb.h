@@ -299,7 +299,9 @@ typedef struct {
                                       workgroups per instance (a batch gives each member
                                       what the device holds at once divided among them,
                                       at least 1); 0 = no chain, levels return to the host
-                                      one by one; 1..64 (default 64) */
+                                      one by one; 0..64 (default 0: on MI355X a chain level
+                                      takes about as long as a host-built level, round trip
+                                      included, so plain solves run without it) */
     uint32_t chain_cands;          /* a level is built inside the chain when it has at most
                                       chain_cands candidates per workgroup of its instance
                                       (and at most 2^18), its solve has unit costs and

@@ -2,12 +2,11 @@
 
 A solve's small levels are built inside one launch, by a team of workgroups
 per instance (a single workgroup in large batches, up to 64 for a plain
-solve).  Every result must equal the host path's (vfb_tuning.chain_blocks = 0)
-field by field -- counts per level, work counters of every completed level,
+solve; off by default, vfb_tuning.chain_blocks = 0).  Every result must equal
+the host path's field by field -- counts per level, work counters of every completed level,
 formula, ordinal -- for every team size and level limit, for each key kind
 (exact, sampled types 1 and 2, hashed type 3), under row caps, with NegType 2,
-plain and batched; the other GPU tests compare the default (chain on) with
-the oracle.
+plain and batched; the other GPU tests compare the host path with the oracle.
 """
 import pytest
 
@@ -121,6 +120,7 @@ def test_chain_in_second_process_equals_host(tuning):
     V.set_tuning(chain_blocks=0)
     want = [V.solve(i, max_size=9) for i in insts]
     code = ("import json, vfb_amd as V\n"
+            "V.set_tuning(chain_blocks=64)\n"
             "got = [V.solve(V.Instance.generate(2, k), max_size=9) for k in (0, 1, 4)]\n"
             "print(json.dumps({'got': got, 'stats': V.kernel_stats()}))\n")
     print("[chain] second process", flush=True)

@@ -300,7 +300,7 @@ def _make_opts(max_size, relaxed_type, timing, budget, verbose, max_rows, search
     o.relaxed_type = relaxed_type
     # True / 1: every kernel class; 2: the expand kernel only (fewer events)
     o.timing = int(timing) if not isinstance(timing, bool) else (1 if timing else 0)
-    o.verbose = 1 if verbose else 0
+    o.verbose = int(verbose) if not isinstance(verbose, bool) else (1 if verbose else 0)
     o.hbm_budget = int(budget)
     o.max_rows = int(max_rows)
     return o

@@ -31,11 +31,21 @@ namespace vfb {
 
 constexpr uint32_t kSpinLimit = 1u << 22;
 
+constexpr uint32_t kBmWords = (uint32_t)(kChainMaxCands / 32);   // winner bitmap of a level, 32-bit words
+constexpr uint32_t kChunkW32 = 128;                               // 32-bit words per 4096-ordinal chunk
+constexpr int kMaxChunks = (int)(kChainMaxCands / 4096);
+
 struct ChainShared {
     InstDesc d;                 // the instance's descriptor, updated level by level
     Task tasks[kMaxTasks];
+    uint32_t bm[kBmWords];      // the level's winners (bit = level-local ordinal)
+    uint32_t chunk_pre[kMaxChunks + 1];   // survivors ordered before each chunk
+    uint32_t item_pre[kMaxChunks + 1];    // store items (64 survivors each) before each chunk
+    uint32_t tsum[kChainThreads / 64];    // block scan: per-wavefront totals
+    uint16_t wpre[kChainThreads / 64][64];   // store: survivors before each lane's 64 bits
     uint64_t N;                 // candidates of the level
     uint64_t waves;             // wavefronts of its expansion
+    uint64_t W, found_rank;     // survivors of the level; those ordered before the match
     int ntasks;
     int ok;                     // outcome of the last team barrier
 };
@@ -123,18 +133,107 @@ __device__ void chain_plan(ChainShared& sh, int s, int neg_type) {
     sh.waves = waves;
 }
 
-// Winner test of fresh entry e (resolve_entry's), with the slot's ordinal
-// read at agent scope: the atomicMin that set it may have come from another
-// workgroup of this launch.
-__device__ __forceinline__ void chain_resolve_entry(const InstDesc& d, uint64_t e) {
+// Winner test of fresh entry e (resolve_entry's): the candidate still holds
+// its key's lowest ordinal -- the slot's ordinal read at agent scope, since
+// the atomicMin that set it may have come from another workgroup.  Winners
+// set their bit in `bm` (LDS for a level of one workgroup, else the team's
+// bitmap in global memory); a matching winner lowers ctr->found.
+template <bool LDS>
+__device__ __forceinline__ void chain_resolve(const InstDesc& d, uint64_t e, uint32_t* bm) {
     const uint64_t of = glob(d.fresh_ord)[e];
     const uint32_t slot = glob(d.fresh_slot)[e];
     const uint64_t o = of & ~kMatchBit;
     if (ld_dev(&glob(d.table)[slot].ord) == o) {
-        const uint64_t local = o - d.epoch_base;
-        glob(d.winbyte)[local] = 1;
+        const uint32_t local = (uint32_t)(o - d.epoch_base);
+        if constexpr (LDS) atomicOr(&bm[local >> 5], 1u << (local & 31));
+        else (void)__hip_atomic_fetch_or(glob(&bm[local >> 5]), 1u << (local & 31), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
         if (of & kMatchBit) min64(glob(&d.ctr->found), local);
     }
+}
+
+// Counts the winner bitmap in LDS (words 0..nw-1; every workgroup of a team
+// the same): survivors W, survivors before each 4096-ordinal chunk, store
+// items (64 survivors) before each chunk, and the rank of `found`.
+__device__ void chain_scan(ChainShared& sh, uint32_t nw, uint64_t found) {
+    const int t = (int)threadIdx.x, lane = t & 63, w = t >> 6;
+    constexpr uint32_t per = kBmWords / kChainThreads;   // words per thread (32: a quarter chunk)
+    if (t == 0) sh.found_rank = 0;
+    uint32_t cnt = 0;
+    for (uint32_t i = 0; i < per; ++i) {
+        const uint32_t x = (uint32_t)t * per + i;
+        if (x < nw) cnt += (uint32_t)__popc(sh.bm[x]);
+    }
+    uint32_t inc = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(inc, off);
+        if (lane >= off) inc += u;
+    }
+    if (lane == 63) sh.tsum[w] = inc;
+    __syncthreads();
+    uint32_t pre = inc - cnt;
+    for (int k = 0; k < w; ++k) pre += sh.tsum[k];
+    if ((t & 3) == 0) sh.chunk_pre[t >> 2] = pre;        // 4 threads per chunk
+    if (t == kChainThreads - 1) {
+        sh.chunk_pre[kMaxChunks] = pre + cnt;
+        sh.W = pre + cnt;
+    }
+    if (found != kEmpty && (uint32_t)(found >> 5) / per == (uint32_t)t) {
+        const uint32_t fw = (uint32_t)(found >> 5);
+        uint32_t r = pre;
+        for (uint32_t x = (uint32_t)t * per; x < fw; ++x) r += (uint32_t)__popc(sh.bm[x]);
+        r += (uint32_t)__popc(sh.bm[fw] & ((1u << (found & 31)) - 1u));
+        sh.found_rank = r;
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int c = 0; c < kMaxChunks; ++c) {
+            sh.item_pre[c] = acc;
+            acc += (sh.chunk_pre[c + 1] - sh.chunk_pre[c] + 63) / 64;
+        }
+        sh.item_pre[kMaxChunks] = acc;
+    }
+    __syncthreads();
+}
+
+// Store of one item: survivors part*64 .. part*64+63 of chunk c (lane i takes
+// the (part*64+i)-th set bit), traces t0..t1-1, as rows chunk_pre[c] + j of
+// the level block (column-major, out_n = W rows); slice 0 also writes the
+// parent ordinals.  The store_item of kernels_dev.h with the chunk's bits in
+// LDS and a per-lane select instead of a list.
+__device__ __forceinline__ void chain_store_item(const ChainShared& sh, uint32_t c, uint32_t part, int t0, int t1,
+                                                 gu64* out_rows, gu64* out_par, uint64_t out_n, uint16_t* wpre,
+                                                 int lane) {
+    const InstDesc& d = sh.d;
+    const uint32_t* cw = sh.bm + (uint64_t)c * kChunkW32;
+    const uint64_t bits = ((uint64_t)cw[2 * lane + 1] << 32) | cw[2 * lane];
+    const uint32_t pc = (uint32_t)__popcll(bits);
+    uint32_t inc = pc;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(inc, off);
+        if (lane >= off) inc += u;
+    }
+    wpre[lane] = (uint16_t)(inc - pc);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    const uint32_t n = sh.chunk_pre[c + 1] - sh.chunk_pre[c];
+    const uint32_t j = part * 64 + (uint32_t)lane;
+    if (j < n) {
+        int lo = 0, hi = 63;   // the last lane whose bits start at or before survivor j
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (wpre[mid] <= j) lo = mid; else hi = mid - 1;
+        }
+        uint64_t b = ((uint64_t)cw[2 * lo + 1] << 32) | cw[2 * lo];
+        for (uint32_t k = j - wpre[lo]; k; --k) b &= b - 1;
+        const uint64_t o = (uint64_t)c * 4096 + (uint64_t)lo * 64 + (uint64_t)__ffsll((unsigned long long)b) - 1;
+        const Cand cd = decode_from(d, o, unit_of(d, (uint64_t)c * 4096));
+        const uint64_t row = sh.chunk_pre[c] + j;
+        for (int t = t0; t < t1; ++t) out_rows[(uint64_t)t * out_n + row] = cand_word(cd, t, d.lenmask[t]);
+        if (t0 == 0) out_par[row] = o;
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Barrier of the G workgroups of one team; false once any workgroup of the
@@ -170,10 +269,8 @@ __device__ __forceinline__ void store_u64(unsigned long long* p, uint64_t v) {
     __hip_atomic_store(glob(p), (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut* outs,
-                                                         int G) {
+__global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut* outs, int G) {
     __shared__ ChainShared sh;
-    __shared__ uint16_t lists[kChainThreads / 64][64 * kChunkWords];
     const int team = (int)blockIdx.x / G;
     const int rank = (int)blockIdx.x - team * G;
     ChainIn& io = ins[team];
@@ -192,10 +289,6 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
         d.fresh_ord = io.fresh_ord;
         d.fresh_slot = io.fresh_slot;
         d.fresh_cap = io.n_max;
-        d.winbyte = io.winbyte;
-        d.winbits = io.winbits;
-        d.blkcount = io.blkcount;
-        d.blkprefix = io.blkprefix;
         d.ctr_host = nullptr;
         d.check_only = 0;
         d.mark = 1;
@@ -203,21 +296,30 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
         d.found_below = kEmpty;
         d.ord_lo = 0;
         d.ord_hi = kEmpty;
-        d.store_traces = kStoreTraces;
     }
     const bool lead = rank == 0 && threadIdx.x == 0;
+    if (lead) out.t0 = wall_clock64();
     uint32_t nbar = 0;
     int level = io.level;
     uint64_t committed = io.committed, gbase = io.gbase, rows_left = io.rows_left, used = 0;
     uint64_t host_fcap = io.fresh_cap;
     double win_ratio = io.win_ratio, fresh_ratio = io.fresh_ratio;
     const uint64_t load_limit = io.cap * io.pct / 100;
+    const uint32_t bm_words = (uint32_t)((io.n_max + 31) / 32);
     for (int k = 0;; ++k) {
         const int s = level + 1;
         __syncthreads();
         if (threadIdx.x == 0) chain_plan(sh, s, io.neg_type);
         __syncthreads();
         const uint64_t N = sh.N;
+        {
+            // the other parity's global bitmap, for level s + 1, cleared by
+            // the team: it was last read at level s - 1
+            uint32_t* nb = io.bitmap[(s + 1) & 1];
+            for (uint32_t i = (uint32_t)rank * kChainThreads + threadIdx.x; i < bm_words;
+                 i += (uint32_t)G * kChainThreads)
+                nb[i] = 0u;
+        }
         const uint64_t row_bytes = 8 * (uint64_t)d.T + 8;
         const uint64_t nbytes = (N * row_bytes + 255) & ~255ull;   // the most the level's rows take
         // admission, decided alike in every workgroup: a level within the
@@ -252,88 +354,134 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
             if (fcap > N) fcap = N;
             if (fcap > host_fcap) host_fcap = fcap;
         }
+        // a level of at most solo_max candidates is built by the team's
+        // first workgroup alone (no team barrier inside it)
+        const bool solo = G == 1 || N <= io.solo_max;
+        const bool active = !solo || rank == 0;
+        const uint32_t nw = (uint32_t)((N + 31) / 32);
+        const uint32_t nw_chunks = (nw + kChunkW32 - 1) / kChunkW32 * kChunkW32;
+        uint32_t* const gbm = io.bitmap[s & 1];
+        if (lead) out.t[k][0] = wall_clock64();
         if (threadIdx.x == 0) {
             d.level = s;
             d.epoch_base = gbase;
             d.claim_limit = load_limit > committed ? load_limit - committed : 0;
             d.ctr = &io.ctr[s & 1];
-            d.nwords = (N + 63) / 64;
-            d.nblk = (d.nwords + kChunkWords - 1) / kChunkWords;
         }
+        if (solo && rank == 0)   // the LDS bitmap a solo level resolves into
+            for (uint32_t i = threadIdx.x; i < nw_chunks; i += kChainThreads) sh.bm[i] = 0u;
         __syncthreads();
 
-        // ---- expansion: the team's wavefronts over the level's tasks ----
-        for (uint64_t w = (uint64_t)rank * kWaves + wv; w < sh.waves; w += (uint64_t)G * kWaves) {
-            int lo = 0, hi = sh.ntasks - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (sh.tasks[mid].wave_begin <= w) lo = mid; else hi = mid - 1;
-            }
-            const Task& tk = sh.tasks[lo];
-            const uint64_t wl = w - tk.wave_begin;
-            if (tk.kind == 0) {
-                if (d.key_kind == KEY_MIX) expand_unary_part<KEY_MIX, kWideLanes>(d, tk, wl, lane);
-                else if (d.key_kind == KEY_FIELD) expand_unary_part<KEY_FIELD, kWideLanes>(d, tk, wl, lane);
-                else expand_unary_part<KEY_STRIDE, kWideLanes>(d, tk, wl, lane);
-            } else {
-                if (d.key_kind == KEY_MIX) expand_binary_part<KEY_MIX, kWideLanes>(d, tk, wl, lane);
-                else if (d.key_kind == KEY_FIELD) expand_binary_part<KEY_FIELD, kWideLanes>(d, tk, wl, lane);
-                else expand_binary_part<KEY_STRIDE, kWideLanes>(d, tk, wl, lane);
+        // ---- expansion: the wavefronts of the active workgroups ----
+        if (active) {
+            const uint64_t nb = solo ? 1 : (uint64_t)G, rb = solo ? 0 : (uint64_t)rank;
+            for (uint64_t w = rb * kWaves + wv; w < sh.waves; w += nb * kWaves) {
+                int lo = 0, hi = sh.ntasks - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (sh.tasks[mid].wave_begin <= w) lo = mid; else hi = mid - 1;
+                }
+                const Task& tk = sh.tasks[lo];
+                const uint64_t wl = w - tk.wave_begin;
+                if (tk.kind == 0) {
+                    if (d.key_kind == KEY_MIX) expand_unary_part<KEY_MIX, kWideLanes>(d, tk, wl, lane);
+                    else if (d.key_kind == KEY_FIELD) expand_unary_part<KEY_FIELD, kWideLanes>(d, tk, wl, lane);
+                    else expand_unary_part<KEY_STRIDE, kWideLanes>(d, tk, wl, lane);
+                } else {
+                    if (d.key_kind == KEY_MIX) expand_binary_part<KEY_MIX, kWideLanes>(d, tk, wl, lane);
+                    else if (d.key_kind == KEY_FIELD) expand_binary_part<KEY_FIELD, kWideLanes>(d, tk, wl, lane);
+                    else expand_binary_part<KEY_STRIDE, kWideLanes>(d, tk, wl, lane);
+                }
             }
         }
-        if (!team_sync(io, out, G, nbar, sh)) return;
+        if (solo) __syncthreads();
+        else if (!team_sync(io, out, G, nbar, sh)) return;
+        if (lead) out.t[k][1] = wall_clock64();
 
-        // ---- resolve ----
+        // ---- resolve: winners into the bitmap ----
         const uint64_t fc = ld_dev(glob(&d.ctr->fresh_count));
         const uint64_t nf = fc < io.n_max ? fc : io.n_max;
-        for (uint64_t e = (uint64_t)rank * kChainThreads + threadIdx.x; e < nf; e += (uint64_t)G * kChainThreads)
-            chain_resolve_entry(d, e);
-        if (!team_sync(io, out, G, nbar, sh)) return;
+        if (solo) {
+            if (rank == 0)
+                for (uint64_t e = threadIdx.x; e < nf; e += kChainThreads) chain_resolve<true>(d, e, sh.bm);
+            __syncthreads();
+        } else {
+            for (uint64_t e = (uint64_t)rank * kChainThreads + threadIdx.x; e < nf; e += (uint64_t)G * kChainThreads)
+                chain_resolve<false>(d, e, gbm);
+            if (!team_sync(io, out, G, nbar, sh)) return;
+            // every workgroup counts the level's winners from its own copy
+            for (uint32_t i = threadIdx.x; i < nw_chunks; i += kChainThreads) sh.bm[i] = i < nw ? gbm[i] : 0u;
+            __syncthreads();
+        }
+        if (lead) out.t[k][2] = wall_clock64();
 
-        // ---- pack, count and scan: the team's first workgroup ----
-        if (rank == 0) scan_one(d);
-        if (!team_sync(io, out, G, nbar, sh)) return;
-
-        // ---- outcome, read alike by every workgroup ----
-        const uint32_t ovf = __hip_atomic_load(glob(&d.ctr->overflow), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
-                             (fc > host_fcap ? 2u : 0u);
-        const uint64_t found = ld_dev(glob(&d.ctr->found));
-        const uint64_t W = ld_dev(glob(&d.ctr->winners));
+        // ---- count, and the level's outcome (alike in every active workgroup) ----
+        uint32_t ovf = 0;
+        uint64_t found = kEmpty;
+        if (active) {
+            found = ld_dev(glob(&d.ctr->found));
+            chain_scan(sh, nw, found);
+            ovf = __hip_atomic_load(glob(&d.ctr->overflow), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+                  (fc > host_fcap ? 2u : 0u);
+        }
+        if (lead) out.t[k][3] = wall_clock64();
+        const uint64_t W = active ? sh.W : 0;
         uint64_t bytes = W ? (W * row_bytes + 255) & ~255ull : 0;   // as the host allocates a level's rows
         if (W && bytes < 256) bytes = 256;
-        if (ovf || found != kEmpty) {
+        unsigned long long* rows = io.slab + used / 8;
+
+        // ---- store: survivors in ordinal order (unless the level ends the chain) ----
+        if (active && !ovf && found == kEmpty && W) {
+            const uint32_t nsl = (uint32_t)((d.T + kStoreTraces - 1) / kStoreTraces);
+            const uint32_t nch = (uint32_t)((N + 4095) / 4096);
+            const uint64_t items = (uint64_t)sh.item_pre[nch] * nsl;
+            gu64* const orows = glob(rows);
+            gu64* const opar = glob(rows + (uint64_t)d.T * W);
+            const uint64_t nb = solo ? 1 : (uint64_t)G, rb = solo ? 0 : (uint64_t)rank;
+            for (uint64_t g = rb * kWaves + wv; g < items; g += nb * kWaves) {
+                const uint32_t ci = (uint32_t)(g / nsl);
+                const int t0 = (int)(g - (uint64_t)ci * nsl) * kStoreTraces;
+                const int t1 = t0 + kStoreTraces < d.T ? t0 + kStoreTraces : d.T;
+                int lo = 0, hi = (int)nch - 1;   // the chunk holding item ci
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (sh.item_pre[mid] <= ci) lo = mid; else hi = mid - 1;
+                }
+                chain_store_item(sh, (uint32_t)lo, ci - sh.item_pre[lo], t0, t1, orows, opar, W, sh.wpre[wv], lane);
+            }
+        }
+        if (lead) {
+            // the level's outcome for the other workgroups and the host
+            store_u64(&d.ctr->winners, W);
+            store_u64(&d.ctr->found_rank, sh.found_rank);
+            if (ovf) __hip_atomic_fetch_or(glob(&d.ctr->overflow), ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (!team_sync(io, out, G, nbar, sh)) return;
+        if (lead) out.t[k][4] = wall_clock64();
+        const uint64_t Wt = ld_dev(glob(&d.ctr->winners));
+        const uint64_t ft = ld_dev(glob(&d.ctr->found));
+        const uint32_t ot = __hip_atomic_load(glob(&d.ctr->overflow), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ot || ft != kEmpty) {
+            if (!solo)   // this level's bitmap starts the next launch cleared
+                for (uint32_t i = (uint32_t)rank * kChainThreads + threadIdx.x; i < nw; i += (uint32_t)G * kChainThreads)
+                    gbm[i] = 0u;
             if (lead) {
-                out.exit = ovf ? CH_REDO : CH_FOUND;
+                out.exit = ot ? CH_REDO : CH_FOUND;
                 out.nlev = k;
-                out.overflow = ovf;
-                out.found = found;
+                out.overflow = ot;
+                out.found = ft;
                 out.found_rank = ld_dev(glob(&d.ctr->found_rank));
                 out.N_exit = N;
-                out.W_exit = W;
+                out.W_exit = Wt;
                 out.fresh_exit = fc;
             }
             return;
         }
-
-        // ---- store: survivors in ordinal order, then the level is done ----
-        unsigned long long* rows = io.slab + used / 8;
-        {
-            const uint64_t nsl = (uint64_t)((d.T + kStoreTraces - 1) / kStoreTraces);
-            gu64* const orows = glob(rows);
-            gu64* const opar = glob(rows + (uint64_t)d.T * W);
-            if (d.nblk <= kStoreSmallBlk) {
-                const uint64_t items = d.nblk * kStorePartsSmall * nsl;
-                for (uint64_t g = (uint64_t)rank * kWaves + wv; g < items; g += (uint64_t)G * kWaves)
-                    store_item<kStorePartsSmall>(d, g, lists[wv], orows, opar, W, lane);
-            } else {
-                const uint64_t items = d.nblk * kStoreParts * nsl;
-                for (uint64_t g = (uint64_t)rank * kWaves + wv; g < items; g += (uint64_t)G * kWaves)
-                    store_item<kStoreParts>(d, g, lists[wv], orows, opar, W, lane);
-            }
-        }
+        uint64_t bt = Wt ? (Wt * row_bytes + 255) & ~255ull : 0;
+        if (Wt && bt < 256) bt = 256;
         if (lead) {
             out.N[k] = N;
-            out.W[k] = W;
+            out.W[k] = Wt;
             out.fresh[k] = fc;
             out.off[k] = used;
             // the counters of level s + 2 start cleared (no workgroup reads
@@ -348,16 +496,15 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
         }
         if (threadIdx.x == 0) {
             d.lvl[s] = rows;
-            d.nrows[s] = (uint32_t)W;
+            d.nrows[s] = (uint32_t)Wt;
         }
-        committed += W;
+        committed += Wt;
         gbase += N;
-        rows_left -= W;
-        used += bytes;
-        win_ratio = (double)W / (double)(N > 1 ? N : 1);
+        rows_left -= Wt;
+        used += bt;
+        win_ratio = (double)Wt / (double)(N > 1 ? N : 1);
         fresh_ratio = (double)fc / (double)(N > 1 ? N : 1);
         level = s;
-        if (!team_sync(io, out, G, nbar, sh)) return;
     }
 }
 

@@ -1255,12 +1255,10 @@ void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
     v.swap(rest);
     if (ch.empty()) return;
     const size_t n = ch.size();
-    // workspace: the winner maps of all instances first, then per instance
-    // its fresh list and chunk counts
-    const uint64_t nw = (n_max + 63) / 64, nb = (nw + kChunkWords - 1) / kChunkWords;
-    const size_t wb = align256(nw * 64), fo = align256(n_max * 8), fs = align256(n_max * 4);
-    const size_t bits = align256(nw * 8), bc = align256(nb * 4), bp = align256(nb * 8);
-    const size_t per = fo + fs + bits + bc + bp;
+    // workspace: the two winner bitmaps of every instance first, then per
+    // instance its fresh list
+    const size_t bmb = align256((n_max + 31) / 32 * 4), fo = align256(n_max * 8), fs = align256(n_max * 4);
+    const size_t wb = 2 * bmb, per = fo + fs;
     const size_t wb_all = n * wb, need = wb_all + n * per;
     if (need > chain_ws_cap_) {
         sync();
@@ -1292,13 +1290,12 @@ void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
         c.slab = (unsigned long long*)I.slabs.back().p;
         c.slab_bytes = I.slabs.back().bytes;
         char* w = d_chain_ws_ + wb_all + k * per;
-        c.winbyte = (unsigned char*)(d_chain_ws_ + k * wb);
+        c.bitmap[0] = (uint32_t*)(d_chain_ws_ + k * wb);
+        c.bitmap[1] = (uint32_t*)(d_chain_ws_ + k * wb + bmb);
         c.fresh_ord = (unsigned long long*)w;
         c.fresh_slot = (uint32_t*)(w + fo);
-        c.winbits = (unsigned long long*)(w + fo + fs);
-        c.blkcount = (uint32_t*)(w + fo + fs + bits);
-        c.blkprefix = (unsigned long long*)(w + fo + fs + bits + bc);
         c.n_max = n_max;
+        c.solo_max = std::min<uint64_t>(tun_.chain_cands, kChainSoloMax);
         c.rows_left = I.max_rows ? (I.max_rows > I.stored ? I.max_rows - I.stored : 0) : ~0ull;
         c.committed = I.committed;
         c.cap = I.cap;
@@ -1329,7 +1326,7 @@ void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
     for (size_t k = 0; k < n; ++k)
         if (h_chain_out_[k].err || h_chain_out_[k].exit == CH_ERROR)
             throw DeviceError("level chain: a team barrier timed out");
-    chain_wb_clean_ = wb_all;   // every level the chain began, it packed (and so cleared)
+    chain_wb_clean_ = wb_all;   // a team leaves both its bitmaps cleared
     for (size_t k = 0; k < n; ++k) chain_adopt(*ch[k], h_chain_out_[k]);
 }
 
@@ -1340,6 +1337,14 @@ void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
 // next level, which the host builds.
 void Engine::chain_adopt(Inst& I, const ChainOut& o) {
     vfb_result& R = *I.res;
+    if (opt_.verbose >= 2) {
+        // per level: microseconds from the launch to the end of each phase
+        std::fprintf(stderr, "chain: exit %d after %d levels\n", o.exit, o.nlev);
+        for (int j = 0; j <= o.nlev && j < kMaxS; ++j)
+            std::fprintf(stderr, "chain level %d: plan %.1f expand %.1f resolve %.1f scan %.1f store %.1f us\n",
+                         I.level + 1 + j, (o.t[j][0] - o.t0) * 0.01, (o.t[j][1] - o.t0) * 0.01,
+                         (o.t[j][2] - o.t0) * 0.01, (o.t[j][3] - o.t0) * 0.01, (o.t[j][4] - o.t0) * 0.01);
+    }
     Buf& slab = I.slabs.back();
     auto level_work = [&]() {   // the fused pass's algorithmic bytes (pass_work)
         I.pass = P_FUSED;
@@ -2110,7 +2115,7 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     t->max_load_pct = 70;
     t->store_traces = 64;
     t->check_rows = kCheckRows;
-    t->chain_blocks = kChainMaxTeam;
+    t->chain_blocks = 0;
     t->chain_cands = 1024;
     t->pad_tuning = 0;
 }

@@ -285,7 +285,7 @@ class Engine {
     std::vector<Inst*> parked_;   // those members, resumed one at a time after the rest
     // persistent level chain: workspace of the launch's instances (fresh
     // lists, winner maps, chunk counts; not part of any solve's budget),
-    // laid out winner maps first; the first chain_wb_clean_ bytes are zero
+    // laid out winner bitmaps first; the first chain_wb_clean_ bytes are zero
     char* d_chain_ws_ = nullptr;
     size_t chain_ws_cap_ = 0, chain_wb_clean_ = 0;
     ChainOut* h_chain_out_ = nullptr;   // pinned read-back

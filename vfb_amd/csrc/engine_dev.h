@@ -199,7 +199,8 @@ enum ChainExit : int32_t {
 };
 constexpr int kChainThreads = 256;
 constexpr int kChainMaxTeam = 64;               // workgroups per instance, at most
-constexpr uint64_t kChainMaxCands = 1ull << 18; // a chain level fits one workgroup's scan (64 chunks)
+constexpr uint64_t kChainMaxCands = 1ull << 18; // a chain level's winner bitmap fits LDS (64 chunks)
+constexpr uint64_t kChainSoloMax = 64;          // levels this small are built by a team's first workgroup alone
 constexpr int kMaxTasks = 1 + kMaxS / 2;        // ~XFG task + one per split
 
 // per instance, uploaded for one launch
@@ -207,13 +208,11 @@ struct ChainIn {
     InstDesc d;                 // descriptor with lvl / nrows of levels 1..level
     unsigned long long* slab;   // rows of the chain's levels, back to back (256-byte aligned blocks)
     uint64_t slab_bytes;
-    unsigned long long* fresh_ord;   // workspace of n_max candidates (winbyte all zero)
+    unsigned long long* fresh_ord;   // workspace of n_max candidates
     uint32_t* fresh_slot;
-    unsigned char* winbyte;
-    unsigned long long* winbits;
-    uint32_t* blkcount;
-    unsigned long long* blkprefix;
+    uint32_t* bitmap[2];        // winners of team levels of even / odd size, n_max bits each (zero)
     uint64_t n_max;             // a level enters the chain only with at most this many candidates
+    uint64_t solo_max;          // a level of at most this many is built by the team's first workgroup
     uint64_t rows_left;         // rows the solve may still store (max_rows), ~0 for no cap
     uint64_t committed;         // keys in the set
     uint64_t cap;               // slots of the set
@@ -237,6 +236,9 @@ struct ChainOut {
     uint64_t found, found_rank; // CH_FOUND: level-local ordinal, survivors ordered before it
     uint64_t N_exit, W_exit, fresh_exit;   // candidates / survivors / fresh entries of the exit level
     uint64_t N[kMaxS], W[kMaxS], fresh[kMaxS], off[kMaxS];   // chain level k is level + 1 + k
+    // device clock (100 MHz) of the team's first workgroup: launch, then per
+    // level the end of its plan, expansion, resolve, scan and store phases
+    uint64_t t0, t[kMaxS][5];
 };
 
 }  // namespace vfb

@@ -169,13 +169,16 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     // only, so they can change where a budget-limited solve is cut)
     static const uint64_t cand_st[] = {4, 8, 16, 32, 64};
     static const uint64_t cand_rows[] = {1, 4, 8, 16};
+    static const uint64_t cand_chain[] = {0, 4, 16, 64};
+    static const uint64_t cand_chain_cands[] = {256, 1024, 4096};
     struct Axis {
         const Field* f;
         const uint64_t* v;
         size_t nv;
     } axes[] = {{&kFields[0], cand_pass, 5}, {&kFields[1], cand_wide, 7}, {&kFields[2], cand_expand, 4},
                 {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}, {&kFields[5], cand_park, 4},
-                {&kFields[8], cand_st, 5},      {&kFields[9], cand_rows, 4}};
+                {&kFields[8], cand_st, 5},      {&kFields[9], cand_rows, 4},   {&kFields[10], cand_chain, 4},
+                {&kFields[11], cand_chain_cands, 3}};
     // every setting timed, for the cache file: "# tried <key> <value> <ms>"
     std::string log = "# tried start - " + std::to_string(t_default) + "\n";
     for (const Axis& ax : axes) {
