@@ -395,6 +395,12 @@ static void key_params(const vfb_instance& in, int relaxed_type, InstDesc& h) {
         h.koff[t] = 0;
         h.kbits[t] = 0;
     }
+    // positive (0..P-1) and negative traces alternating, the longer list's
+    // rest at the end
+    for (int i = 0, p = 0, n = in.P; i < T; ++i) {
+        const bool pos = p < in.P && (n >= T || (i & 1) == 0);
+        h.check_order[i] = (uint8_t)(pos ? p++ : n++);
+    }
     h.half = T / 2;
     h.stride = 1;
     if (lenSum <= 126) {                         // exact concatenation, :94-106

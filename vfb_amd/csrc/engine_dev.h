@@ -98,6 +98,8 @@ struct InstDesc {
     uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
     int16_t koff[kMaxT];        // first key bit of the trace's field / samples
     int16_t kbits[kMaxT];       // field width (KEY_FIELD) or number of samples (KEY_STRIDE)
+    uint8_t check_order[kMaxT]; // target checks visit the traces positives and negatives
+                                // alternating, so that most candidates miss one early
     const ColInfo* cinfo;       // mw: [C] column descriptors
     const int32_t* tcol;        // mw: [T + 1] first column of each trace
     const unsigned long long* lvl[kMaxS + 2];

@@ -287,19 +287,23 @@ __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task&
     const uint64_t nl = tk.nl;
     const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
     const gu64* __restrict__ R = glob(d.lvl[tk.lvl_r]);
+    // mismatch bits per left row, starting with the candidates that do not
+    // count (no pair, or not in the task's part): once every candidate of
+    // the wavefront has missed a trace, the rest of the traces are skipped
     uint32_t mm[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) mm[k] = 0;
+    for (int k = 0; k < K; ++k)
+        mm[k] = (!active || l0 + k >= nl || (diag && r <= l0 + k)) ? 0xfu : (~(uint32_t)tk.part & 0xfu);
     const int T = d.T;
     for (int t0 = 0; t0 < T; t0 += kTraceGroup) {
         uint64_t bv[kTraceGroup];
 #pragma unroll
         for (int j = 0; j < kTraceGroup; ++j)
-            if (t0 + j < T) bv[j] = R[(uint64_t)(t0 + j) * tk.nr + rr];
+            if (t0 + j < T) bv[j] = R[(uint64_t)d.check_order[t0 + j] * tk.nr + rr];
 #pragma unroll
         for (int j = 0; j < kTraceGroup; ++j) {
-            const int t = t0 + j;
-            if (t >= T) break;
+            if (t0 + j >= T) break;
+            const int t = d.check_order[t0 + j];
             const uint64_t b = bv[j];
             const uint64_t want = t < d.P ? 1ull : 0ull;
 #pragma unroll
@@ -310,6 +314,10 @@ __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task&
                 mm[k] |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
             }
         }
+        uint32_t all = 0xfu;
+#pragma unroll
+        for (int k = 0; k < K; ++k) all &= mm[k];
+        if (!__ballot(all != 0xfu)) break;
     }
     uint64_t best = kEmpty;
 #pragma unroll

@@ -436,16 +436,24 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
     const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
     Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     uint32_t mm = 0;
+    // target checks only: candidates that do not count are settled from the
+    // start, and the traces left are skipped once every candidate this lane
+    // sees (and every other lane's) has missed one
+    const uint32_t dead = CHECK ? (active ? (~(uint32_t)tk.part & 0xfu) : 0xfu) : 0u;
     const int T = d.T;
     for (int t0 = sub; t0 < T; t0 += G * kTraceGroup) {
+        if constexpr (CHECK) {
+            if (!__ballot((mm | dead) != 0xfu)) break;
+        }
         uint64_t av[kTraceGroup];
+        // (a check visits the traces in check_order; keys need them in order)
 #pragma unroll
         for (int j = 0; j < kTraceGroup; ++j)
-            if (t0 + j * G < T) av[j] = L[(uint64_t)(t0 + j * G) * tk.nl + xr];
+            if (t0 + j * G < T) av[j] = L[(uint64_t)(CHECK ? d.check_order[t0 + j * G] : t0 + j * G) * tk.nl + xr];
 #pragma unroll
         for (int j = 0; j < kTraceGroup; ++j) {
-            const int t = t0 + j * G;
-            if (t >= T) break;
+            if (t0 + j * G >= T) break;
+            const int t = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
             const uint64_t a = av[j];
             const uint64_t m = d.lenmask[t];
             const uint64_t want = d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
@@ -505,19 +513,25 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     const gu64* __restrict__ R = glob(d.lvl[tk.lvl_r]);
     Key k[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     uint32_t mm = 0;
+    // target checks only: early exit as in expand_unary
+    const uint32_t dead = CHECK ? (active ? (~(uint32_t)tk.part & 0xfu) : 0xfu) : 0u;
     const int T = d.T;
     for (int t0 = sub; t0 < T; t0 += G * kTraceGroup) {
+        if constexpr (CHECK) {
+            if (!__ballot((mm | dead) != 0xfu)) break;
+        }
         uint64_t av[kTraceGroup], bv[kTraceGroup];
 #pragma unroll
         for (int j = 0; j < kTraceGroup; ++j)
             if (t0 + j * G < T) {
-                av[j] = L[(uint64_t)(t0 + j * G) * tk.nl + l];
-                bv[j] = R[(uint64_t)(t0 + j * G) * tk.nr + rr];
+                const uint64_t tt = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
+                av[j] = L[tt * tk.nl + l];
+                bv[j] = R[tt * tk.nr + rr];
             }
 #pragma unroll
         for (int j = 0; j < kTraceGroup; ++j) {
-            const int t = t0 + j * G;
-            if (t >= T) break;
+            if (t0 + j * G >= T) break;
+            const int t = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
             const uint64_t a = av[j], b = bv[j];
             const uint64_t want = d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
             const uint64_t cmask = d.bs_mode ? ~0ull : 1ull;
