@@ -756,6 +756,32 @@ __device__ __forceinline__ uint64_t cand_word(const Cand& c, int t, uint64_t m) 
     return apply_op(c.op, a, b, m);
 }
 
+// Traces t0..t1-1 of candidate c into rows `row` of out (column-major, out_n
+// rows): the operand words of kStoreGroup traces are loaded before any of
+// them is stored, so their loads are in flight together (a store between
+// them would otherwise order each trace's loads after the last one's store).
+constexpr int kStoreGroup = 8;
+__device__ __forceinline__ void cand_words_out(const Cand& c, int t0, int t1, const uint64_t* lenmask, gu64* out,
+                                               uint64_t out_n, uint64_t row) {
+    const bool unary = c.op == VFB_OP_NOT || c.op == VFB_OP_NEXT || c.op == VFB_OP_FINALLY || c.op == VFB_OP_GLOBALLY;
+    for (int t = t0; t < t1; t += kStoreGroup) {
+        uint64_t a[kStoreGroup], b[kStoreGroup];
+#pragma unroll
+        for (int j = 0; j < kStoreGroup; ++j) {
+            a[j] = t + j < t1 ? c.L[(uint64_t)(t + j) * c.nl + c.l] : 0ull;
+            b[j] = t + j < t1 && !unary ? c.R[(uint64_t)(t + j) * c.nr + c.r] : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < kStoreGroup; ++j) {
+            if (t + j >= t1) break;
+            uint64_t v;
+            if (c.op == VFB_OP_UNTIL) v = c.swap ? until_op(b[j], a[j]) : until_op(a[j], b[j]);
+            else v = apply_op(c.op, a[j], b[j], lenmask[t + j]);
+            out[(uint64_t)(t + j) * out_n + row] = v;
+        }
+    }
+}
+
 // trace t of candidate c over its words (multi-word rows), last word first
 __device__ __forceinline__ void store_trace_mw(const InstDesc& d, const Cand& cd, int t, gu64* out,
                                                uint64_t out_n, uint64_t row) {
@@ -829,7 +855,7 @@ __device__ __forceinline__ void store_item(const InstDesc& d, uint64_t item, uin
         if (d.mw) {
             for (int t = t0; t < t1; ++t) store_trace_mw(d, cd, t, out_rows, out_n, row);
         } else {
-            for (int t = t0; t < t1; ++t) out_rows[(uint64_t)t * out_n + row] = cand_word(cd, t, d.lenmask[t]);
+            cand_words_out(cd, t0, t1, d.lenmask, out_rows, out_n, row);
         }
         if (t0 == 0) out_par[row] = o;
     }
