@@ -356,7 +356,7 @@ void Engine::flush_ops() {
         const unsigned long long* src =
             f.staged == SIZE_MAX ? nullptr : (const unsigned long long*)(d_ops_ + o_st) + f.staged;
         fs[k] = OpSeg{f.dst, src, f.words, f.value, ftot};
-        ftot += f.words;
+        ftot += (f.words + 7) / 8;   // k_fill_multi takes 8 words per thread
     }
     RehashSeg* rs = (RehashSeg*)(host.data() + o_rs);
     uint64_t rtot = 0;

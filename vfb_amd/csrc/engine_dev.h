@@ -151,8 +151,9 @@ struct Task {
     uint64_t wave_begin;        // prefix over tasks, in wavefronts
 };
 
-// One segment of a batched fill / copy: words [begin, begin + words) of the
-// flat index space set dst[i] = src ? src[i] : value.
+// One segment of a batched fill / copy: dst[i] = src ? src[i] : value for
+// i < words, as chunks [begin, begin + ceil(words / 8)) of the flat chunk
+// space (8 words per chunk).
 struct OpSeg {
     unsigned long long* dst;
     const unsigned long long* src;

@@ -509,13 +509,28 @@ __global__ void k_rehash_multi(const RehashSeg* __restrict__ segs, int n, uint64
     }
 }
 
-// fills and copies of many buffers in one launch
-__global__ void k_fill_multi(const OpSeg* __restrict__ segs, int n, uint64_t total) {
+// fills and copies of many buffers in one launch: one thread per chunk of 8
+// words (64 bytes) of a segment (segment k holds chunks begin_k ..), a whole
+// chunk of a fill as four 16-byte stores (every destination is 256-byte
+// aligned), copies and segment tails word by word
+__global__ void k_fill_multi(const OpSeg* __restrict__ segs, int n, uint64_t total_chunks) {
+    typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total; g += stride) {
+    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total_chunks; g += stride) {
         const OpSeg& sg = segs[seg_of(segs, n, g)];
-        const uint64_t i = g - sg.begin;
-        glob(sg.dst)[i] = sg.src ? glob(sg.src)[i] : sg.value;
+        const uint64_t i0 = (g - sg.begin) * 8;
+        const uint64_t m = sg.words - i0 < 8 ? sg.words - i0 : 8;
+        gu64* dst = glob(sg.dst) + i0;
+        if (!sg.src && m == 8) {
+            const u64x2 v = {sg.value, sg.value};
+            GLB u64x2* d2 = (GLB u64x2*)dst;
+            d2[0] = v;
+            d2[1] = v;
+            d2[2] = v;
+            d2[3] = v;
+        } else {
+            for (uint64_t j = 0; j < m; ++j) dst[j] = sg.src ? glob(sg.src)[i0 + j] : sg.value;
+        }
     }
 }
 
