@@ -106,6 +106,8 @@ Engine::~Engine() {
     if (d_flag_) (void)hipFree(d_flag_);
     if (pin_) (void)hipHostFree(pin_);
     if (d_chain_ws_) (void)hipFree(d_chain_ws_);
+    if (d_gather_) (void)hipFree(d_gather_);
+    if (h_gather_) (void)hipHostFree(h_gather_);
     if (h_chain_out_) (void)hipHostFree(h_chain_out_);
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     for (auto& t : timed_) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -859,11 +861,79 @@ void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
     I.res->table_slots = need;
 }
 
+// device address of the parent ordinal of row `row` of level `level`
+static const unsigned long long* parent_addr(const Inst& I, int level, uint64_t row) {
+    return (const unsigned long long*)I.rows[level].p + (uint64_t)I.h.C * I.nrows[level] + row;
+}
+
+// the operand nodes of candidate o of level `level` (as cand_string decodes
+// it), collecting the parent ordinals not yet in par_cache_
+void Engine::walk_cand(Inst& I, int level, uint64_t o, std::vector<const unsigned long long*>& need) {
+    const auto& U = I.units[level];
+    size_t k = 0;
+    for (size_t q = 0; q < U.size(); ++q)
+        if (U[q].count && U[q].base <= o) k = q;
+    const Unit& u = U[k];
+    const uint64_t x = o - u.base;
+    if (u.op == VFB_OP_NOT || u.op == VFB_OP_NEXT || u.op == VFB_OP_FINALLY || u.op == VFB_OP_GLOBALLY) {
+        walk_node(I, u.lvl_l, x, need);
+        return;
+    }
+    const uint64_t p = u.op == VFB_OP_UNTIL ? x >> 1 : x;
+    walk_node(I, u.lvl_l, p / u.nr, need);
+    walk_node(I, u.lvl_r, p % u.nr, need);
+}
+
+void Engine::walk_node(Inst& I, int level, uint64_t row, std::vector<const unsigned long long*>& need) {
+    if (level == I.cost[0] && !I.bs_mode) return;
+    const unsigned long long* a = parent_addr(I, level, row);
+    const auto it = par_cache_.find(a);
+    if (it == par_cache_.end()) {
+        need.push_back(a);
+        return;
+    }
+    if (I.bs_mode && level <= I.ltl_levels) walk_node(*I.ltl, level, it->second, need);
+    else walk_cand(I, level, it->second, need);
+}
+
+// Reads the parent ordinals of every node of the formulas refs will spell
+// out, one tree depth at a time: a gather launch and one copy per depth for
+// all of them, so that node_string then reads none itself.
+void Engine::prefetch_parents(const std::vector<FormulaRef>& refs) {
+    par_cache_.clear();
+    if (fixed_ || refs.empty()) return;   // a fixed engine allocates nothing: node_string copies
+    for (int depth = 0; depth <= 2 * kMaxS; ++depth) {
+        std::vector<const unsigned long long*> need;
+        for (const auto& f : refs) walk_cand(*f.I, f.level, f.o, need);
+        if (need.empty()) return;
+        std::sort(need.begin(), need.end());
+        need.erase(std::unique(need.begin(), need.end()), need.end());
+        const size_t n = need.size();
+        if (n > gather_cap_) {
+            sync();
+            if (d_gather_) CK(hipFree(d_gather_));
+            if (h_gather_) CK(hipHostFree(h_gather_));
+            d_gather_ = nullptr;
+            h_gather_ = nullptr;
+            gather_cap_ = std::max<size_t>(n, 2 * gather_cap_);
+            CK(hipMalloc((void**)&d_gather_, 2 * gather_cap_ * 8));
+            CK(hipHostMalloc((void**)&h_gather_, gather_cap_ * 8, hipHostMallocDefault));
+        }
+        h2d(d_gather_, need.data(), n * 8);
+        CK(launch_gather((const unsigned long long* const*)d_gather_, n, d_gather_ + gather_cap_, st_));
+        CK(hipMemcpyAsync(h_gather_, d_gather_ + gather_cap_, n * 8, hipMemcpyDeviceToHost, st_));
+        sync();
+        for (size_t i = 0; i < n; ++i) par_cache_[need[i]] = h_gather_[i];
+    }
+}
+
 std::string Engine::node_string(Inst& I, int level, uint64_t row) {
     if (level == I.cost[0] && !I.bs_mode) return I.in->names[row];
     uint64_t o = 0;
-    const uint64_t* par = (const uint64_t*)I.rows[level].p + (uint64_t)I.h.C * I.nrows[level];
-    CK(hipMemcpy(&o, par + row, 8, hipMemcpyDeviceToHost));
+    const unsigned long long* a = parent_addr(I, level, row);
+    const auto it = par_cache_.find(a);
+    if (it != par_cache_.end()) o = it->second;
+    else CK(hipMemcpy(&o, a, 8, hipMemcpyDeviceToHost));
     // a BS leaf is the CS row it was taken from
     if (I.bs_mode && level <= I.ltl_levels) return node_string(*I.ltl, level, o);
     return cand_string(I, level, o, nullptr);
@@ -1333,15 +1403,27 @@ void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
         if (h_chain_out_[k].err || h_chain_out_[k].exit == CH_ERROR)
             throw DeviceError("level chain: a team barrier timed out");
     chain_wb_clean_ = wb_all;   // a team leaves both its bitmaps cleared
-    for (size_t k = 0; k < n; ++k) chain_adopt(*ch[k], h_chain_out_[k]);
+    std::vector<FormulaRef> found;
+    for (size_t k = 0; k < n; ++k)
+        if (chain_adopt(*ch[k], h_chain_out_[k])) found.push_back(FormulaRef{ch[k], ch[k]->level + 1, h_chain_out_[k].found});
+    // the formulas found inside the chain, spelled out after batched reads
+    prefetch_parents(found);
+    for (auto& f : found) {
+        vfb_result& R = *f.I->res;
+        std::string s = cand_string(*f.I, f.level, f.o, &R.top_op);
+        R.formula = strdup(s.c_str());
+        finish(*f.I, VFB_OK);
+    }
+    par_cache_.clear();
 }
 
 // The levels one instance's team built (o.nlev, stored in its slab) become
 // its levels, with the counters, results and host state a host-built level
-// leaves; then its exit: the found formula, or a level for the host to build
-// again (its keys in the set dropped, as after an overflow there), or the
-// next level, which the host builds.
-void Engine::chain_adopt(Inst& I, const ChainOut& o) {
+// leaves; then its exit: a match (true: the caller spells out the formula and
+// finishes the instance), or a level for the host to build again (its keys in
+// the set dropped, as after an overflow there), or the next level, which the
+// host builds.
+bool Engine::chain_adopt(Inst& I, const ChainOut& o) {
     vfb_result& R = *I.res;
     if (opt_.verbose >= 2) {
         // per level: microseconds from the launch to the end of each phase
@@ -1410,7 +1492,7 @@ void Engine::chain_adopt(Inst& I, const ChainOut& o) {
         // the host builds level s; a level the chain refused outright is
         // not offered to it again
         if (o.nlev == 0) I.chain_skip = s;
-        return;
+        return false;
     }
     if (o.nlev) build_units(I);
     if (I.N != o.N_exit) throw DeviceError("level chain: a level's candidates differ from the host's units");
@@ -1424,10 +1506,7 @@ void Engine::chain_adopt(Inst& I, const ChainOut& o) {
         R.min_size = s;
         R.found_ordinal = o.found;
         R.bs_stage = 0;
-        std::string f = cand_string(I, s, o.found, &R.top_op);
-        R.formula = strdup(f.c_str());
-        finish(I, VFB_OK);
-        return;
+        return true;
     }
     // CH_REDO: the host builds level s again; its keys in the set are dropped
     // by rebuilding the set from the completed levels' keys, grown as the
@@ -1438,9 +1517,9 @@ void Engine::chain_adopt(Inst& I, const ChainOut& o) {
         const uint64_t keys = I.cap * tun_.max_load_pct / 100;
         ensure_table(I, (o.overflow & 5) ? keys * 4 : 0, true);
     } catch (CapacityError& e) {
-        if (on_capacity(I, e)) return;
-        I.next_check_only = true;   // cannot rebuild: the level is checked only, as on the host
+        if (!on_capacity(I, e)) I.next_check_only = true;   // cannot rebuild: checked only, as on the host
     }
+    return false;
 }
 
 void Engine::begin(const vfb_options& opt, bool batch) {
@@ -1707,6 +1786,14 @@ void Engine::step(std::vector<Inst*>& act) {
         }
 
         // ---- 4. results, overflow decisions, stores ----
+        {
+            // the formulas found this step: their nodes' parent ordinals in
+            // batched reads before they are spelled out below
+            std::vector<FormulaRef> fr;
+            for (auto* Ip : step)
+                if (!Ip->done && Ip->ctr.found != kEmpty) fr.push_back(FormulaRef{Ip, Ip->level + 1, Ip->ctr.found});
+            prefetch_parents(fr);
+        }
         std::vector<Inst*> st;
         for (auto* Ip : step) {
             Inst& I = *Ip;
@@ -1815,6 +1902,7 @@ void Engine::step(std::vector<Inst*>& act) {
             I.gbase += I.N;
             I.level += 1;
         }
+        par_cache_.clear();
         st.erase(std::remove_if(st.begin(), st.end(), [](Inst* I) { return I->done; }), st.end());
         if (!st.empty()) store(st);
     }

@@ -9,6 +9,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "engine_dev.h"
@@ -38,6 +39,7 @@ hipError_t launch_insert_keys(Slot* tab, uint64_t mask, const unsigned long long
                               hipStream_t st);
 hipError_t launch_rehash_multi(const RehashSeg* segs, int n, uint64_t total, unsigned int* fail, int grid,
                                hipStream_t st);
+hipError_t launch_gather(const unsigned long long* const* addr, uint64_t n, unsigned long long* out, hipStream_t st);
 // small: every level of the launch has at most kStoreSmallBlk chunks (kStorePartsSmall subsets)
 hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t total_items, StorePatch patch, int small,
                         int grid, hipStream_t st);
@@ -291,6 +293,13 @@ class Engine {
     ChainOut* h_chain_out_ = nullptr;   // pinned read-back
     size_t h_chain_out_cap_ = 0;
     int chain_resident_ = 0;      // k_chain workgroups resident on the device at once
+    // formulas found in one step: their parent ordinals, read in batched
+    // gathers (one per depth of the formula trees) instead of one copy per node
+    struct FormulaRef { Inst* I; int level; uint64_t o; };
+    std::unordered_map<const unsigned long long*, unsigned long long> par_cache_;
+    unsigned long long* d_gather_ = nullptr;   // addresses, then values
+    unsigned long long* h_gather_ = nullptr;   // pinned read-back
+    size_t gather_cap_ = 0;
 
     void sync();
     void h2d(void* dst, const void* src, size_t bytes);
@@ -309,6 +318,9 @@ class Engine {
     void defer(Inst& I);
     bool on_capacity(Inst& I, const CapacityError& e);
     std::string node_string(Inst& I, int level, uint64_t row);
+    void walk_cand(Inst& I, int level, uint64_t o, std::vector<const unsigned long long*>& need);
+    void walk_node(Inst& I, int level, uint64_t row, std::vector<const unsigned long long*>& need);
+    void prefetch_parents(const std::vector<FormulaRef>& refs);
     std::string cand_string(Inst& I, int level, uint64_t o, int* top_op);
     void fill(Inst& I, void* p, uint64_t words, uint64_t v);
     void upload(Inst& I, void* dst, const uint64_t* src, uint64_t words);
@@ -322,7 +334,7 @@ class Engine {
     void prepare_or_check(Inst& I, std::vector<Inst*>& step);
     bool chain_admits(const Inst& I, uint64_t n_max) const;
     void chain(std::vector<Inst*>& v, int G, uint64_t n_max);
-    void chain_adopt(Inst& I, const ChainOut& o);
+    bool chain_adopt(Inst& I, const ChainOut& o);
 };
 
 // The process-wide engine (its arena holds all device memory of every solve)

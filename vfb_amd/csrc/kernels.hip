@@ -633,9 +633,22 @@ __global__ __launch_bounds__(256) void k_store(const InstDesc* __restrict__ D, c
     }
 }
 
+// out[i] = *addr[i]: the parent ordinals the host needs to spell out the
+// formulas found in one step, read in one launch
+__global__ void k_gather(const unsigned long long* const* __restrict__ addr, uint64_t n,
+                         unsigned long long* __restrict__ out) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *glob(addr[i]);
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers (host)
 // ---------------------------------------------------------------------------
+
+hipError_t launch_gather(const unsigned long long* const* addr, uint64_t n, unsigned long long* out, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, addr, n, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_expand(const InstDesc* D, const Task* tasks, int ntasks, uint64_t total_waves, int wide, int mw,
                          int grid, hipStream_t st) {
