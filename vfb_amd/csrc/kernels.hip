@@ -540,22 +540,39 @@ __global__ void k_fill(unsigned long long* p, uint64_t n, unsigned long long v) 
 }
 
 // per-instance offsets of the fresh lists in one flat index space, and its
-// size, computed on the device so that the resolve launch needs no host sync
-__global__ void k_fresh_prefix(const InstDesc* __restrict__ D, Seg* segs, int n, unsigned long long* total) {
-    if (threadIdx.x != 0) return;
-    unsigned long long acc = 0;
-    for (int k = 0; k < n; ++k) {
-        const InstDesc& d = D[k];
-        segs[k].inst = (uint32_t)k;
-        segs[k].begin = acc;
-        if (!d.check_only) {
-            const unsigned long long c = d.ctr->fresh_count;
-            acc += c < d.fresh_cap ? c : d.fresh_cap;
+// size, computed on the device so that the resolve launch needs no host sync;
+// one workgroup scans 256 instances at a time
+__global__ __launch_bounds__(256) void k_fresh_prefix(const InstDesc* __restrict__ D, Seg* segs, int n,
+                                                      unsigned long long* total) {
+    __shared__ unsigned long long wsum[4];
+    __shared__ unsigned long long carry;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 256) {
+        const int k = base + (int)threadIdx.x;
+        unsigned long long c = 0;
+        if (k < n && !D[k].check_only) {
+            const unsigned long long f = D[k].ctr->fresh_count;
+            c = f < D[k].fresh_cap ? f : D[k].fresh_cap;
         }
+        const unsigned long long inc = wave_incl_scan(c);
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        unsigned long long pre = carry;
+        for (int q = 0; q < w; ++q) pre += wsum[q];
+        if (k < n) {
+            segs[k].inst = (uint32_t)k;
+            segs[k].begin = pre + inc - c;
+        }
+        __syncthreads();
+        if (threadIdx.x == 255) carry = pre + inc;
+        __syncthreads();
     }
-    *total = acc;
+    if (threadIdx.x == 0) *total = carry;
 }
 
+// Winner test of fresh-list entry e: the candidate still holds the lowest
 __global__ __launch_bounds__(256) void k_resolve(const InstDesc* __restrict__ D, const Seg* __restrict__ segs,
                                                  int nseg, const unsigned long long* __restrict__ total_p,
                                                  int single) {
@@ -718,7 +735,7 @@ hipError_t launch_fill(unsigned long long* p, uint64_t n, unsigned long long v, 
 }
 
 hipError_t launch_fresh_prefix(const InstDesc* D, Seg* segs, int n, unsigned long long* total, hipStream_t st) {
-    hipLaunchKernelGGL(k_fresh_prefix, dim3(1), dim3(64), 0, st, D, segs, n, total);
+    hipLaunchKernelGGL(k_fresh_prefix, dim3(1), dim3(256), 0, st, D, segs, n, total);
     return hipGetLastError();
 }
 
