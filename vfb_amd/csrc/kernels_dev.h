@@ -45,6 +45,10 @@ __device__ __forceinline__ uint64_t add64(gu64* p, uint64_t v) {
 __device__ __forceinline__ void or32(gu32* p, uint32_t v) {
     (void)__hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void st_dev(gu64* p, uint64_t v) {
+    // a relaxed store at agent scope: seen by ld_dev() on every XCD
+    __hip_atomic_store(p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __device__ __forceinline__ uint64_t dmix64(uint64_t x) {
     // SplitMix64 finaliser (LTL_BS.cu:80-83)
@@ -163,12 +167,16 @@ __device__ __forceinline__ uint64_t ld_dev(const gu64* p) {
 enum { PR_DUP = 0, PR_FRESH = 1, PR_FAIL = 2 };
 
 // Insert-or-find of a 128-bit key with lowest-ordinal tracking, lock-free:
-// hi and lo are each claimed by a 64-bit CAS from EMPTY; a prober decides on
-// the values the CAS returns, so every prober of a key stops at the same slot.
-// Keys from earlier epochs carry ord < epoch_base and are never modified.
-// The first slot's fields (th, tl, to) are passed in, loaded ahead by the
-// caller.  PR_FRESH only for a candidate that may still hold the key's lowest
-// ordinal: one that sees a smaller ordinal of this epoch has certainly lost.
+// a slot is claimed by a 64-bit CAS of hi from EMPTY; the claimer alone then
+// writes lo (an agent-scope store, no CAS) and lowers ord.  A prober that
+// finds hi equal to its own and lo still EMPTY reads lo again until the
+// claimer's store shows (the claimer writes it right after its CAS, with no
+// wait in between; the re-reads count against kProbeLimit).  Every prober of
+// a key thus stops at the same slot.  Keys from earlier epochs carry
+// ord < epoch_base and are never modified.  The first slot's fields
+// (th, tl, to) are passed in, loaded ahead by the caller.  PR_FRESH only for
+// a candidate that may still hold the key's lowest ordinal: one that sees a
+// smaller ordinal of this epoch has certainly lost.
 __device__ __forceinline__ int probe_from(GLB Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo, uint64_t gord,
                                           uint64_t epoch_base, uint64_t i, uint64_t th, uint64_t tl,
                                           uint64_t to, uint32_t* slot_out, int* claimed) {
@@ -176,19 +184,22 @@ __device__ __forceinline__ int probe_from(GLB Slot* tab, uint64_t mask, uint64_t
         GLB Slot* s = tab + i;
         if (th == kEmpty) {
             th = cas64(&s->hi, kEmpty, hi);
-            if (th == kEmpty) { th = hi; *claimed += 1; }
+            if (th == kEmpty) {
+                th = hi;
+                tl = lo;
+                st_dev(&s->lo, lo);
+                *claimed += 1;
+            }
         }
-        if (th == hi) {
-            if (tl == kEmpty) {
-                tl = cas64(&s->lo, kEmpty, lo);
-                if (tl == kEmpty) tl = lo;
-            }
-            if (tl == lo) {
-                *slot_out = (uint32_t)i;
-                if (to < gord) return PR_DUP;      // older key, or a smaller ordinal of this epoch
-                const uint64_t old = min64(&s->ord, gord);
-                return old < gord ? PR_DUP : PR_FRESH;
-            }
+        if (th == hi && tl == kEmpty) {
+            tl = ld_dev(&s->lo);          // the claimer's lo is on its way
+            continue;
+        }
+        if (th == hi && tl == lo) {
+            *slot_out = (uint32_t)i;
+            if (to < gord) return PR_DUP;      // older key, or a smaller ordinal of this epoch
+            const uint64_t old = min64(&s->ord, gord);
+            return old < gord ? PR_DUP : PR_FRESH;
         }
         i = (i + 1) & mask;
         s = tab + i;
@@ -226,10 +237,11 @@ __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t* tot
 }
 
 // Probes the candidates of this lane (up to 4) as four interleaved state
-// machines -- per slot visited, one round of hi-CAS, one of lo-CAS and one of
-// atomicMin, each round issuing the atomics of all four candidates together so
-// that their latencies overlap -- then appends the fresh ones with one atomic
-// per wavefront.  Per-candidate semantics are those of probe_from().
+// machines -- per slot visited, one round of hi-CAS (a claimer stores lo right
+// away) and one of atomicMin, each round issuing the atomics of all four
+// candidates together so that their latencies overlap -- then appends the
+// fresh ones with one atomic per wavefront.  Per-candidate semantics are those
+// of probe_from().
 __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* key, const uint64_t* ord,
                                                   uint32_t mismatch, uint32_t enable, int lane) {
     GLB Slot* tab = glob(d.table);
@@ -254,15 +266,15 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if (((act >> q) & 1u) && th[q] == kEmpty) {
-                if (r[q] == kEmpty) { th[q] = key[q].hi; ++claimed; } else th[q] = r[q];
+                if (r[q] == kEmpty) {
+                    th[q] = key[q].hi;
+                    tl[q] = key[q].lo;
+                    st_dev(&tab[idx[q]].lo, key[q].lo);
+                    ++claimed;
+                } else {
+                    th[q] = r[q];
+                }
             }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (((act >> q) & 1u) && th[q] == key[q].hi && tl[q] == kEmpty)
-                r[q] = cas64(&tab[idx[q]].lo, kEmpty, key[q].lo);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (((act >> q) & 1u) && th[q] == key[q].hi && tl[q] == kEmpty) tl[q] = r[q] == kEmpty ? key[q].lo : r[q];
         uint32_t need_min = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -283,11 +295,15 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
             }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if ((act >> q) & 1u) {                      // mismatch: next slot
-                idx[q] = (idx[q] + 1) & mask;
-                th[q] = ld_any(&tab[idx[q]].hi);
-                tl[q] = ld_any(&tab[idx[q]].lo);
-                to[q] = ld_dev(&tab[idx[q]].ord);
+            if ((act >> q) & 1u) {
+                if (th[q] == key[q].hi && tl[q] == kEmpty) {
+                    tl[q] = ld_dev(&tab[idx[q]].lo);   // the claimer's lo is on its way
+                } else {                                 // mismatch: next slot
+                    idx[q] = (idx[q] + 1) & mask;
+                    th[q] = ld_any(&tab[idx[q]].hi);
+                    tl[q] = ld_any(&tab[idx[q]].lo);
+                    to[q] = ld_dev(&tab[idx[q]].ord);
+                }
             }
     }
     // one fresh-list reservation and one claims update per wavefront
