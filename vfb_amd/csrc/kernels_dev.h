@@ -167,32 +167,34 @@ __device__ __forceinline__ uint64_t ld_dev(const gu64* p) {
 enum { PR_DUP = 0, PR_FRESH = 1, PR_FAIL = 2 };
 
 // Insert-or-find of a 128-bit key with lowest-ordinal tracking, lock-free:
-// a slot is claimed by a 64-bit CAS of hi from EMPTY; the claimer alone then
-// writes lo (an agent-scope store, no CAS) and lowers ord.  A prober that
-// finds hi equal to its own and lo still EMPTY reads lo again until the
-// claimer's store shows (the claimer writes it right after its CAS, with no
-// wait in between; the re-reads count against kProbeLimit).  Every prober of
-// a key thus stops at the same slot.  Keys from earlier epochs carry
-// ord < epoch_base and are never modified.  The first slot's fields
-// (th, tl, to) are passed in, loaded ahead by the caller.  PR_FRESH only for
-// a candidate that may still hold the key's lowest ordinal: one that sees a
-// smaller ordinal of this epoch has certainly lost.
+// a slot is claimed by a 64-bit CAS of its ord from EMPTY to the candidate's
+// ordinal; the claimer alone then writes hi and lo (agent-scope stores, no
+// wait) and is done.  A prober that finds ord taken and hi or lo still EMPTY
+// reads them again until the claimer's stores show (the re-reads count
+// against kProbeLimit); with both shown it compares the key, and on a match
+// lowers ord.  Every prober of a key thus stops at the same slot.  Keys from
+// earlier epochs carry ord < epoch_base and are never modified.  The first
+// slot's fields (th, tl, to) are passed in, loaded ahead by the caller.
+// PR_FRESH only for a candidate that may still hold the key's lowest
+// ordinal: one that sees a smaller ordinal of this epoch has certainly lost.
 __device__ __forceinline__ int probe_from(GLB Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo, uint64_t gord,
                                           uint64_t epoch_base, uint64_t i, uint64_t th, uint64_t tl,
                                           uint64_t to, uint32_t* slot_out, int* claimed) {
     for (int n = 0; n < kProbeLimit; ++n) {
         GLB Slot* s = tab + i;
-        if (th == kEmpty) {
-            th = cas64(&s->hi, kEmpty, hi);
-            if (th == kEmpty) {
-                th = hi;
-                tl = lo;
+        if (to == kEmpty) {
+            to = cas64(&s->ord, kEmpty, gord);
+            if (to == kEmpty) {
+                st_dev(&s->hi, hi);
                 st_dev(&s->lo, lo);
                 *claimed += 1;
+                *slot_out = (uint32_t)i;
+                return PR_FRESH;
             }
         }
-        if (th == hi && tl == kEmpty) {
-            tl = ld_dev(&s->lo);          // the claimer's lo is on its way
+        if (th == kEmpty || tl == kEmpty) {
+            th = ld_dev(&s->hi);          // the claimer's key is on its way
+            tl = ld_dev(&s->lo);
             continue;
         }
         if (th == hi && tl == lo) {
@@ -237,9 +239,9 @@ __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t* tot
 }
 
 // Probes the candidates of this lane (up to 4) as four interleaved state
-// machines -- per slot visited, one round of hi-CAS (a claimer stores lo right
-// away) and one of atomicMin, each round issuing the atomics of all four
-// candidates together so that their latencies overlap -- then appends the
+// machines -- per slot visited, one round of ord-CAS (a claimer stores its key
+// and is done) and one of atomicMin, each round issuing the atomics of all
+// four candidates together so that their latencies overlap -- then appends the
 // fresh ones with one atomic per wavefront.  Per-candidate semantics are those
 // of probe_from().
 __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* key, const uint64_t* ord,
@@ -262,17 +264,18 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
         uint64_t r[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (((act >> q) & 1u) && th[q] == kEmpty) r[q] = cas64(&tab[idx[q]].hi, kEmpty, key[q].hi);
+            if (((act >> q) & 1u) && to[q] == kEmpty) r[q] = cas64(&tab[idx[q]].ord, kEmpty, d.epoch_base + ord[q]);
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (((act >> q) & 1u) && th[q] == kEmpty) {
-                if (r[q] == kEmpty) {
-                    th[q] = key[q].hi;
-                    tl[q] = key[q].lo;
+            if (((act >> q) & 1u) && to[q] == kEmpty) {
+                if (r[q] == kEmpty) {                   // claimed: the key goes in, fresh
+                    st_dev(&tab[idx[q]].hi, key[q].hi);
                     st_dev(&tab[idx[q]].lo, key[q].lo);
                     ++claimed;
+                    fresh |= 1u << q;
+                    act &= ~(1u << q);
                 } else {
-                    th[q] = r[q];
+                    to[q] = r[q];
                 }
             }
         uint32_t need_min = 0;
@@ -296,9 +299,10 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if ((act >> q) & 1u) {
-                if (th[q] == key[q].hi && tl[q] == kEmpty) {
-                    tl[q] = ld_dev(&tab[idx[q]].lo);   // the claimer's lo is on its way
-                } else {                                 // mismatch: next slot
+                if (th[q] == kEmpty || tl[q] == kEmpty) {   // the claimer's key is on its way
+                    th[q] = ld_dev(&tab[idx[q]].hi);
+                    tl[q] = ld_dev(&tab[idx[q]].lo);
+                } else {                                     // another key: next slot
                     idx[q] = (idx[q] + 1) & mask;
                     th[q] = ld_any(&tab[idx[q]].hi);
                     tl[q] = ld_any(&tab[idx[q]].lo);
