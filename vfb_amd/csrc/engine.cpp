@@ -1016,6 +1016,8 @@ void Engine::start_bs(Inst& I) {
     B->ltl_levels = L;
     std::memset(&B->h, 0, sizeof(B->h));
     key_params(*B->bs_in, 3, B->h);      // one trace of T bits: the exact key is the word
+    if (B->h.key_kind != KEY_FIELD)      // the expansion checks BS rows only with field keys
+        throw std::logic_error("BS stage without an exact field key");
     B->h.T = 1;
     B->h.C = 1;
     B->h.P = 0;

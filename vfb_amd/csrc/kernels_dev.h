@@ -398,6 +398,62 @@ __device__ __forceinline__ uint32_t in_window(const InstDesc& d, const uint64_t*
     return m;
 }
 
+// Per-trace accumulation of one candidate's CS word c.  KEY_MIX keys are
+// the XOR of dmix64(word) over the traces below InstDesc::half (hi) and over
+// the rest (lo): the loops run those two trace ranges one after the other, so
+// a word folds into the running half `acc` with no per-trace select.  The
+// other keys place the word's bits by trace (key_acc).  The target check
+// keeps the OR of (c ^ want): bit 0 decides a miss (label at time 0), every
+// bit in the BS stage (the word against the positives mask).
+// (The BS stage's one-word rows have at most 63 bits and so exact field keys:
+// a KEY_MIX instance never has bs_mode, and its check needs the low word only.)
+template <int KEY>
+__device__ __forceinline__ void fold_word(Key& k, uint64_t& acc, uint64_t& mq, uint64_t c, uint64_t want,
+                                          const InstDesc& d, int t) {
+    if constexpr (KEY == KEY_MIX) {
+        mq |= (uint64_t)((uint32_t)c ^ (uint32_t)want);
+        acc ^= dmix64(c);
+    } else {
+        mq |= c ^ want;
+        key_acc<KEY>(k, c, d, t);
+    }
+}
+
+// First trace of the second range (lo half) of a KEY_MIX fold; T (one range)
+// otherwise
+template <int KEY, bool CHECK>
+__device__ __forceinline__ int trace_split(const InstDesc& d) {
+    if constexpr (KEY == KEY_MIX && !CHECK) return d.half < 0 ? 0 : (d.half < d.T ? d.half : d.T);
+    return d.T;
+}
+
+// End of the first range: its running half is the key's hi, and the lane's
+// next trace (t = sub mod G) is the first at or after tmid
+template <int KEY, int G>
+__device__ __forceinline__ void next_split(Key* k, uint64_t* acc, int& tb, int tmid, int sub) {
+    if constexpr (KEY == KEY_MIX) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            k[q].hi = acc[q];
+            acc[q] = 0;
+        }
+    }
+    tb = tmid + ((sub - tmid) & (G - 1));
+}
+
+// The lo half of a KEY_MIX key, and the miss bits of the four candidates
+template <int KEY>
+__device__ __forceinline__ uint32_t fold_end(Key* k, const uint64_t* acc, const uint64_t* mq, const InstDesc& d) {
+    const uint64_t fm = KEY != KEY_MIX && d.bs_mode ? ~0ull : 1ull;
+    uint32_t mm = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if constexpr (KEY == KEY_MIX) k[q].lo = acc[q];
+        mm |= ((mq[q] & fm) != 0ull ? 1u : 0u) << q;
+    }
+    return mm;
+}
+
 // Combines the per-lane partial keys / mismatch bits of the G lanes that
 // share a candidate (G > 1: each lane folded every G-th trace), then probes
 // (or, in a check-only level, checks) the four candidates.
@@ -461,44 +517,45 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
     // sees (and every other lane's) has missed one
     const uint32_t dead = CHECK ? (active ? (~(uint32_t)tk.part & 0xfu) : 0xfu) : 0u;
     const int T = d.T;
-    for (int t0 = sub; t0 < T; t0 += G * kTraceGroup) {
-        if constexpr (CHECK) {
-            if (!__ballot((mm | dead) != 0xfu)) break;
-        }
-        uint64_t av[kTraceGroup];
-        // (a check visits the traces in check_order; keys need them in order)
-#pragma unroll
-        for (int j = 0; j < kTraceGroup; ++j)
-            if (t0 + j * G < T) av[j] = L[(uint64_t)(CHECK ? d.check_order[t0 + j * G] : t0 + j * G) * tk.nl + xr];
-#pragma unroll
-        for (int j = 0; j < kTraceGroup; ++j) {
-            if (t0 + j * G >= T) break;
-            const int t = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
-            const uint64_t a = av[j];
-            const uint64_t m = d.lenmask[t];
-            const uint64_t want = d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
-            const uint64_t cmask = d.bs_mode ? ~0ull : 1ull;
+    // the running key half of KEY_MIX and the OR of (word ^ target) per candidate
+    uint64_t acc[4] = {0, 0, 0, 0}, mq[4] = {0, 0, 0, 0};
+    const int tmid = trace_split<KEY, CHECK>(d);
+    int tb = sub;
+    for (int sg = 0; sg < (tmid < T ? 2 : 1); ++sg) {
+        const int te = sg == 0 ? tmid : T;
+        for (int t0 = tb; t0 < te; t0 += G * kTraceGroup) {
             if constexpr (CHECK) {
-                // only bit 0 (time 0) matters: closed forms of the operators there
-                mm |= bit0_mismatch(~a & 1ull, (a >> 1) & 1ull, a != 0, a == m, want);
-                continue;
+                if (!__ballot((mm | dead) != 0xfu)) break;
             }
-            const uint64_t c0 = ~a & m;                      // ~
-            if constexpr ((SEL & 1) != 0) {
-                mm |= miss(c0, want, cmask);
-                key_acc<KEY>(k[0], c0, d, t);
-            }
-            if constexpr ((SEL & 0xe) != 0) {
-                const uint64_t c1 = a >> 1;                  // X
-                const uint64_t c2 = suffix_or(a);            // F
-                const uint64_t c3 = a & ~suffix_or(c0);      // G
-                mm |= (miss(c1, want, cmask) << 1) | (miss(c2, want, cmask) << 2) | (miss(c3, want, cmask) << 3);
-                key_acc<KEY>(k[1], c1, d, t);
-                key_acc<KEY>(k[2], c2, d, t);
-                key_acc<KEY>(k[3], c3, d, t);
+            uint64_t av[kTraceGroup];
+            // (a check visits the traces in check_order; keys need them in order)
+#pragma unroll
+            for (int j = 0; j < kTraceGroup; ++j)
+                if (t0 + j * G < te) av[j] = L[(uint64_t)(CHECK ? d.check_order[t0 + j * G] : t0 + j * G) * tk.nl + xr];
+#pragma unroll
+            for (int j = 0; j < kTraceGroup; ++j) {
+                if (t0 + j * G >= te) break;
+                const int t = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
+                const uint64_t a = av[j];
+                const uint64_t m = d.lenmask[t];
+                const uint64_t want = (KEY != KEY_MIX || CHECK) && d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
+                if constexpr (CHECK) {
+                    // only bit 0 (time 0) matters: closed forms of the operators there
+                    mm |= bit0_mismatch(~a & 1ull, (a >> 1) & 1ull, a != 0, a == m, want);
+                    continue;
+                }
+                const uint64_t c0 = ~a & m;                      // ~
+                if constexpr ((SEL & 1) != 0) fold_word<KEY>(k[0], acc[0], mq[0], c0, want, d, t);
+                if constexpr ((SEL & 0xe) != 0) {
+                    fold_word<KEY>(k[1], acc[1], mq[1], a >> 1, want, d, t);                // X
+                    fold_word<KEY>(k[2], acc[2], mq[2], suffix_or(a), want, d, t);          // F
+                    fold_word<KEY>(k[3], acc[3], mq[3], a & ~suffix_or(c0), want, d, t);    // G
+                }
             }
         }
+        if (sg == 0) next_split<KEY, G>(k, acc, tb, tmid, sub);
     }
+    if constexpr (!CHECK) mm = fold_end<KEY>(k, acc, mq, d);
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
     // SEL_ALL also serves any other mask (the cost function's): tk.part enables
     const uint32_t win = CHECK ? 0xfu : in_window(d, ord);
@@ -536,49 +593,45 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     // target checks only: early exit as in expand_unary
     const uint32_t dead = CHECK ? (active ? (~(uint32_t)tk.part & 0xfu) : 0xfu) : 0u;
     const int T = d.T;
-    for (int t0 = sub; t0 < T; t0 += G * kTraceGroup) {
-        if constexpr (CHECK) {
-            if (!__ballot((mm | dead) != 0xfu)) break;
-        }
-        uint64_t av[kTraceGroup], bv[kTraceGroup];
-#pragma unroll
-        for (int j = 0; j < kTraceGroup; ++j)
-            if (t0 + j * G < T) {
-                const uint64_t tt = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
-                av[j] = L[tt * tk.nl + l];
-                bv[j] = R[tt * tk.nr + rr];
-            }
-#pragma unroll
-        for (int j = 0; j < kTraceGroup; ++j) {
-            if (t0 + j * G >= T) break;
-            const int t = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
-            const uint64_t a = av[j], b = bv[j];
-            const uint64_t want = d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
-            const uint64_t cmask = d.bs_mode ? ~0ull : 1ull;
+    uint64_t acc[4] = {0, 0, 0, 0}, mq[4] = {0, 0, 0, 0};
+    const int tmid = trace_split<KEY, CHECK>(d);
+    int tb = sub;
+    for (int sg = 0; sg < (tmid < T ? 2 : 1); ++sg) {
+        const int te = sg == 0 ? tmid : T;
+        for (int t0 = tb; t0 < te; t0 += G * kTraceGroup) {
             if constexpr (CHECK) {
-                // the probe and check-only levels evaluate every operator
-                mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
-                continue;
+                if (!__ballot((mm | dead) != 0xfu)) break;
             }
-            if constexpr ((SEL & 1) != 0) {
-                const uint64_t c0 = a & b;
-                mm |= miss(c0, want, cmask);
-                key_acc<KEY>(k[0], c0, d, t);
-            }
-            if constexpr ((SEL & 2) != 0) {
-                const uint64_t c1 = a | b;
-                mm |= miss(c1, want, cmask) << 1;
-                key_acc<KEY>(k[1], c1, d, t);
-            }
-            if constexpr ((SEL & 0xc) != 0) {
-                const uint64_t c2 = until_op(a, b);
-                const uint64_t c3 = until_op(b, a);
-                mm |= (miss(c2, want, cmask) << 2) | (miss(c3, want, cmask) << 3);
-                key_acc<KEY>(k[2], c2, d, t);
-                key_acc<KEY>(k[3], c3, d, t);
+            uint64_t av[kTraceGroup], bv[kTraceGroup];
+#pragma unroll
+            for (int j = 0; j < kTraceGroup; ++j)
+                if (t0 + j * G < te) {
+                    const uint64_t tt = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
+                    av[j] = L[tt * tk.nl + l];
+                    bv[j] = R[tt * tk.nr + rr];
+                }
+#pragma unroll
+            for (int j = 0; j < kTraceGroup; ++j) {
+                if (t0 + j * G >= te) break;
+                const int t = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
+                const uint64_t a = av[j], b = bv[j];
+                const uint64_t want = (KEY != KEY_MIX || CHECK) && d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
+                if constexpr (CHECK) {
+                    // the probe and check-only levels evaluate every operator
+                    mm |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
+                    continue;
+                }
+                if constexpr ((SEL & 1) != 0) fold_word<KEY>(k[0], acc[0], mq[0], a & b, want, d, t);
+                if constexpr ((SEL & 2) != 0) fold_word<KEY>(k[1], acc[1], mq[1], a | b, want, d, t);
+                if constexpr ((SEL & 0xc) != 0) {
+                    fold_word<KEY>(k[2], acc[2], mq[2], until_op(a, b), want, d, t);
+                    fold_word<KEY>(k[3], acc[3], mq[3], until_op(b, a), want, d, t);
+                }
             }
         }
+        if (sg == 0) next_split<KEY, G>(k, acc, tb, tmid, sub);
     }
+    if constexpr (!CHECK) mm = fold_end<KEY>(k, acc, mq, d);
     const uint64_t pair = l * tk.nr + r;
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
