@@ -307,7 +307,11 @@ typedef struct {
                                       (and at most 2^18), its solve has unit costs and
                                       one-word traces, and its hash set need not grow;
                                       64..65536 (default 1024) */
-    uint32_t pad_tuning;
+    uint32_t lds_chain;            /* 1: the smallest levels of a solve (at most 4096
+                                      candidates, its set at most 2048 keys with all of them
+                                      new) are built one after another by one workgroup with
+                                      the solve's whole hash set in LDS, before any other
+                                      path; 0: off (default 1) */
 } vfb_tuning;
 
 void vfb_tuning_default(vfb_tuning* t);

@@ -53,7 +53,8 @@ struct ChainShared {
 // Units of level s in the canonical order of the host's build_units (unit
 // costs: ~, & splits, | splits, X, F, G, U splits; LTL_BS.cu:1236-1488) and
 // the fused tasks over them, with kWideLanes lanes per candidate.
-__device__ void chain_plan(ChainShared& sh, int s, int neg_type) {
+template <class SH>
+__device__ void chain_plan(SH& sh, int s, int neg_type) {
     InstDesc& d = sh.d;
     uint64_t base = 0;
     int nu = 0;
@@ -202,7 +203,8 @@ __device__ void chain_scan(ChainShared& sh, uint32_t nw, uint64_t found) {
 // the level block (column-major, out_n = W rows); slice 0 also writes the
 // parent ordinals.  The store_item of kernels_dev.h with the chunk's bits in
 // LDS and a per-lane select instead of a list.
-__device__ __forceinline__ void chain_store_item(const ChainShared& sh, uint32_t c, uint32_t part, int t0, int t1,
+template <class SH>
+__device__ __forceinline__ void chain_store_item(const SH& sh, uint32_t c, uint32_t part, int t0, int t1,
                                                  gu64* out_rows, gu64* out_par, uint64_t out_n, uint16_t* wpre,
                                                  int lane) {
     const InstDesc& d = sh.d;
@@ -230,7 +232,7 @@ __device__ __forceinline__ void chain_store_item(const ChainShared& sh, uint32_t
         const uint64_t o = (uint64_t)c * 4096 + (uint64_t)lo * 64 + (uint64_t)__ffsll((unsigned long long)b) - 1;
         const Cand cd = decode_from(d, o, unit_of(d, (uint64_t)c * 4096));
         const uint64_t row = sh.chunk_pre[c] + j;
-        for (int t = t0; t < t1; ++t) out_rows[(uint64_t)t * out_n + row] = cand_word(cd, t, d.lenmask[t]);
+        cand_words_out(cd, t0, t1, d.lenmask, out_rows, out_n, row);   // loads of 8 traces issued together
         if (t0 == 0) out_par[row] = o;
     }
     __builtin_amdgcn_wave_barrier();
@@ -506,6 +508,390 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
         fresh_ratio = (double)fc / (double)(N > 1 ? N : 1);
         level = s;
     }
+}
+
+// ---------------------------------------------------------------------------
+// The level chain in LDS (k_chain_lds): one workgroup per instance builds the
+// smallest levels with the instance's whole hash set in its LDS.  A level of
+// a few hundred candidates is bound by the latency of its dependent memory
+// operations; in the device-wide set each candidate waits on global loads and
+// atomics (slot, claim, fresh-list append), here on LDS ones, and the resolve
+// needs no fresh list: a slot holding a key of the level holds its lowest
+// ordinal, the survivor.  At the start the set is filled with the keys of the
+// instance's stored rows (levels 1..level), recomputed from their words --
+// every key of the device set, whose count `committed` the workgroup checks;
+// at the end the keys of the levels it completed go into the device set,
+// each claiming an empty slot, with the ordinal the host path would have left
+// there.  A level enters only if its candidates, and the set's keys with all
+// of them new, fit (kLCands, kLKeys), under the same admission rules as
+// k_chain; a level whose new keys pass the device set's load limit or the
+// host's fresh list is left to the host (CH_REDO), its keys not written back.
+// ---------------------------------------------------------------------------
+constexpr int kLThreads = 512;                  // threads of an LDS chain workgroup (8 wavefronts)
+constexpr int kLSlots = kLdsChainSlots;         // LDS set slots (24 B each)
+constexpr uint64_t kLKeys = kLdsChainKeys;
+constexpr uint64_t kLCands = kLdsChainCands;
+static_assert(kLCands <= 4096, "an LDS level is one 4096-ordinal chunk");
+
+struct LSlot {
+    unsigned long long hi, lo, ord;
+};
+
+struct LdsShared {
+    InstDesc d;
+    Task tasks[kMaxTasks];
+    LSlot set[kLSlots];
+    uint32_t bm[kChunkW32];        // the level's winners (bit = level-local ordinal)
+    uint32_t mbm[kChunkW32];       // its candidates with no target mismatch
+    uint32_t chunk_pre[2];         // survivors before the level's one chunk, and in it
+    uint16_t wpre[kLThreads / 64][64];
+    uint64_t N, waves;
+    uint32_t claims, fresh, ovf, found, nkeys, W, found_rank;
+    int ntasks;
+};
+
+__shared__ LdsShared g_lsh;
+
+__device__ __forceinline__ unsigned long long lds_ld(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_st(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ unsigned long long lds_cas(unsigned long long* p, unsigned long long e,
+                                                      unsigned long long v) {
+    __hip_atomic_compare_exchange_strong(p, &e, v, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return e;
+}
+
+// Insert-or-find in the LDS set with the device set's protocol (probe_from):
+// a slot is claimed by a CAS of its ord, after which the claimer alone writes
+// hi and lo; a prober that finds ord taken and the key not yet shown reads it
+// again on its next step; a match lowers ord.  Returns PR_FRESH for a claim
+// or a lowered ord, PR_DUP otherwise, PR_FAIL past the probe limit.  Every
+// lane leaves the loop at its end only, so a claimer's stores are issued in
+// the step of its claim even while a lane of its own wavefront waits on them
+// (with an exit from inside the loop, the waiting lanes' path may run first
+// and spin while the claimer's is held back to the loop's exit).
+__device__ __forceinline__ int lds_probe(LSlot* set, unsigned long long hi, unsigned long long lo,
+                                         unsigned long long gord, bool* claimed) {
+    uint32_t i = (uint32_t)slot_hash(hi, lo) & (kLSlots - 1);
+    unsigned long long to = lds_ld(&set[i].ord), th = lds_ld(&set[i].hi), tl = lds_ld(&set[i].lo);
+    int r = PR_FAIL;
+    bool done = false;
+    for (int n = 0; n < kProbeLimit && !done; ++n) {
+        if (to == kEmpty) {
+            to = lds_cas(&set[i].ord, kEmpty, gord);
+            if (to == kEmpty) {
+                lds_st(&set[i].hi, hi);
+                lds_st(&set[i].lo, lo);
+                *claimed = true;
+                r = PR_FRESH;
+                done = true;
+            }
+        }
+        if (done) {
+        } else if (th == kEmpty || tl == kEmpty) {   // the claimer's key is on its way
+            th = lds_ld(&set[i].hi);
+            tl = lds_ld(&set[i].lo);
+        } else if (th == hi && tl == lo) {
+            r = to < gord || __hip_atomic_fetch_min(&set[i].ord, gord, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP) < gord
+                    ? PR_DUP : PR_FRESH;
+            done = true;
+        } else {
+            i = (i + 1) & (kLSlots - 1);
+            to = lds_ld(&set[i].ord);
+            th = lds_ld(&set[i].hi);
+            tl = lds_ld(&set[i].lo);
+        }
+    }
+    return r;
+}
+
+// The expansion's per-candidate end in the LDS chain (finish_one's counters
+// and flags, kept in LDS): a candidate with no mismatch marks its ordinal.
+struct LdsSink {
+    static constexpr bool kGlobal = false;
+    static __device__ __forceinline__ void finish(const InstDesc& d, const Key& key, uint64_t ord, uint32_t mismatch,
+                                                  bool en, int lane) {
+        (void)lane;
+        if (!en) return;
+        LdsShared& sh = g_lsh;
+        if (!mismatch) atomicOr(&sh.mbm[ord >> 5], 1u << (ord & 31));
+        bool claimed = false;
+        const int r = lds_probe(sh.set, key.hi, key.lo, d.epoch_base + ord, &claimed);
+        if (claimed) atomicAdd(&sh.claims, 1u);
+        if (r == PR_FRESH) atomicAdd(&sh.fresh, 1u);
+        if (r == PR_FAIL) atomicOr(&sh.ovf, 1u);
+    }
+};
+
+// the key of a stored row (levels 1..level), as the expansion folded it
+template <int KEY>
+__device__ __forceinline__ Key row_key(const InstDesc& d, const gu64* rows, uint64_t nrows, uint64_t row) {
+    Key k{0, 0};
+    for (int t = 0; t < d.T; ++t) key_acc<KEY>(k, rows[(uint64_t)t * nrows + row], d, t);
+    key_fin(k);
+    return k;
+}
+
+__global__ __launch_bounds__(kLThreads) void k_chain_lds(ChainIn* ins, ChainOut* outs) {
+    LdsShared& sh = g_lsh;
+    ChainIn& io = ins[blockIdx.x];
+    ChainOut& out = outs[blockIdx.x];
+    InstDesc& d = sh.d;
+    const int tid = (int)threadIdx.x, lane = tid & 63;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr uint32_t kWaves = kLThreads / 64;
+    {
+        const uint64_t* src = (const uint64_t*)&io.d;
+        uint64_t* dst = (uint64_t*)&d;
+        for (uint32_t i = threadIdx.x; i < sizeof(InstDesc) / 8; i += kLThreads) dst[i] = src[i];
+    }
+    for (int i = tid; i < kLSlots; i += kLThreads) sh.set[i] = LSlot{kEmpty, kEmpty, kEmpty};
+    __syncthreads();
+    const bool lead = tid == 0;
+    if (lead) {
+        out.t0 = wall_clock64();
+        d.check_only = 0;
+        d.mark = 1;
+        d.bs_mode = 0;
+        d.found_below = kEmpty;
+        d.ord_lo = 0;
+        d.ord_hi = kEmpty;
+        d.ctr_host = nullptr;
+        sh.nkeys = 0;
+        sh.ovf = 0;
+    }
+    // ---- the set: keys of the stored rows, with ordinal 0 (older than any level's) ----
+    uint64_t nrows_all = 0;
+    for (int s = 1; s <= io.level; ++s) nrows_all += d.nrows[s];
+    __syncthreads();
+    if (nrows_all <= kLKeys) {
+        for (int s = 1; s <= io.level; ++s) {
+            const gu64* rows = glob((const unsigned long long*)d.lvl[s]);
+            const uint64_t n = d.nrows[s];
+            for (uint64_t r = (uint64_t)tid; r < n; r += kLThreads) {
+                Key k;
+                if (d.key_kind == KEY_MIX) k = row_key<KEY_MIX>(d, rows, n, r);
+                else if (d.key_kind == KEY_FIELD) k = row_key<KEY_FIELD>(d, rows, n, r);
+                else k = row_key<KEY_STRIDE>(d, rows, n, r);
+                bool claimed = false;
+                if (lds_probe(sh.set, k.hi, k.lo, 0, &claimed) == PR_FAIL) atomicOr(&sh.ovf, 1u);
+                if (claimed) atomicAdd(&sh.nkeys, 1u);
+            }
+        }
+    }
+    __syncthreads();
+    // the rows' keys are the device set's only if there are as many
+    if (nrows_all > kLKeys || sh.ovf || sh.nkeys != io.committed) {
+        if (lead) {
+            out.exit = CH_BEFORE;
+            out.nlev = 0;
+        }
+        return;
+    }
+
+    int level = io.level;
+    uint64_t committed = io.committed, gbase = io.gbase, rows_left = io.rows_left, used = 0;
+    const uint64_t gbase0 = io.gbase;
+    uint64_t host_fcap = io.fresh_cap;
+    double win_ratio = io.win_ratio, fresh_ratio = io.fresh_ratio;
+    const uint64_t load_limit = io.cap * io.pct / 100;
+    for (int k = 0;; ++k) {
+        const int s = level + 1;
+        __syncthreads();
+        if (tid == 0) chain_plan(sh, s, io.neg_type);
+        __syncthreads();
+        const uint64_t N = sh.N;
+        const uint64_t row_bytes = 8 * (uint64_t)d.T + 8;
+        const uint64_t nbytes = (N * row_bytes + 255) & ~255ull;
+        // admission (alike in every thread): k_chain's, and the LDS set's room
+        bool stop = s > io.last_level || k >= kMaxS || N == 0 || N > kLCands || N > io.n_max || N > rows_left ||
+                    committed + N > kLKeys || used + (nbytes < 256 ? 256 : nbytes) > io.slab_bytes ||
+                    (io.pass_min && N >= io.pass_min && s >= 3);
+        if (!stop) {
+            const double f = win_ratio * 1.3;
+            const double m = f < 1.0 ? f : 1.0;
+            uint64_t est = (uint64_t)((double)N * m);
+            if (est < 4096) est = 4096;
+            if (est > N) est = N;
+            stop = (committed + est) * 100 / io.pct + 1 > io.cap;
+        }
+        if (stop) {
+            if (lead) {
+                out.exit = CH_BEFORE;
+                out.nlev = k;
+            }
+            break;
+        }
+        {
+            const double f = fresh_ratio * 1.5;
+            uint64_t fcap = (uint64_t)((double)N * (f < 1.0 ? f : 1.0)) + 4096;
+            if (fcap > N) fcap = N;
+            if (fcap > host_fcap) host_fcap = fcap;
+        }
+        if (lead) out.t[k][0] = wall_clock64();
+        if (tid == 0) {
+            d.level = s;
+            d.epoch_base = gbase;
+            sh.claims = 0;
+            sh.fresh = 0;
+            sh.found = ~0u;
+        }
+        for (int i = tid; i < (int)kChunkW32; i += kLThreads) {
+            sh.bm[i] = 0u;
+            sh.mbm[i] = 0u;
+        }
+        __syncthreads();
+
+        // ---- expansion: keys into the LDS set ----
+        for (uint64_t w = wv; w < sh.waves; w += kWaves) {
+            int lo = 0, hi = sh.ntasks - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sh.tasks[mid].wave_begin <= w) lo = mid; else hi = mid - 1;
+            }
+            const Task& tk = sh.tasks[lo];
+            const uint64_t wl = w - tk.wave_begin;
+            if (tk.kind == 0) {
+                if (d.key_kind == KEY_MIX) expand_unary_part<KEY_MIX, kWideLanes, LdsSink>(d, tk, wl, lane);
+                else if (d.key_kind == KEY_FIELD) expand_unary_part<KEY_FIELD, kWideLanes, LdsSink>(d, tk, wl, lane);
+                else expand_unary_part<KEY_STRIDE, kWideLanes, LdsSink>(d, tk, wl, lane);
+            } else {
+                if (d.key_kind == KEY_MIX) expand_binary_part<KEY_MIX, kWideLanes, LdsSink>(d, tk, wl, lane);
+                else if (d.key_kind == KEY_FIELD) expand_binary_part<KEY_FIELD, kWideLanes, LdsSink>(d, tk, wl, lane);
+                else expand_binary_part<KEY_STRIDE, kWideLanes, LdsSink>(d, tk, wl, lane);
+            }
+        }
+        __syncthreads();
+        if (lead) out.t[k][1] = wall_clock64();
+
+        // ---- resolve: a slot holding a key of this level names its survivor ----
+        for (int i = tid; i < kLSlots; i += kLThreads) {
+            const unsigned long long o = sh.set[i].ord;
+            if (o != kEmpty && o >= gbase) {
+                const uint32_t local = (uint32_t)(o - gbase);
+                atomicOr(&sh.bm[local >> 5], 1u << (local & 31));
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < (int)kChunkW32; i += kLThreads) {
+            const uint32_t m = sh.bm[i] & sh.mbm[i];
+            if (m) atomicMin(&sh.found, (uint32_t)i * 32u + (uint32_t)(__ffs((int)m) - 1));
+        }
+        __syncthreads();
+        if (lead) out.t[k][2] = wall_clock64();
+        // ---- survivors and the rank of a match (wavefront 0, two words per lane) ----
+        if (wv == 0) {
+            const uint32_t f = sh.found;
+            uint32_t cnt = 0, below = 0;
+            for (int j = 0; j < 2; ++j) {
+                const uint32_t x = (uint32_t)lane * 2u + (uint32_t)j;
+                const uint32_t b = sh.bm[x];
+                cnt += (uint32_t)__popc(b);
+                if (f != ~0u) {
+                    const uint32_t fw = f >> 5;
+                    if (x < fw) below += (uint32_t)__popc(b);
+                    else if (x == fw) below += (uint32_t)__popc(b & ((1u << (f & 31)) - 1u));
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                cnt += __shfl_xor(cnt, off);
+                below += __shfl_xor(below, off);
+            }
+            if (lane == 0) {
+                sh.W = cnt;
+                sh.found_rank = below;
+                sh.chunk_pre[0] = 0;
+                sh.chunk_pre[1] = cnt;
+            }
+        }
+        __syncthreads();
+        if (lead) out.t[k][3] = wall_clock64();
+        const uint64_t W = sh.W;
+        const uint64_t fc = sh.fresh;
+        const uint32_t ovf = sh.ovf | (committed + sh.claims > load_limit ? 4u : 0u) | (fc > host_fcap ? 2u : 0u);
+        if (ovf || sh.found != ~0u) {
+            if (lead) {
+                out.exit = ovf ? CH_REDO : CH_FOUND;
+                out.nlev = k;
+                out.overflow = ovf;
+                out.found = sh.found;
+                out.found_rank = sh.found_rank;
+                out.N_exit = N;
+                out.W_exit = W;
+                out.fresh_exit = fc;
+            }
+            break;
+        }
+        // ---- store: survivors in ordinal order into the slab ----
+        uint64_t bt = W ? (W * row_bytes + 255) & ~255ull : 0;
+        if (W && bt < 256) bt = 256;
+        unsigned long long* rows = io.slab + used / 8;
+        if (W) {
+            // 64 survivors per item, the traces split into as many slices
+            // as leave every wavefront an item (a slice at least 8 traces)
+            const uint32_t parts = (uint32_t)((W + 63) / 64);
+            const uint32_t most = (uint32_t)((d.T + kStoreGroup - 1) / kStoreGroup);
+            uint32_t nsl = kWaves / parts;
+            nsl = nsl < 1 ? 1 : (nsl > most ? most : nsl);
+            const int sw = (d.T + (int)nsl - 1) / (int)nsl;
+            const uint64_t items = (uint64_t)parts * nsl;
+            gu64* const orows = glob(rows);
+            gu64* const opar = glob(rows + (uint64_t)d.T * W);
+            for (uint64_t g = wv; g < items; g += kWaves) {
+                const uint32_t part = (uint32_t)(g / nsl);
+                const int t0 = (int)(g - (uint64_t)part * nsl) * sw;
+                const int t1 = t0 + sw < d.T ? t0 + sw : d.T;
+                chain_store_item(sh, 0u, part, t0, t1, orows, opar, W, sh.wpre[wv], lane);
+            }
+        }
+        __syncthreads();   // the level's rows, visible to this workgroup's next loads
+        if (lead) {
+            out.t[k][4] = wall_clock64();
+            out.N[k] = N;
+            out.W[k] = W;
+            out.fresh[k] = fc;
+            out.off[k] = used;
+        }
+        if (tid == 0) {
+            d.lvl[s] = rows;
+            d.nrows[s] = (uint32_t)W;
+        }
+        committed += W;
+        gbase += N;
+        rows_left -= W;
+        used += bt;
+        win_ratio = (double)W / (double)(N > 1 ? N : 1);
+        fresh_ratio = (double)fc / (double)(N > 1 ? N : 1);
+        level = s;
+    }
+    // ---- the completed levels' keys into the device set ----
+    __syncthreads();
+    GLB Slot* tab = glob(d.table);
+    const uint64_t mask = d.table_mask;
+    for (int i = tid; i < kLSlots; i += kLThreads) {
+        const unsigned long long o = sh.set[i].ord;
+        if (o == kEmpty || o < gbase0 || o >= gbase) continue;
+        const unsigned long long hi = sh.set[i].hi, lo = sh.set[i].lo;
+        uint64_t j = slot_hash(hi, lo) & mask;
+        int n = 0;
+        // a new key of distinct value: the first empty slot is its
+        for (; n < kProbeLimit && cas64(&tab[j].ord, kEmpty, o) != kEmpty; ++n) j = (j + 1) & mask;
+        if (n == kProbeLimit) {
+            or32(glob(&out.err), 1u);
+            continue;
+        }
+        st_dev(&tab[j].hi, hi);
+        st_dev(&tab[j].lo, lo);
+    }
+}
+
+hipError_t launch_chain_lds(ChainIn* ins, ChainOut* outs, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_chain_lds, dim3((unsigned)n), dim3(kLThreads), 0, st, ins, outs);
+    return hipGetLastError();
 }
 
 // Teams of G > 1 need every workgroup resident at once: a cooperative

@@ -1307,13 +1307,24 @@ bool Engine::chain_admits(const Inst& I, uint64_t n_max) const {
     return start <= I.cap && (I.committed + est) * 100 / tun_.max_load_pct + 1 <= I.cap;
 }
 
+// Whether the LDS level chain may build the next level: chain_admits' rules
+// with its limits (the level's candidates, and the set's keys with all of
+// them new), over rows the workgroup can key (every stored level's rows
+// together at most the set's keys).
+bool Engine::lds_chain_admits(const Inst& I) const {
+    if (!chain_admits(I, kLdsChainCands) || I.committed + I.N > kLdsChainKeys) return false;
+    uint64_t rows = 0;
+    for (int s = 1; s <= I.level; ++s) rows += I.nrows[s];
+    return rows <= kLdsChainKeys;
+}
+
 // One launch of the persistent level chain over instances v (k_chain in
 // chain.hip), then one stream sync, after which each instance takes over the
 // levels the chain built as if the host had built them.  Each instance gets a
 // slab for the rows of those levels (given back beyond what they use); one
 // whose budget or the arena has no room keeps building on the host, and is
 // left in v.
-void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
+void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max, bool lds) {
     std::vector<Inst*> ch, rest;
     for (auto* I : v) {
         const uint64_t row_bytes = 8 * (uint64_t)I->h.C + 8;
@@ -1394,7 +1405,8 @@ void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
     hipEvent_t ev = nullptr;
     time_begin(&ev, 4);
     int G_used = G;
-    CK(launch_chain((ChainIn*)d, (ChainOut*)(d + o_out), (int)n, G, st_, &G_used));
+    if (lds) CK(launch_chain_lds((ChainIn*)d, (ChainOut*)(d + o_out), (int)n, st_));
+    else CK(launch_chain((ChainIn*)d, (ChainOut*)(d + o_out), (int)n, G, st_, &G_used));
     time_end(ev, 4, ch);
     g_kstats.chain_launches++;
     if (G_used != G) g_kstats.chain_solo_launches++;
@@ -1403,7 +1415,8 @@ void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max) {
     sync();
     for (size_t k = 0; k < n; ++k)
         if (h_chain_out_[k].err || h_chain_out_[k].exit == CH_ERROR)
-            throw DeviceError("level chain: a team barrier timed out");
+            throw DeviceError(lds ? "level chain in LDS: a key found no slot in the device set"
+                                  : "level chain: a team barrier timed out");
     chain_wb_clean_ = wb_all;   // a team leaves both its bitmaps cleared
     std::vector<FormulaRef> found;
     for (size_t k = 0; k < n; ++k)
@@ -1429,7 +1442,7 @@ bool Engine::chain_adopt(Inst& I, const ChainOut& o) {
     vfb_result& R = *I.res;
     if (opt_.verbose >= 2) {
         // per level: microseconds from the launch to the end of each phase
-        std::fprintf(stderr, "chain: exit %d after %d levels\n", o.exit, o.nlev);
+        std::fprintf(stderr, "chain: exit %d after %d levels (overflow %u)\n", o.exit, o.nlev, o.overflow);
         for (int j = 0; j <= o.nlev && j < kMaxS; ++j)
             std::fprintf(stderr, "chain level %d: plan %.1f expand %.1f resolve %.1f scan %.1f store %.1f us\n",
                          I.level + 1 + j, (o.t[j][0] - o.t0) * 0.01, (o.t[j][1] - o.t0) * 0.01,
@@ -1650,6 +1663,8 @@ void Engine::step(std::vector<Inst*>& act) {
         size_t running = 0;
         for (auto* I : act) running += I->done ? 0 : 1;
         const bool chain_on = tun_.chain_blocks > 0 && !fixed_ && chain_resident_ > 0;
+        const bool lds_on = tun_.lds_chain && !fixed_;
+        std::vector<Inst*> pre_lds;
         for (auto* I : act) {
             if (I->done) continue;
             I->check_only = I->next_check_only;
@@ -1662,11 +1677,22 @@ void Engine::step(std::vector<Inst*>& act) {
                 parked_.push_back(I);
                 continue;
             }
+            if (lds_on && lds_chain_admits(*I)) {
+                pre_lds.push_back(I);
+                continue;
+            }
             if (chain_on && chain_admits(*I, kChainMaxCands)) {
                 pre.push_back(I);
                 continue;
             }
             prepare_or_check(*I, step);
+        }
+        if (!pre_lds.empty()) {
+            // the smallest levels, one workgroup per instance with its set
+            // in LDS; members it could not take (no room for their slab)
+            // build this level on the host
+            chain(pre_lds, 1, kLdsChainCands, true);
+            for (auto* I : pre_lds) prepare_or_check(*I, step);
         }
         if (!pre.empty()) {
             // a team of G workgroups per instance, as many as the device
@@ -2213,7 +2239,7 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     t->check_rows = kCheckRows;
     t->chain_blocks = 0;
     t->chain_cands = 1024;
-    t->pad_tuning = 0;
+    t->lds_chain = 1;
 }
 
 // the ranges vfb_set_tuning accepts (vfb_tuning_load applies the same)
@@ -2222,7 +2248,8 @@ bool vfb::tuning_valid(const vfb_tuning& t) {
     return ok(t.expand_blocks_per_cu) && ok(t.resolve_blocks_per_cu) && ok(t.store_blocks_per_cu) &&
            t.table_min_slots <= (1ull << 32) && t.max_load_pct >= 50 && t.max_load_pct <= 90 && t.store_traces >= 1 &&
            t.store_traces <= 128 && (t.check_rows == 1 || t.check_rows == 4 || t.check_rows == 8 || t.check_rows == 16) &&
-           t.chain_blocks <= (uint32_t)kChainMaxTeam && t.chain_cands >= 64 && t.chain_cands <= 65536;
+           t.chain_blocks <= (uint32_t)kChainMaxTeam && t.chain_cands >= 64 && t.chain_cands <= 65536 &&
+           t.lds_chain <= 1;
 }
 
 extern "C" void vfb_get_tuning(vfb_tuning* t) {

@@ -49,6 +49,9 @@ hipError_t launch_store(const InstDesc* D, const Seg* segs, int nseg, uint64_t t
 // resident per CU
 hipError_t launch_chain(ChainIn* ins, ChainOut* outs, int nteams, int G, hipStream_t st, int* G_used);
 int chain_blocks_per_cu();
+// the level chain in LDS (chain.hip): one workgroup per instance, its hash
+// set in LDS (kLdsChain* in engine_dev.h)
+hipError_t launch_chain_lds(ChainIn* ins, ChainOut* outs, int n, hipStream_t st);
 
 struct DeviceError : std::runtime_error {
     explicit DeviceError(const std::string& s) : std::runtime_error(s) {}
@@ -333,7 +336,8 @@ class Engine {
     void store(std::vector<Inst*>& v);
     void prepare_or_check(Inst& I, std::vector<Inst*>& step);
     bool chain_admits(const Inst& I, uint64_t n_max) const;
-    void chain(std::vector<Inst*>& v, int G, uint64_t n_max);
+    bool lds_chain_admits(const Inst& I) const;
+    void chain(std::vector<Inst*>& v, int G, uint64_t n_max, bool lds = false);
     bool chain_adopt(Inst& I, const ChainOut& o);
 };
 

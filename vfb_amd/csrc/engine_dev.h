@@ -205,6 +205,11 @@ constexpr int kChainMaxTeam = 64;               // workgroups per instance, at m
 constexpr uint64_t kChainMaxCands = 1ull << 18; // a chain level's winner bitmap fits LDS (64 chunks)
 constexpr uint64_t kChainSoloMax = 64;          // levels this small are built by a team's first workgroup alone
 constexpr int kMaxTasks = 1 + kMaxS / 2;        // ~XFG task + one per split
+// the level chain in LDS (k_chain_lds): its set's slots, the keys it takes
+// (load 50 %), and the candidates of a level it builds (one 4096-ordinal chunk)
+constexpr int kLdsChainSlots = 4096;
+constexpr uint64_t kLdsChainKeys = kLdsChainSlots / 2;
+constexpr uint64_t kLdsChainCands = 4096;
 
 // per instance, uploaded for one launch
 struct ChainIn {

@@ -454,10 +454,21 @@ __device__ __forceinline__ uint32_t fold_end(Key* k, const uint64_t* acc, const 
     return mm;
 }
 
+// Where a candidate probed by one lane goes: the device-wide hash set
+// (finish_one).  The LDS level chain passes its own sink with the same
+// per-candidate semantics (chain.hip).
+struct GlobalSink {
+    static constexpr bool kGlobal = true;   // finish_candidates (G = 1) probes the device set too
+    static __device__ __forceinline__ void finish(const InstDesc& d, const Key& key, uint64_t ord, uint32_t mismatch,
+                                                  bool en, int lane) {
+        finish_one(d, key, ord, mismatch, en, lane);
+    }
+};
+
 // Combines the per-lane partial keys / mismatch bits of the G lanes that
 // share a candidate (G > 1: each lane folded every G-th trace), then probes
 // (or, in a check-only level, checks) the four candidates.
-template <int KEY, bool CHECK, int G>
+template <int KEY, bool CHECK, int G, class Sink = GlobalSink>
 __device__ __forceinline__ void finish_group(const InstDesc& d, Key* k, const uint64_t* ord, uint32_t mm,
                                              uint32_t enable, int lane) {
     if constexpr (G > 1) {
@@ -480,6 +491,7 @@ __device__ __forceinline__ void finish_group(const InstDesc& d, Key* k, const ui
 #pragma unroll
     for (int q = 0; q < 4; ++q) key_fin(k[q]);
     if constexpr (G == 1) {
+        static_assert(Sink::kGlobal, "one lane per candidate probes the device-wide set only");
         finish_candidates(d, k, ord, mm, enable, lane);
     } else {
         // lane `sub` (< 4) of the group probes candidate `sub` alone
@@ -495,14 +507,14 @@ __device__ __forceinline__ void finish_group(const InstDesc& d, Key* k, const ui
             o1 |= ord[i] & m;
         }
         const bool en1 = sub < 4 && ((enable >> q) & 1u);
-        finish_one(d, k1, o1, (mm >> q) & 1u, en1, lane);
+        Sink::finish(d, k1, o1, (mm >> q) & 1u, en1, lane);
     }
 }
 
 // ~, X, F, G of one row (SEL: mask of the four computed).  G lanes per row;
 // lane `sub` folds traces sub, sub+G, ... in groups whose loads are issued
 // before their first use.
-template <int KEY, bool CHECK, int G, int SEL>
+template <int KEY, bool CHECK, int G, int SEL, class Sink = GlobalSink>
 __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
     constexpr int W = 64 / G;
     const int sub = lane % G;
@@ -559,19 +571,19 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
     const uint64_t ord[4] = {tk.base[0] + x, tk.base[1] + x, tk.base[2] + x, tk.base[3] + x};
     // SEL_ALL also serves any other mask (the cost function's): tk.part enables
     const uint32_t win = CHECK ? 0xfu : in_window(d, ord);
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part & win) : 0u, lane);
+    finish_group<KEY, CHECK, G, Sink>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part & win) : 0u, lane);
 }
 
-template <int KEY, int G>
+template <int KEY, int G, class Sink = GlobalSink>
 __device__ __forceinline__ void expand_unary_part(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
-    if (tk.part == SEL_NOT) expand_unary<KEY, false, G, SEL_NOT>(d, tk, wl, lane);
-    else if (tk.part == SEL_XFG) expand_unary<KEY, false, G, SEL_XFG>(d, tk, wl, lane);
-    else expand_unary<KEY, false, G, SEL_ALL>(d, tk, wl, lane);
+    if (tk.part == SEL_NOT) expand_unary<KEY, false, G, SEL_NOT, Sink>(d, tk, wl, lane);
+    else if (tk.part == SEL_XFG) expand_unary<KEY, false, G, SEL_XFG, Sink>(d, tk, wl, lane);
+    else expand_unary<KEY, false, G, SEL_ALL, Sink>(d, tk, wl, lane);
 }
 
 // &, |, l U r, r U l of one pair (left row uniform per wavefront); SEL is
 // the mask of the four computed
-template <int KEY, bool CHECK, int G, int SEL>
+template <int KEY, bool CHECK, int G, int SEL, class Sink = GlobalSink>
 __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
     constexpr int W = 64 / G;
     const int sub = lane % G;
@@ -636,17 +648,17 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
     const uint64_t ord[4] = {tk.base[0] + pair, tk.base[1] + pair, tk.base[2] + 2 * pair,
                              tk.base[2] + 2 * pair + 1};
     const uint32_t win = CHECK ? 0xfu : in_window(d, ord);
-    finish_group<KEY, CHECK, G>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part & win) : 0u, lane);
+    finish_group<KEY, CHECK, G, Sink>(d, k, ord, mm, active ? ((uint32_t)SEL & tk.part & win) : 0u, lane);
 }
 
-template <int KEY, int G>
+template <int KEY, int G, class Sink = GlobalSink>
 __device__ __forceinline__ void expand_binary_part(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
     switch (tk.part) {
-        case SEL_AND: expand_binary<KEY, false, G, SEL_AND>(d, tk, wl, lane); break;
-        case SEL_ANDOR: expand_binary<KEY, false, G, SEL_ANDOR>(d, tk, wl, lane); break;
-        case SEL_ORU: expand_binary<KEY, false, G, SEL_ORU>(d, tk, wl, lane); break;
-        case SEL_U: expand_binary<KEY, false, G, SEL_U>(d, tk, wl, lane); break;
-        default: expand_binary<KEY, false, G, SEL_ALL>(d, tk, wl, lane); break;
+        case SEL_AND: expand_binary<KEY, false, G, SEL_AND, Sink>(d, tk, wl, lane); break;
+        case SEL_ANDOR: expand_binary<KEY, false, G, SEL_ANDOR, Sink>(d, tk, wl, lane); break;
+        case SEL_ORU: expand_binary<KEY, false, G, SEL_ORU, Sink>(d, tk, wl, lane); break;
+        case SEL_U: expand_binary<KEY, false, G, SEL_U, Sink>(d, tk, wl, lane); break;
+        default: expand_binary<KEY, false, G, SEL_ALL, Sink>(d, tk, wl, lane); break;
     }
 }
 

@@ -40,6 +40,7 @@ const Field kFields[] = {
     {"check_rows", false, offsetof(vfb_tuning, check_rows)},
     {"chain_blocks", false, offsetof(vfb_tuning, chain_blocks)},
     {"chain_cands", false, offsetof(vfb_tuning, chain_cands)},
+    {"lds_chain", false, offsetof(vfb_tuning, lds_chain)},
 };
 
 uint64_t get(const vfb_tuning& t, const Field& f) {
@@ -171,6 +172,7 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     static const uint64_t cand_rows[] = {1, 4, 8, 16};
     static const uint64_t cand_chain[] = {0, 4, 16, 64};
     static const uint64_t cand_chain_cands[] = {256, 1024, 4096};
+    static const uint64_t cand_lds[] = {0, 1};
     struct Axis {
         const Field* f;
         const uint64_t* v;
@@ -178,7 +180,7 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     } axes[] = {{&kFields[0], cand_pass, 5}, {&kFields[1], cand_wide, 7}, {&kFields[2], cand_expand, 4},
                 {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}, {&kFields[5], cand_park, 4},
                 {&kFields[8], cand_st, 5},      {&kFields[9], cand_rows, 4},   {&kFields[10], cand_chain, 4},
-                {&kFields[11], cand_chain_cands, 3}};
+                {&kFields[11], cand_chain_cands, 3}, {&kFields[12], cand_lds, 2}};
     // every setting timed, for the cache file: "# tried <key> <value> <ms>"
     std::string log = "# tried start - " + std::to_string(t_default) + "\n";
     for (const Axis& ax : axes) {
