@@ -509,28 +509,36 @@ __global__ void k_rehash_multi(const RehashSeg* __restrict__ segs, int n, uint64
     }
 }
 
-// fills and copies of many buffers in one launch: one thread per chunk of 8
-// words (64 bytes) of a segment (segment k holds chunks begin_k ..), a whole
-// chunk of a fill as four 16-byte stores (every destination is 256-byte
-// aligned), copies and segment tails word by word
+// fills and copies of many buffers in one launch over a flat space of chunks
+// of 8 words (64 bytes; segment k holds chunks begin_k ..).  A wavefront
+// takes 64 chunks (4 KB): inside one fill segment as four 16-byte stores per
+// lane, each store of the wavefront 1 KB contiguous (every destination is
+// 256-byte aligned; 64 contiguous bytes per lane, the other layout, measured
+// 3.7 against 5.5-5.8 TB/s of writes); chunks of copies, segment tails and
+// boundaries one per lane, word by word
 __global__ void k_fill_multi(const OpSeg* __restrict__ segs, int n, uint64_t total_chunks) {
     typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < total_chunks; g += stride) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t wv = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+    const uint64_t nwv = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t w = wv; w * 64 < total_chunks; w += nwv) {
+        const uint64_t g0 = w * 64;
+        const uint64_t g1 = g0 + 64 < total_chunks ? g0 + 64 : total_chunks;
+        const OpSeg& s0 = segs[seg_of(segs, n, g0)];
+        if (!s0.src && (g1 - s0.begin) * 8 <= s0.words) {
+            const u64x2 v = {s0.value, s0.value};
+            GLB u64x2* d2 = (GLB u64x2*)(glob(s0.dst) + (g0 - s0.begin) * 8);
+            const uint32_t units = (uint32_t)(g1 - g0) * 4;
+            for (uint32_t u = (uint32_t)lane; u < units; u += 64) d2[u] = v;
+            continue;
+        }
+        const uint64_t g = g0 + (uint64_t)lane;
+        if (g >= g1) continue;
         const OpSeg& sg = segs[seg_of(segs, n, g)];
         const uint64_t i0 = (g - sg.begin) * 8;
         const uint64_t m = sg.words - i0 < 8 ? sg.words - i0 : 8;
         gu64* dst = glob(sg.dst) + i0;
-        if (!sg.src && m == 8) {
-            const u64x2 v = {sg.value, sg.value};
-            GLB u64x2* d2 = (GLB u64x2*)dst;
-            d2[0] = v;
-            d2[1] = v;
-            d2[2] = v;
-            d2[3] = v;
-        } else {
-            for (uint64_t j = 0; j < m; ++j) dst[j] = sg.src ? glob(sg.src)[i0 + j] : sg.value;
-        }
+        for (uint64_t j = 0; j < m; ++j) dst[j] = sg.src ? glob(sg.src)[i0 + j] : sg.value;
     }
 }
 
