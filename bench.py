@@ -135,10 +135,11 @@ def headline(V, steps, warmup, autotune, batch_only=False, tuning_file=None, sav
         V.set_tuning(**tuned["tuning"])
         log(f"tuning from {tuning_file}: {tuned['tuning']}")
     elif autotune:
-        # a sample of the mix: its first 48 instances and two whose largest
-        # levels reach 10^7 candidates (the large levels set the mix's pace)
-        sample = insts[:48] + [insts[219], insts[456]]
-        log("autotuning on instances 0..47, 219 and 456 of the mix (batched)")
+        # a sample of the mix: its first 48 instances and three whose largest
+        # levels reach 10^7-10^9 candidates (the large levels set the mix's
+        # pace; 104 has the mix's largest, a 2 G-candidate level)
+        sample = insts[:48] + [insts[104], insts[219], insts[456]]
+        log("autotuning on instances 0..47, 104, 219 and 456 of the mix (batched)")
         cache = save_tuning or os.path.join(ROOT, "gpurun_out", "mix_tuning_autotune.txt")
         os.makedirs(os.path.dirname(cache), exist_ok=True)
         tuned = V.autotune(sample, max_size=MIX_MAX, batched=True, reps=2, cache=cache)

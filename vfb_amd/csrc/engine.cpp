@@ -2229,9 +2229,9 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     if (!t) return;
     t->pass_min = 1ull << 22;
     t->wide_max_waves = kWideMaxWaves;
-    t->expand_blocks_per_cu = 32;
+    t->expand_blocks_per_cu = 64;
     t->resolve_blocks_per_cu = 16;
-    t->store_blocks_per_cu = 8;
+    t->store_blocks_per_cu = 64;
     t->park_min = 1ull << 24;
     t->table_min_slots = kMinSlots;
     t->max_load_pct = 70;

@@ -161,9 +161,9 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
     double t_best = t_default;
     static const uint64_t cand_pass[] = {0, 1ull << 20, 1ull << 22, 1ull << 24, 1ull << 26};
     static const uint64_t cand_wide[] = {0, 128, 256, 512, 1024, 2048, 8192};
-    static const uint64_t cand_expand[] = {4, 8, 16, 32};
-    static const uint64_t cand_resolve[] = {8, 16, 32};
-    static const uint64_t cand_store[] = {4, 8, 16};
+    static const uint64_t cand_expand[] = {8, 16, 32, 64};
+    static const uint64_t cand_resolve[] = {8, 16, 32, 64};
+    static const uint64_t cand_store[] = {8, 16, 32, 64};
     static const uint64_t cand_park[] = {0, 1ull << 22, 1ull << 24, 1ull << 26};
     // (table_min_slots and max_load_pct are not searched: they size the sets,
     // and a level whose set and scratch do not fit the HBM budget is checked
@@ -178,7 +178,7 @@ extern "C" vfb_status vfb_autotune(const vfb_instance* const* sample, size_t n, 
         const uint64_t* v;
         size_t nv;
     } axes[] = {{&kFields[0], cand_pass, 5}, {&kFields[1], cand_wide, 7}, {&kFields[2], cand_expand, 4},
-                {&kFields[3], cand_resolve, 3}, {&kFields[4], cand_store, 3}, {&kFields[5], cand_park, 4},
+                {&kFields[3], cand_resolve, 4}, {&kFields[4], cand_store, 4}, {&kFields[5], cand_park, 4},
                 {&kFields[8], cand_st, 5},      {&kFields[9], cand_rows, 4},   {&kFields[10], cand_chain, 4},
                 {&kFields[11], cand_chain_cands, 3}, {&kFields[12], cand_lds, 2}};
     // every setting timed, for the cache file: "# tried <key> <value> <ms>"
