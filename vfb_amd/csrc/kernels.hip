@@ -268,12 +268,23 @@ __global__ __launch_bounds__(256) void k_expand_mw(const InstDesc* __restrict__ 
 }
 
 // Target check of tk.lrows (= K) left rows against one right row per lane,
-// one-word rows: each right word is loaded once for the K left rows (scalar
-// loads of consecutive words), so the larger operand level is streamed
-// ceil(nl / K) times instead of nl.  The lowest matching ordinal of the wave
-// goes to ctr->found.
+// one-word rows: each right word is loaded once for the K left rows, so the
+// larger operand level is streamed ceil(nl / K) times instead of nl.  Per
+// group of 8 traces one load instruction fetches the K left words of all 8
+// (lane 8j + i: row l0 + i of the group's trace j; K = 16 takes two), and
+// each word reaches the lanes with v_readlane; the group's trace numbers come
+// from the wavefront's copy of the check order (4 per lane, read once), so
+// the group's right and left words are in flight together.  The lowest
+// matching ordinal of the wave goes to ctr->found.
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int src) {
+    return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(x >> 32), src) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((uint32_t)x, src);
+}
+
 template <int K>
 __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task& tk, uint64_t wl, int lane) {
+    static_assert(K == 1 || K == 4 || K == 8 || K == 16, "left rows per wavefront");
+    constexpr int NQ = (K + 7) / 8;            // left-word loads per trace group
     const uint64_t lg = wl / tk.nchunk;
     const uint64_t rc = wl - lg * tk.nchunk;
     const uint64_t l0 = lg * K;
@@ -287,6 +298,9 @@ __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task&
     const uint64_t nl = tk.nl;
     const gu64* __restrict__ L = glob(d.lvl[tk.lvl_l]);
     const gu64* __restrict__ R = glob(d.lvl[tk.lvl_r]);
+    // the check order, 4 traces per lane (lanes 0..kMaxT/4-1)
+    static_assert(kMaxT <= 256 && offsetof(InstDesc, check_order) % 4 == 0, "check order in 32-bit lanes");
+    const uint32_t ordw = lane < kMaxT / 4 ? ((const uint32_t*)d.check_order)[lane] : 0u;
     // mismatch bits per left row, starting with the candidates that do not
     // count (no pair, or not in the task's part): once every candidate of
     // the wavefront has missed a trace, the rest of the traces are skipped
@@ -295,22 +309,32 @@ __device__ __forceinline__ void check_binary_rows(const InstDesc& d, const Task&
     for (int k = 0; k < K; ++k)
         mm[k] = (!active || l0 + k >= nl || (diag && r <= l0 + k)) ? 0xfu : (~(uint32_t)tk.part & 0xfu);
     const int T = d.T;
+    const int jl = lane >> 3, il = lane & 7;
     for (int t0 = 0; t0 < T; t0 += kTraceGroup) {
-        uint64_t bv[kTraceGroup];
+        const uint64_t o8 = ((uint64_t)__builtin_amdgcn_readlane(ordw, t0 / 4 + 1) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane(ordw, t0 / 4);
+        uint64_t bv[kTraceGroup], av[NQ];
 #pragma unroll
         for (int j = 0; j < kTraceGroup; ++j)
-            if (t0 + j < T) bv[j] = R[(uint64_t)d.check_order[t0 + j] * tk.nr + rr];
+            if (t0 + j < T) bv[j] = R[((o8 >> (8 * j)) & 0xffu) * tk.nr + rr];
+        {
+            // rows past the level repeat its last row (masked by mm)
+            const uint64_t tl = (o8 >> (8 * jl)) & 0xffu;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const uint64_t row = l0 + (uint64_t)(q * 8 + il) < nl ? l0 + (uint64_t)(q * 8 + il) : nl - 1;
+                av[q] = t0 + jl < T && q * 8 + il < K ? L[tl * nl + row] : 0ull;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < kTraceGroup; ++j) {
             if (t0 + j >= T) break;
-            const int t = d.check_order[t0 + j];
+            const int t = (int)((o8 >> (8 * j)) & 0xffu);
             const uint64_t b = bv[j];
             const uint64_t want = t < d.P ? 1ull : 0ull;
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-                // rows past the level repeat its last row (masked below)
-                const uint64_t l = l0 + k < nl ? l0 + k : nl - 1;
-                const uint64_t a = L[(uint64_t)t * nl + l];
+                const uint64_t a = readlane64(av[k / 8], j * 8 + (k & 7));
                 mm[k] |= bit0_mismatch(a & b & 1ull, (a | b) & 1ull, until_bit0(a, b), until_bit0(b, a), want);
             }
         }
@@ -353,9 +377,6 @@ __device__ __forceinline__ uint64_t wave_min_ordinal(const Task& tk, uint64_t wl
     return m;
 }
 
-// Target check only (the probe of a large level, or an onTheFly level): the
-// lowest matching ordinal is all it finds, so a wave whose candidates all lie
-// above a match already found skips (a stale read of `found` only skips less).
 template <int G>
 __global__ __launch_bounds__(256) void k_check(const InstDesc* __restrict__ D, const Task* __restrict__ tasks,
                                                int ntasks, uint64_t total_waves) {

@@ -398,6 +398,20 @@ __device__ __forceinline__ uint32_t in_window(const InstDesc& d, const uint64_t*
     return m;
 }
 
+// Traces t0..t0+7 of the target-check order, t0 a multiple of 8, as one
+// aligned 8-byte load of InstDesc::check_order (a wavefront-uniform value, so
+// the operand loads of the 8 traces issue together instead of each waiting on
+// a load of its trace number); trace j is byte j.
+static_assert(offsetof(InstDesc, check_order) % 8 == 0 && kMaxT % 8 == 0 && kTraceGroup == 8,
+              "check_order in aligned groups of one trace group");
+__device__ __forceinline__ uint64_t check_order8(const InstDesc& d, int t0) {
+    const uint64_t v = *(const uint64_t*)(d.check_order + t0);
+    // every lane loaded the same word: scalar registers, so that loads
+    // indexed by these traces (the left rows of k_check) are scalar too
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+}
+
 // Per-trace accumulation of one candidate's CS word c.  KEY_MIX keys are
 // the XOR of dmix64(word) over the traces below InstDesc::half (hi) and over
 // the rest (lo): the loops run those two trace ranges one after the other, so
@@ -541,13 +555,19 @@ __device__ __forceinline__ void expand_unary(const InstDesc& d, const Task& tk, 
             }
             uint64_t av[kTraceGroup];
             // (a check visits the traces in check_order; keys need them in order)
+            const uint64_t o8 = CHECK && G == 1 ? check_order8(d, t0) : 0;
+            auto trace = [&](int j) -> int {
+                if constexpr (!CHECK) return t0 + j * G;
+                else if constexpr (G == 1) return (int)((o8 >> (8 * j)) & 0xffu);
+                else return d.check_order[t0 + j * G];
+            };
 #pragma unroll
             for (int j = 0; j < kTraceGroup; ++j)
-                if (t0 + j * G < te) av[j] = L[(uint64_t)(CHECK ? d.check_order[t0 + j * G] : t0 + j * G) * tk.nl + xr];
+                if (t0 + j * G < te) av[j] = L[(uint64_t)trace(j) * tk.nl + xr];
 #pragma unroll
             for (int j = 0; j < kTraceGroup; ++j) {
                 if (t0 + j * G >= te) break;
-                const int t = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
+                const int t = trace(j);
                 const uint64_t a = av[j];
                 const uint64_t m = d.lenmask[t];
                 const uint64_t want = (KEY != KEY_MIX || CHECK) && d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
@@ -615,17 +635,23 @@ __device__ __forceinline__ void expand_binary(const InstDesc& d, const Task& tk,
                 if (!__ballot((mm | dead) != 0xfu)) break;
             }
             uint64_t av[kTraceGroup], bv[kTraceGroup];
+            const uint64_t o8 = CHECK && G == 1 ? check_order8(d, t0) : 0;
+            auto trace = [&](int j) -> int {
+                if constexpr (!CHECK) return t0 + j * G;
+                else if constexpr (G == 1) return (int)((o8 >> (8 * j)) & 0xffu);
+                else return d.check_order[t0 + j * G];
+            };
 #pragma unroll
             for (int j = 0; j < kTraceGroup; ++j)
                 if (t0 + j * G < te) {
-                    const uint64_t tt = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
+                    const uint64_t tt = (uint64_t)trace(j);
                     av[j] = L[tt * tk.nl + l];
                     bv[j] = R[tt * tk.nr + rr];
                 }
 #pragma unroll
             for (int j = 0; j < kTraceGroup; ++j) {
                 if (t0 + j * G >= te) break;
-                const int t = CHECK ? d.check_order[t0 + j * G] : t0 + j * G;
+                const int t = trace(j);
                 const uint64_t a = av[j], b = bv[j];
                 const uint64_t want = (KEY != KEY_MIX || CHECK) && d.bs_mode ? d.bs_target : (t < d.P ? 1ull : 0ull);
                 if constexpr (CHECK) {
