@@ -298,6 +298,7 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
         d.found_below = kEmpty;
         d.ord_lo = 0;
         d.ord_hi = kEmpty;
+        d.defer_claims = 0;   // claims go to the counter at once (read back at the level's end)
     }
     const bool lead = rank == 0 && threadIdx.x == 0;
     if (lead) out.t0 = wall_clock64();
@@ -661,6 +662,7 @@ __global__ __launch_bounds__(kLThreads) void k_chain_lds(ChainIn* ins, ChainOut*
         d.ord_lo = 0;
         d.ord_hi = kEmpty;
         d.ctr_host = nullptr;
+        d.defer_claims = 0;
         sh.nkeys = 0;
         sh.ovf = 0;
     }

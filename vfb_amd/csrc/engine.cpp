@@ -1164,6 +1164,8 @@ void Engine::epoch(std::vector<Inst*>& v) {
         hd[k] = I.h;
         hd[k].ctr_host = all_scan ? h_ctr_ + k : nullptr;
         hd[k].check_only = checks_only(I) ? 1 : 0;
+        // the launch holds this instance alone: its claims are added up per workgroup
+        hd[k].defer_claims = n == 1 ? 1 : 0;
         if (I.pass != P_REST) {
             hc[k] = Ctr{};          // first pass of the level's epoch: fresh counters
             hc[k].found = kEmpty;

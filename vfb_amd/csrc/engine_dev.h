@@ -93,7 +93,8 @@ struct InstDesc {
     int mark;         // resolve: 1 sets the winner bitmap (last pass), 0 only reports a match
     int bs_mode;      // BS stage: one word per row, a match is the word == bs_target
     int store_traces; // traces per k_store work item
-    int pad_desc;
+    int defer_claims; // the launch holds this instance alone: k_expand adds up its claims per
+                      // workgroup in LDS and checks the set's load limit at the workgroup's end
     uint64_t lenmask[kMaxT];
     uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
     int16_t koff[kMaxT];        // first key bit of the trace's field / samples

@@ -32,6 +32,7 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
     const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t wpb = blockDim.x >> 6;
     const uint64_t stride = (uint64_t)gridDim.x * wpb;
+    begin_claims();
     for (uint64_t w = (uint64_t)blockIdx.x * wpb + wib; w < total_waves; w += stride) {
         int lo = 0, hi = ntasks - 1;
         while (lo < hi) {
@@ -55,6 +56,7 @@ __global__ __launch_bounds__(256) void k_expand(const InstDesc* __restrict__ D, 
             else expand_binary_part<KEY_STRIDE, G>(d, tk, wl, lane);
         }
     }
+    flush_claims(D, tasks, ntasks);
 }
 
 // ---------------------------------------------------------------------------
@@ -230,6 +232,7 @@ __global__ __launch_bounds__(256) void k_expand_mw(const InstDesc* __restrict__ 
     const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t wpb = blockDim.x >> 6;
     const uint64_t stride = (uint64_t)gridDim.x * wpb;
+    begin_claims();
     for (uint64_t w = (uint64_t)blockIdx.x * wpb + wib; w < total_waves; w += stride) {
         int lo = 0, hi = ntasks - 1;
         while (lo < hi) {
@@ -265,6 +268,7 @@ __global__ __launch_bounds__(256) void k_expand_mw(const InstDesc* __restrict__ 
             else expand_binary_mw_part<KEY_STRIDE, G>(d, tk, wl, lane);
         }
     }
+    flush_claims(D, tasks, ntasks);
 }
 
 // Target check of tk.lrows (= K) left rows against one right row per lane,
