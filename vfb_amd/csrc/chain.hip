@@ -299,6 +299,7 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
         d.ord_lo = 0;
         d.ord_hi = kEmpty;
         d.defer_claims = 0;   // claims go to the counter at once (read back at the level's end)
+        d.fresh_chunk = 0;
     }
     const bool lead = rank == 0 && threadIdx.x == 0;
     if (lead) out.t0 = wall_clock64();
@@ -663,6 +664,7 @@ __global__ __launch_bounds__(kLThreads) void k_chain_lds(ChainIn* ins, ChainOut*
         d.ord_hi = kEmpty;
         d.ctr_host = nullptr;
         d.defer_claims = 0;
+        d.fresh_chunk = 0;
         sh.nkeys = 0;
         sh.ovf = 0;
     }

@@ -769,6 +769,25 @@ static void pass_work(const Inst& I, uint64_t* cand, uint64_t* bytes) {
     }
 }
 
+// Fresh-list entries a large level holds beyond its estimate for the ones
+// its wavefronts reserve and leave unused: a launch over that instance alone
+// lets each wavefront reserve kFreshChunk entries at a time (one atomic per
+// chunk on the instance's counter instead of one per wavefront), and each
+// of at most two appending passes leaves at most one partial chunk per
+// wavefront of its grid (at most 64 workgroups of 4 wavefronts per CU, the
+// largest expand_blocks_per_cu).  Sized from the level alone, not from the
+// batch or the tuning, so that plain, batched and served solves hold the
+// same memory under every setting.
+uint64_t Engine::fresh_reserve(const Inst& I) const {
+    if (fixed_ || I.N < kFreshChunkMin) return 0;
+    return 2ull * (uint64_t)cus_ * 64 * 4 * kFreshChunk;
+}
+
+// fresh entries of the level's epoch (the reserved ones left unused excluded)
+uint64_t Engine::fresh_exact(const Inst& I) const {
+    return I.fresh_chunked ? I.ctr.fresh_real : I.ctr.fresh_count;
+}
+
 // scratch and hash set for a deduplicating level, sized from the previous
 // level's ratios and grown on overflow
 void Engine::prepare_scratch(Inst& I) {
@@ -780,7 +799,7 @@ void Engine::prepare_scratch(Inst& I) {
     // chain built (the floors), so that memory use does not depend on the chain
     uint64_t fcap = std::min<uint64_t>(
         N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
-    fcap = std::max(fcap, I.floor_fcap);
+    fcap = std::max(fcap, I.floor_fcap) + fresh_reserve(I);
     const uint64_t want_nw = std::max(nwords, I.floor_nwords);
     const uint64_t want_nb = std::max(nblk, I.floor_nblk);
     I.floor_fcap = I.floor_nwords = I.floor_nblk = 0;
@@ -1164,8 +1183,14 @@ void Engine::epoch(std::vector<Inst*>& v) {
         hd[k] = I.h;
         hd[k].ctr_host = all_scan ? h_ctr_ + k : nullptr;
         hd[k].check_only = checks_only(I) ? 1 : 0;
-        // the launch holds this instance alone: its claims are added up per workgroup
+        // the launch holds this instance alone: its claims are added up per
+        // workgroup, and a large level's wavefronts reserve fresh entries in
+        // chunks (the list holds fresh_reserve(I) for their unused tails)
         hd[k].defer_claims = n == 1 ? 1 : 0;
+        const bool chunk = n == 1 && !checks_only(I) && fresh_reserve(I) > 0;
+        hd[k].fresh_chunk = chunk ? kFreshChunk : 0;
+        if (I.pass != P_REST) I.fresh_chunked = chunk;
+        else I.fresh_chunked = I.fresh_chunked || chunk;
         if (I.pass != P_REST) {
             hc[k] = Ctr{};          // first pass of the level's epoch: fresh counters
             hc[k].found = kEmpty;
@@ -1746,7 +1771,8 @@ void Engine::step(std::vector<Inst*>& act) {
                     try {
                         if (I.ctr.overflow & 2) {
                             uint64_t fcap = std::min<uint64_t>(
-                                I.N, std::max<uint64_t>(I.fresh_cap * 4, I.ctr.fresh_count + 4096));
+                                I.N, std::max<uint64_t>(I.fresh_cap * 4, I.ctr.fresh_count + 4096)) +
+                                            fresh_reserve(I);
                             ensure(I, I.fresh_ord, fcap * 8);
                             ensure(I, I.fresh_slot, fcap * 4);
                             I.fresh_cap = fcap;
@@ -1834,7 +1860,7 @@ void Engine::step(std::vector<Inst*>& act) {
             if (opt_.verbose)
                 std::fprintf(stderr, "Cost %-2d | %s | candidates: %-14llu | fresh: %-12llu | distinct: %-12llu | table: %llu\n",
                              s, I.check_only ? "check" : "dedup", (unsigned long long)I.N,
-                             (unsigned long long)I.ctr.fresh_count, (unsigned long long)I.ctr.winners,
+                             (unsigned long long)fresh_exact(I), (unsigned long long)I.ctr.winners,
                              (unsigned long long)I.cap);
             if (I.check_only) {
                 // onTheFly last round (processOnTheFly, LTL_BS.cu:331-349)
@@ -1852,7 +1878,7 @@ void Engine::step(std::vector<Inst*>& act) {
                 continue;
             }
             R.level_winners[s] = I.ctr.winners;
-            R.fresh_total += I.ctr.fresh_count;
+            R.fresh_total += fresh_exact(I);
             if (I.ctr.found != kEmpty) {
                 R.candidates[s] = I.ctr.found + 1;
                 R.distinct[s] = I.ctr.found_rank;
@@ -1926,7 +1952,7 @@ void Engine::step(std::vector<Inst*>& act) {
             }
             const uint64_t N = std::max<uint64_t>(I.N, 1);
             I.win_ratio = (double)W / N;
-            I.fresh_ratio = (double)I.ctr.fresh_count / N;
+            I.fresh_ratio = (double)fresh_exact(I) / N;
             I.committed += W;
             I.stored += keep;
             I.gbase += I.N;

@@ -44,6 +44,8 @@ constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are coun
 constexpr uint64_t kSmallFresh = 4096;  // fresh lists up to this long are resolved in the scan's workgroup
 constexpr uint64_t kMinWinWords = 1024; // winner maps are allocated for at least 64K candidates
 constexpr uint64_t kMinSlots = 1ull << 17; // hash sets start at 4 MB
+constexpr int kFreshChunk = 256;      // fresh-list entries a wavefront reserves at a time (InstDesc::fresh_chunk)
+constexpr uint64_t kFreshChunkMin = 1ull << 22;  // levels of this many candidates reserve them so
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 
@@ -63,13 +65,15 @@ struct ColInfo {
 };
 
 struct Ctr {
-    unsigned long long fresh_count;  // entries appended to the fresh list this epoch
+    unsigned long long fresh_count;  // entries appended to the fresh list this epoch (with
+                                     // InstDesc::fresh_chunk: entries reserved, unused ones included)
     unsigned long long claims;       // slots claimed this epoch (= distinct new keys)
     unsigned long long found;        // lowest matching winner, level-local ordinal
     unsigned long long winners;      // survivors of the epoch (set by the scan)
     unsigned long long found_rank;   // survivors ordered before `found`
     unsigned int overflow;           // 1 probe limit, 2 fresh list full, 4 load limit
     unsigned int pad;
+    unsigned long long fresh_real;   // with InstDesc::fresh_chunk: the entries appended
 };
 
 // One segment of a level's canonical candidate order: operator `op` applied to
@@ -95,6 +99,9 @@ struct InstDesc {
     int store_traces; // traces per k_store work item
     int defer_claims; // the launch holds this instance alone: k_expand adds up its claims per
                       // workgroup in LDS and checks the set's load limit at the workgroup's end
+    int fresh_chunk;  // > 0 (with defer_claims): each wavefront reserves fresh-list entries this
+                      // many at a time; entries it leaves unused hold kEmpty (kFreshHole)
+    int pad_desc;
     uint64_t lenmask[kMaxT];
     uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
     int16_t koff[kMaxT];        // first key bit of the trace's field / samples

@@ -164,6 +164,12 @@ __device__ __forceinline__ uint64_t ld_dev(const gu64* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 enum { PR_DUP = 0, PR_FRESH = 1, PR_FAIL = 2 };
 
 // Claims (new keys) of the workgroup not yet added to its instance's counter:
@@ -173,6 +179,9 @@ enum { PR_DUP = 0, PR_FRESH = 1, PR_FAIL = 2 };
 // about 83 M/s on MI355X (measured), below the rate the expansion of a large
 // level issues them.
 __shared__ unsigned long long g_pending_claims;
+// with InstDesc::fresh_chunk: each wavefront's reserved range [cur, end) of
+// the fresh list, and the entries the workgroup appended (not yet counted)
+__shared__ unsigned long long g_fresh_cur[4], g_fresh_end[4], g_fresh_real;
 
 __device__ __forceinline__ void count_claims(const InstDesc& d, uint32_t ctot) {
     if (d.defer_claims) {
@@ -187,17 +196,74 @@ __device__ __forceinline__ void count_claims(const InstDesc& d, uint32_t ctot) {
 // the counter of the launch's one instance, with the load-limit check the
 // undeferred update makes (the last workgroup to flush sees the total)
 __device__ __forceinline__ void begin_claims() {
-    if (threadIdx.x == 0) g_pending_claims = 0;
+    if (threadIdx.x == 0) {
+        g_pending_claims = 0;
+        g_fresh_real = 0;
+    }
+    if (threadIdx.x < 4) g_fresh_cur[threadIdx.x] = g_fresh_end[threadIdx.x] = 0;
     __syncthreads();
 }
 __device__ __forceinline__ void flush_claims(const InstDesc* D, const Task* tasks, int ntasks) {
+    if (ntasks > 0) {
+        const InstDesc& d = D[tasks[0].inst];
+        if (d.fresh_chunk) {
+            // this wavefront's unused reserved entries: holes the resolve skips
+            const int w = (int)(threadIdx.x >> 6);
+            const uint64_t end = g_fresh_end[w] < d.fresh_cap ? g_fresh_end[w] : d.fresh_cap;
+            for (uint64_t e = g_fresh_cur[w] + (threadIdx.x & 63); e < end; e += 64) {
+                glob(d.fresh_ord)[e] = kEmpty;
+                glob(d.fresh_slot)[e] = 0u;
+            }
+        }
+    }
     __syncthreads();
-    if (threadIdx.x == 0 && ntasks > 0 && g_pending_claims) {
+    if (threadIdx.x == 0 && ntasks > 0) {
         const InstDesc& d = D[tasks[0].inst];
         const unsigned long long c = g_pending_claims;
-        const unsigned long long before = add64(glob(&d.ctr->claims), c);
-        if (before + c > d.claim_limit) or32(glob(&d.ctr->overflow), 4u);
+        if (c) {
+            const unsigned long long before = add64(glob(&d.ctr->claims), c);
+            if (before + c > d.claim_limit) or32(glob(&d.ctr->overflow), 4u);
+        }
+        if (g_fresh_real) add64(glob(&d.ctr->fresh_real), g_fresh_real);
     }
+}
+
+// Fresh-list positions of a wavefront's ftot fresh entries (wave-uniform):
+// entry i (0 <= i < ftot, in lane order) goes to p0 + i for i < rem, else to
+// p1 + i - rem.  One atomic on the instance's counter per wavefront, or
+// (fresh_chunk) one per fresh_chunk entries the wavefront appends.
+struct FreshPos {
+    uint64_t p0, p1;
+    uint32_t rem;
+};
+__device__ __forceinline__ FreshPos fresh_positions(const InstDesc& d, uint32_t ftot, int lane) {
+    FreshPos fp{0, 0, ftot};
+    if (!ftot) return fp;
+    if (!d.fresh_chunk) {
+        unsigned long long b = 0;
+        if (lane == 0) b = add64(glob(&d.ctr->fresh_count), ftot);
+        fp.p0 = bcast64(b, 0);
+        return fp;
+    }
+    const int w = (int)(threadIdx.x >> 6);
+    const uint64_t cur = g_fresh_cur[w], end = g_fresh_end[w];
+    fp.p0 = cur;
+    fp.rem = (uint32_t)(end - cur < ftot ? end - cur : ftot);
+    unsigned long long b = 0;
+    if (fp.rem < ftot && lane == 0) b = add64(glob(&d.ctr->fresh_count), (unsigned long long)d.fresh_chunk);
+    fp.p1 = bcast64(b, 0);
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        if (fp.rem < ftot) {
+            g_fresh_cur[w] = fp.p1 + (ftot - fp.rem);
+            g_fresh_end[w] = fp.p1 + (uint64_t)d.fresh_chunk;
+        } else {
+            g_fresh_cur[w] = cur + ftot;
+        }
+        atomicAdd(&g_fresh_real, (unsigned long long)ftot);
+    }
+    __builtin_amdgcn_wave_barrier();
+    return fp;
 }
 
 // Insert-or-find of a 128-bit key with lowest-ordinal tracking, lock-free:
@@ -253,12 +319,6 @@ __device__ __forceinline__ int probe_insert(GLB Slot* tab, uint64_t mask, uint64
     GLB Slot* s = tab + i;
     return probe_from(tab, mask, hi, lo, gord, epoch_base, i, ld_any(&s->hi), ld_any(&s->lo), ld_any(&s->ord),
                       slot_out, claimed);
-}
-
-__device__ __forceinline__ uint64_t bcast64(uint64_t v, int src) {
-    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, src);
-    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), src);
-    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t* total) {
@@ -350,18 +410,17 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
     (void)wave_excl_scan_u32(claimed, &ctot);
     const uint64_t fail = __ballot(act != 0);
     if (ftot | ctot | (fail ? 1u : 0u)) {
-        unsigned long long base = 0;
         if (lane == 0) {
-            if (ftot) base = add64(glob(&d.ctr->fresh_count), ftot);
             if (ctot) count_claims(d, ctot);
             if (fail) or32(glob(&d.ctr->overflow), 1u);
         }
-        base = bcast64(base, 0) + fbase;
+        const FreshPos fp = fresh_positions(d, ftot, lane);
         uint32_t k = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if ((fresh >> q) & 1u) {
-                const uint64_t e = base + k++;
+                const uint32_t i = fbase + k++;
+                const uint64_t e = i < fp.rem ? fp.p0 + i : fp.p1 + (i - fp.rem);
                 if (e < d.fresh_cap) {
                     const uint64_t flag = ((mismatch >> q) & 1u) ? 0ull : kMatchBit;
                     glob(d.fresh_ord)[e] = (d.epoch_base + ord[q]) | flag;
@@ -399,13 +458,12 @@ __device__ __forceinline__ void finish_one(const InstDesc& d, const Key& key, ui
     (void)wave_excl_scan_u32((uint32_t)claimed, &ctot);
     const uint64_t fail = __ballot(r == PR_FAIL);
     if (ftot | ctot | (fail ? 1u : 0u)) {
-        unsigned long long base = 0;
         if (lane == 0) {
-            if (ftot) base = add64(glob(&d.ctr->fresh_count), ftot);
             if (ctot) count_claims(d, ctot);
             if (fail) or32(glob(&d.ctr->overflow), 1u);
         }
-        base = bcast64(base, 0) + fpos;
+        const FreshPos fp = fresh_positions(d, ftot, lane);
+        const uint64_t base = fpos < fp.rem ? fp.p0 + fpos : fp.p1 + (fpos - fp.rem);
         if (fresh) {
             if (base < d.fresh_cap) {
                 glob(d.fresh_ord)[base] = (d.epoch_base + ord) | (mismatch ? 0ull : kMatchBit);
@@ -730,6 +788,7 @@ __device__ __forceinline__ int seg_find(const Seg* segs, int nseg, uint64_t g) {
 // pass reports a matching winner below found_below.
 __device__ __forceinline__ void resolve_entry(const InstDesc& d, uint64_t e) {
     const uint64_t of = glob(d.fresh_ord)[e];
+    if (of == kEmpty) return;   // an entry a wavefront reserved and left unused (fresh_chunk)
     // a pass before the last only looks for a settled match
     if (!d.mark && !(of & kMatchBit)) return;
     const uint32_t slot = glob(d.fresh_slot)[e];
@@ -831,6 +890,7 @@ __device__ void scan_one(const InstDesc& d) {
                 // the epoch's outcome straight to host memory: no read-back copy
                 Ctr* h = d.ctr_host;
                 h->fresh_count = d.ctr->fresh_count;
+                h->fresh_real = d.ctr->fresh_real;
                 h->claims = d.ctr->claims;
                 h->found = f;
                 h->winners = carry;
