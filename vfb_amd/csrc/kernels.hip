@@ -567,9 +567,16 @@ __global__ void k_fill_multi(const OpSeg* __restrict__ segs, int n, uint64_t tot
     }
 }
 
+// n words of value v: 16-byte stores, coalesced across the wavefront (p is
+// 256-byte aligned; an odd last word on its own)
 __global__ void k_fill(unsigned long long* p, uint64_t n, unsigned long long v) {
+    typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+    const u64x2 v2 = {v, v};
+    GLB u64x2* p2 = (GLB u64x2*)glob(p);
+    const uint64_t n2 = n / 2;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n2; i += stride) p2[i] = v2;
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) glob(p)[n - 1] = v;
 }
 
 // per-instance offsets of the fresh lists in one flat index space, and its
