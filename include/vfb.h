@@ -312,6 +312,18 @@ typedef struct {
                                       new) are built one after another by one workgroup with
                                       the solve's whole hash set in LDS, before any other
                                       path; 0: off (default 1) */
+    uint32_t set_margin_pct;       /* a level's hash set is sized for the keys it holds plus
+                                      the level's candidates times the last level's survivor
+                                      ratio times set_margin_pct / 100; a level that claims
+                                      more is built again in a larger set (a retry).  Memory
+                                      and retries only, never a result; 1..1000 (default 130) */
+    uint32_t pad_tuning;
+    uint64_t direct_min;           /* a level of at least direct_min candidates keeps no
+                                      fresh list: the expansion marks each new key's
+                                      candidate, and the one a lower ordinal displaces, in
+                                      byte maps that the count kernel packs (no resolve
+                                      pass over the set); smaller levels list their new
+                                      keys and resolve them; >= 1 (default 65536) */
 } vfb_tuning;
 
 void vfb_tuning_default(vfb_tuning* t);

@@ -1,12 +1,22 @@
-"""The per-instance counters of large levels: launches over one instance add
-their claims up per workgroup (InstDesc::defer_claims) and, on levels of
-2^22 candidates and more, reserve fresh-list entries 256 at a time per
-wavefront, leaving hole markers the resolve skips (InstDesc::fresh_chunk).
-A batch that does not park its large members (vfb_tuning.park_min = 0)
-builds the same levels in launches over several instances, which update
-the counters once per wavefront: every result field must agree, field by
-field, and the plain solves must agree with the OpenMP build.
+"""Large levels: winners marked during the expansion (InstDesc::direct) and the
+per-instance counters of launches over one instance.
+
+A level of 2^16 candidates and more keeps no fresh list: each fresh
+candidate marks its byte, a candidate that lowers a slot's ordinal of the
+level marks the displaced one lost, and the count kernel packs "marked and
+not lost" into the winner bitmap.  Launches over one instance add their
+claims and fresh candidates up per workgroup (InstDesc::defer_claims); a
+batch that does not park its large members (vfb_tuning.park_min = 0) builds
+the same levels in launches over several instances, which update the
+counters once per wavefront.  Every result field must agree, field by
+field, with the plain solves and with the OpenMP build, also where the
+level is built in a probe's passes with relaxed keys (the probe's match can
+lose its key to a lower ordinal, and the level's marks then stay for the
+pass that builds the rest) and where a set overflow builds a level again
+(its marks cleared first).
 """
+import random
+
 import pytest
 
 import oracle_lib as O
@@ -16,6 +26,7 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("status", "min_size", "formula", "distinct", "candidates", "level_winners", "level_candidates",
           "otf_level", "otf_stored", "found_ordinal", "levels_done", "top_op", "table_slots")
+DIRECT_MIN = 1 << 16
 
 
 @pytest.fixture
@@ -33,7 +44,7 @@ def _large():
 
 
 def test_large_levels_unparked_batch_equals_plain(tuning):
-    print("[counters] plain (per-workgroup claims, chunked fresh lists) against an unparked batch", flush=True)
+    print("[counters] plain (per-workgroup counts, direct marks) against an unparked batch", flush=True)
     cases = _large()
     plain = [V.solve(i, max_size=m) for i, m in cases]
     assert max(max(r["level_candidates"]) for r in plain) >= 1 << 22
@@ -53,12 +64,73 @@ def test_large_levels_unparked_batch_equals_plain(tuning):
 
 @pytest.mark.parametrize("k", [2, 3])
 def test_config2_large_levels_equal_omp(k):
-    """Config-2 instances whose largest levels (1-5 M candidates) take the
-    chunked fresh lists, against the OpenMP build."""
+    """Config-2 instances whose largest levels (1-5 M candidates) are marked
+    directly, against the OpenMP build."""
     print(f"[counters] config-2 #{k} max 14 against the OpenMP build", flush=True)
     inst = V.Instance.generate(2, k)
     names, lens, atoms, P = inst.arrays()
     want, _ = O.enumerate_omp(O.Instance(names, lens, atoms, P), 14, 3)
     got = V.solve(inst, max_size=14)
+    assert max(got["level_candidates"]) >= DIRECT_MIN
     for f in ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal"):
         assert got[f] == want[f], (k, f, got[f], want[f])
+
+
+@pytest.mark.parametrize("rt", [1, 2, 3])
+@pytest.mark.parametrize("k", [2, 3])
+def test_direct_levels_in_probe_passes_equal_omp(tuning, k, rt):
+    """Every level of a binary operator probed first (pass_min = 1), so that
+    a found level is built up to the probe's match and, when the match lost
+    its relaxed key to a lower ordinal, then above it."""
+    print(f"[counters] config-2 #{k} max 12 relaxed {rt}, every level probed", flush=True)
+    inst = V.Instance.generate(2, k)
+    names, lens, atoms, P = inst.arrays()
+    want, _ = O.enumerate_omp(O.Instance(names, lens, atoms, P), 12, rt)
+    V.set_tuning(pass_min=1)
+    got = V.solve(inst, max_size=12, relaxed_type=rt)
+    assert max(got["level_candidates"]) >= DIRECT_MIN
+    for f in ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "top_op"):
+        assert got[f] == want[f], (k, rt, f, got[f], want[f])
+
+
+def test_direct_levels_built_again_after_set_overflow(tuning):
+    """Sets sized for 1 % of the expected survivors overflow at the large
+    levels: each such level is built again in a larger set, its marks cleared
+    first, and every result but the set's size equals the default's."""
+    cases = [(V.Instance.generate(2, 2), 14), (V.Instance.generate(2, 3), 14), (V.Instance.generate(4, 219), 13)]
+    want = [V.solve(i, max_size=m) for i, m in cases]
+    V.set_tuning(set_margin_pct=1, table_min_slots=1 << 17)
+    print("[counters] set margin 1 %: overflowing large levels", flush=True)
+    got = [V.solve(i, max_size=m) for i, m in cases]
+    assert sum(g["retries"] for g in got) > 0
+    for (i, m), w, g in zip(cases, want, got):
+        assert max(g["level_candidates"]) >= DIRECT_MIN
+        for f in FIELDS[:-1]:
+            assert g[f] == w[f], (m, f, g[f], w[f])
+    print("[counters] the same in one batch", flush=True)
+    V.set_tuning(park_min=0)
+    got = V.solve_batch([i for i, m in cases[:2]], max_size=14)
+    for w, g in zip(want[:2], got):
+        for f in FIELDS[:-1]:
+            assert g[f] == w[f], (f, g[f], w[f])
+
+
+@pytest.mark.parametrize("rt", [1, 2])
+def test_direct_every_level_relaxed_keys_equal_oracle(tuning, rt):
+    """Random instances with relaxed keys, every level marked directly and
+    every level of a binary operator probed first: the probe's match loses
+    its key to a lower ordinal where two CS share a relaxed key."""
+    V.set_tuning(pass_min=1, direct_min=1)
+    rng = random.Random(5150 + rt)
+    for seed in range(12):
+        T = rng.randint(2, 24)
+        lens = [rng.randint(1, 64 if seed % 2 else 10) for _ in range(T)]
+        A = rng.randint(1, 4)
+        atoms = [[sum((rng.random() < 0.5) << j for j in range(lens[t])) for t in range(T)] for _ in range(A)]
+        P = rng.randint(1, T - 1)
+        names = [f"p{a}" for a in range(A)]
+        print(f"[counters] relaxed {rt} random {seed}, direct everywhere", flush=True)
+        want = O.enumerate_cs(O.Instance(names, lens, atoms, P), 7, rt)
+        got = V.solve(V.Instance.from_bits(names, lens, atoms, P), max_size=7, relaxed_type=rt)
+        for f in ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "top_op"):
+            assert got[f] == want[f], (rt, seed, f, got[f], want[f])

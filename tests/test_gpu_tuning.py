@@ -31,6 +31,10 @@ SETTINGS = {
     "high_load_wide_store": dict(pass_min=1, max_load_pct=90, store_traces=64, check_rows=4,
                                  wide_max_waves=0, table_min_slots=1 << 20),
     "check_one_row": dict(pass_min=1, check_rows=1, wide_max_waves=0),
+    # winners marked by the expansion at every level, or at none (fresh
+    # lists and resolves throughout)
+    "direct_everywhere": dict(pass_min=1, direct_min=1),
+    "never_direct": dict(direct_min=1 << 40),
 }
 
 FIELDS = ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "otf_level", "otf_stored")

@@ -155,7 +155,8 @@ def test_tuning_set_get_and_range_checks():
         for bad in ({"expand_blocks_per_cu": 0}, {"resolve_blocks_per_cu": 65}, {"store_blocks_per_cu": 0},
                     {"max_load_pct": 49}, {"max_load_pct": 91}, {"store_traces": 0}, {"store_traces": 129},
                     {"check_rows": 2}, {"check_rows": 32}, {"table_min_slots": (1 << 32) + 1},
-                    {"chain_blocks": 65}, {"chain_cands": 63}, {"chain_cands": 65537}):
+                    {"chain_blocks": 65}, {"chain_cands": 63}, {"chain_cands": 65537},
+                    {"set_margin_pct": 0}, {"set_margin_pct": 1001}, {"direct_min": 0}):
             with pytest.raises(V.VfbError):
                 V.set_tuning(**bad)
             assert V.get_tuning() == got

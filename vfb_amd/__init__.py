@@ -60,10 +60,11 @@ class Tuning(C.Structure):
                 ("store_blocks_per_cu", C.c_uint32), ("park_min", C.c_uint64),
                 ("table_min_slots", C.c_uint64), ("max_load_pct", C.c_uint32),
                 ("store_traces", C.c_uint32), ("check_rows", C.c_uint32), ("chain_blocks", C.c_uint32),
-                ("chain_cands", C.c_uint32), ("lds_chain", C.c_uint32)]
+                ("chain_cands", C.c_uint32), ("lds_chain", C.c_uint32), ("set_margin_pct", C.c_uint32),
+                ("pad_tuning", C.c_uint32), ("direct_min", C.c_uint64)]
 
 
-TUNING_FIELDS = [f for f, _ in Tuning._fields_]
+TUNING_FIELDS = [f for f, _ in Tuning._fields_ if not f.startswith("pad")]
 
 
 class KernelStats(C.Structure):

@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
         d.ord_lo = 0;
         d.ord_hi = kEmpty;
         d.defer_claims = 0;   // claims go to the counter at once (read back at the level's end)
-        d.fresh_chunk = 0;
+        d.direct = 0;
     }
     const bool lead = rank == 0 && threadIdx.x == 0;
     if (lead) out.t0 = wall_clock64();
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kChainThreads) void k_chain(ChainIn* ins, ChainOut*
                     used + (nbytes < 256 ? 256 : nbytes) > io.slab_bytes ||
                     (io.pass_min && N >= io.pass_min && s >= 3);   // probed on the host (a split exists)
         if (!stop) {
-            const double f = win_ratio * 1.3;
+            const double f = win_ratio * io.margin;
             const double m = f < 1.0 ? f : 1.0;
             uint64_t est = (uint64_t)((double)N * m);
             if (est < 4096) est = 4096;
@@ -664,7 +664,7 @@ __global__ __launch_bounds__(kLThreads) void k_chain_lds(ChainIn* ins, ChainOut*
         d.ord_hi = kEmpty;
         d.ctr_host = nullptr;
         d.defer_claims = 0;
-        d.fresh_chunk = 0;
+        d.direct = 0;
         sh.nkeys = 0;
         sh.ovf = 0;
     }
@@ -716,7 +716,7 @@ __global__ __launch_bounds__(kLThreads) void k_chain_lds(ChainIn* ins, ChainOut*
                     committed + N > kLKeys || used + (nbytes < 256 ? 256 : nbytes) > io.slab_bytes ||
                     (io.pass_min && N >= io.pass_min && s >= 3);
         if (!stop) {
-            const double f = win_ratio * 1.3;
+            const double f = win_ratio * io.margin;
             const double m = f < 1.0 ? f : 1.0;
             uint64_t est = (uint64_t)((double)N * m);
             if (est < 4096) est = 4096;

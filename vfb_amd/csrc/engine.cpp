@@ -301,7 +301,7 @@ void Engine::release_all(Inst& I) {
     dfree(I, I.cols);
     for (auto& b : I.slabs) dfree(I, b);
     I.slabs.clear();
-    I.fresh_cap = I.nwords_cap = I.nblk_cap = 0;
+    I.fresh_cap = I.nwords_cap = I.nblk_cap = I.lost_words = 0;
     I.cap = 0;
 }
 
@@ -769,23 +769,15 @@ static void pass_work(const Inst& I, uint64_t* cand, uint64_t* bytes) {
     }
 }
 
-// Fresh-list entries a large level holds beyond its estimate for the ones
-// its wavefronts reserve and leave unused: a launch over that instance alone
-// lets each wavefront reserve kFreshChunk entries at a time (one atomic per
-// chunk on the instance's counter instead of one per wavefront), and each
-// of at most two appending passes leaves at most one partial chunk per
-// wavefront of its grid (at most 64 workgroups of 4 wavefronts per CU, the
-// largest expand_blocks_per_cu).  Sized from the level alone, not from the
-// batch or the tuning, so that plain, batched and served solves hold the
-// same memory under every setting.
-uint64_t Engine::fresh_reserve(const Inst& I) const {
-    if (fixed_ || I.N < kFreshChunkMin) return 0;
-    return 2ull * (uint64_t)cus_ * 64 * 4 * kFreshChunk;
-}
+// Levels that mark their winners directly (InstDesc::direct): a fresh
+// candidate marks its byte and a candidate that displaces another of the
+// epoch marks that one lost, so no fresh list is written and no resolve reads
+// the set again.  The engine with fixed scratch keeps the fresh list.
+bool Engine::direct_admits(const Inst& I) const { return !fixed_ && I.N >= tun_.direct_min; }
 
-// fresh entries of the level's epoch (the reserved ones left unused excluded)
+// fresh candidates of the level's epoch
 uint64_t Engine::fresh_exact(const Inst& I) const {
-    return I.fresh_chunked ? I.ctr.fresh_real : I.ctr.fresh_count;
+    return I.h.direct ? I.ctr.fresh_real : I.ctr.fresh_count;
 }
 
 // scratch and hash set for a deduplicating level, sized from the previous
@@ -799,14 +791,16 @@ void Engine::prepare_scratch(Inst& I) {
     // chain built (the floors), so that memory use does not depend on the chain
     uint64_t fcap = std::min<uint64_t>(
         N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.fresh_ratio * 1.5)) + 4096));
-    fcap = std::max(fcap, I.floor_fcap) + fresh_reserve(I);
+    fcap = std::max(fcap, I.floor_fcap);
     const uint64_t want_nw = std::max(nwords, I.floor_nwords);
     const uint64_t want_nb = std::max(nblk, I.floor_nblk);
     I.floor_fcap = I.floor_nwords = I.floor_nblk = 0;
+    const bool direct = direct_admits(I);
     if (fcap > I.fresh_cap) {
         ensure(I, I.fresh_ord, fcap * 8);
         ensure(I, I.fresh_slot, fcap * 4);
         I.fresh_cap = fcap;
+        I.lost_words = 0;
     }
     if (want_nw > I.nwords_cap) {
         // at least 64K candidates' worth, so that a solve's small levels
@@ -818,6 +812,24 @@ void Engine::prepare_scratch(Inst& I) {
         fill(I, I.winbyte.p, nw * 8, 0);
         I.nwords_cap = nw;
     }
+    if (direct) {
+        // the loser bytes live in the fresh list's memory, which a direct
+        // level does not use: the level's memory (and so every budget
+        // decision) is that of a level with a fresh list, unless its bytes
+        // need more
+        if (I.fresh_ord.bytes < nwords * 64) {
+            ensure(I, I.fresh_ord, nwords * 64);
+            I.lost_words = 0;
+        }
+        if (I.lost_words < nwords) {
+            // cleared here as far as the level reaches (below lost_words they
+            // are clear); the count / scan kernels clear what they pack
+            fill(I, (char*)I.fresh_ord.p + I.lost_words * 64, (nwords - I.lost_words) * 8, 0);
+            I.lost_words = nwords;
+        }
+    } else {
+        I.lost_words = 0;   // the fresh list overwrites them
+    }
     if (want_nb > I.nblk_cap) {
         ensure(I, I.blkcount, std::max<uint64_t>(want_nb, 1) * 4);
         ensure(I, I.blkprefix, std::max<uint64_t>(want_nb, 1) * 8);
@@ -828,12 +840,14 @@ void Engine::prepare_scratch(Inst& I) {
     h.fresh_cap = I.fresh_cap;
     h.winbits = (unsigned long long*)I.winbits.p;
     h.winbyte = (unsigned char*)I.winbyte.p;
+    h.lostbyte = direct ? (unsigned char*)I.fresh_ord.p : nullptr;
+    h.direct = direct ? 1 : 0;
     h.nwords = nwords;
     h.blkcount = (uint32_t*)I.blkcount.p;
     h.blkprefix = (unsigned long long*)I.blkprefix.p;
     h.nblk = nblk;
     const uint64_t est_new =
-        std::min<uint64_t>(N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.win_ratio * 1.3))));
+        std::min<uint64_t>(N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.win_ratio * set_margin()))));
     ensure_table(I, I.committed + est_new, false);
 }
 
@@ -1183,14 +1197,9 @@ void Engine::epoch(std::vector<Inst*>& v) {
         hd[k] = I.h;
         hd[k].ctr_host = all_scan ? h_ctr_ + k : nullptr;
         hd[k].check_only = checks_only(I) ? 1 : 0;
-        // the launch holds this instance alone: its claims are added up per
-        // workgroup, and a large level's wavefronts reserve fresh entries in
-        // chunks (the list holds fresh_reserve(I) for their unused tails)
+        // the launch holds this instance alone: its claims (and a direct
+        // level's fresh candidates) are added up per workgroup
         hd[k].defer_claims = n == 1 ? 1 : 0;
-        const bool chunk = n == 1 && !checks_only(I) && fresh_reserve(I) > 0;
-        hd[k].fresh_chunk = chunk ? kFreshChunk : 0;
-        if (I.pass != P_REST) I.fresh_chunked = chunk;
-        else I.fresh_chunked = I.fresh_chunked || chunk;
         if (I.pass != P_REST) {
             hc[k] = Ctr{};          // first pass of the level's epoch: fresh counters
             hc[k].found = kEmpty;
@@ -1231,13 +1240,22 @@ void Engine::epoch(std::vector<Inst*>& v) {
         const bool single = n == 1;
         // one small instance at its last pass: resolve and scan in one workgroup
         const Inst& I0 = *v[0];
-        if (single && !checks_only(I0) && I0.h.nblk <= (uint64_t)kSmallChunks &&
+        bool all_direct = true;
+        for (auto* I : dd) all_direct &= I->h.direct != 0;
+        if (single && !checks_only(I0) && !I0.h.direct && I0.h.nblk <= (uint64_t)kSmallChunks &&
             I0.h.fresh_cap <= kSmallFresh) {
             CK(launch_resolve_scan1(D, st_));
+        } else if (all_direct) {
+            // winners marked by the expansion: pack, count and scan only
+            if (btot) {
+                int grid = (int)std::min<uint64_t>((btot + 3) / 4, (uint64_t)cus_ * 8);
+                CK(launch_count(D, (const Seg*)(d + o_bseg), (int)bsegs.size(), btot, grid, st_));
+            }
+            if (!ids.empty()) CK(launch_scan(D, (const uint32_t*)(d + o_ids), (int)ids.size(), st_));
         } else {
             if (!single) CK(launch_fresh_prefix(D, (Seg*)(d + o_fseg), (int)n, tot, st_));
             uint64_t fcap = 0;
-            for (auto* I : dd) fcap += I->h.fresh_cap;
+            for (auto* I : dd) fcap += I->h.direct ? 0 : I->h.fresh_cap;
             const int rgrid = (int)std::max<uint64_t>(
                 1, std::min<uint64_t>((fcap + 255) / 256, (uint64_t)cus_ * tun_.resolve_blocks_per_cu));
             CK(launch_resolve(D, (const Seg*)(d + o_fseg), (int)n, tot, single ? 1 : 0, rgrid, st_));
@@ -1329,7 +1347,7 @@ bool Engine::chain_admits(const Inst& I, uint64_t n_max) const {
     if (I.max_rows && I.N > (I.max_rows > I.stored ? I.max_rows - I.stored : 0)) return false;
     if (tun_.pass_min && I.N >= tun_.pass_min && nsplit_of(I) >= 1) return false;   // probed first, on the host
     const uint64_t est = std::min<uint64_t>(
-        I.N, std::max<uint64_t>(4096, (uint64_t)(I.N * std::min(1.0, I.win_ratio * 1.3))));
+        I.N, std::max<uint64_t>(4096, (uint64_t)(I.N * std::min(1.0, I.win_ratio * set_margin()))));
     const uint64_t start = std::max<uint64_t>(kMinSlots, tun_.table_min_slots);
     return start <= I.cap && (I.committed + est) * 100 / tun_.max_load_pct + 1 <= I.cap;
 }
@@ -1421,6 +1439,7 @@ void Engine::chain(std::vector<Inst*>& v, int G, uint64_t n_max, bool lds) {
         c.win_ratio = I.win_ratio;
         c.fresh_ratio = I.fresh_ratio;
         c.pct = tun_.max_load_pct;
+        c.margin = set_margin();
         c.level = I.level;
         c.last_level = I.max_size - 1;
         c.neg_type = I.neg_type;
@@ -1768,14 +1787,19 @@ void Engine::step(std::vector<Inst*>& act) {
                 if (I.done) continue;
                 if (I.ctr.overflow) {
                     I.res->retries++;
+                    if (I.h.direct) {
+                        // the level's marks so far: it is built again from its first pass
+                        fill(I, I.winbyte.p, I.h.nwords * 8, 0);
+                        fill(I, I.h.lostbyte, I.h.nwords * 8, 0);
+                    }
                     try {
                         if (I.ctr.overflow & 2) {
                             uint64_t fcap = std::min<uint64_t>(
-                                I.N, std::max<uint64_t>(I.fresh_cap * 4, I.ctr.fresh_count + 4096)) +
-                                            fresh_reserve(I);
+                                I.N, std::max<uint64_t>(I.fresh_cap * 4, I.ctr.fresh_count + 4096));
                             ensure(I, I.fresh_ord, fcap * 8);
                             ensure(I, I.fresh_slot, fcap * 4);
                             I.fresh_cap = fcap;
+                            I.lost_words = 0;
                             I.h.fresh_ord = (unsigned long long*)I.fresh_ord.p;
                             I.h.fresh_slot = (uint32_t*)I.fresh_slot.p;
                             I.h.fresh_cap = fcap;
@@ -2268,6 +2292,8 @@ extern "C" void vfb_tuning_default(vfb_tuning* t) {
     t->chain_blocks = 0;
     t->chain_cands = 1024;
     t->lds_chain = 1;
+    t->set_margin_pct = 130;
+    t->direct_min = kDirectMin;
 }
 
 // the ranges vfb_set_tuning accepts (vfb_tuning_load applies the same)
@@ -2277,7 +2303,8 @@ bool vfb::tuning_valid(const vfb_tuning& t) {
            t.table_min_slots <= (1ull << 32) && t.max_load_pct >= 50 && t.max_load_pct <= 90 && t.store_traces >= 1 &&
            t.store_traces <= 128 && (t.check_rows == 1 || t.check_rows == 4 || t.check_rows == 8 || t.check_rows == 16) &&
            t.chain_blocks <= (uint32_t)kChainMaxTeam && t.chain_cands >= 64 && t.chain_cands <= 65536 &&
-           t.lds_chain <= 1;
+           t.lds_chain <= 1 && t.set_margin_pct >= 1 && t.set_margin_pct <= 1000 &&
+           t.direct_min >= 1;
 }
 
 extern "C" void vfb_get_tuning(vfb_tuning* t) {

@@ -186,7 +186,7 @@ struct Inst {
     Buf cols;                   // multi-word rows: ColInfo[C] then tcol[T + 1]
     uint64_t fresh_cap = 0, nwords_cap = 0, nblk_cap = 0;
     double win_ratio = 1.0, fresh_ratio = 1.0;
-    bool fresh_chunked = false; // the level's epoch reserved fresh-list entries in chunks (ctr.fresh_real counts them)
+    uint64_t lost_words = 0;    // a direct level's loser bytes (in fresh_ord) cleared for this many bitmap words
     uint64_t N = 0;             // candidates of the level being built
     std::vector<Task> tasks;
     Ctr ctr{};
@@ -338,7 +338,8 @@ class Engine {
     void prepare_or_check(Inst& I, std::vector<Inst*>& step);
     bool chain_admits(const Inst& I, uint64_t n_max) const;
     bool lds_chain_admits(const Inst& I) const;
-    uint64_t fresh_reserve(const Inst& I) const;
+    bool direct_admits(const Inst& I) const;
+    double set_margin() const { return tun_.set_margin_pct / 100.0; }
     uint64_t fresh_exact(const Inst& I) const;
     void chain(std::vector<Inst*>& v, int G, uint64_t n_max, bool lds = false);
     bool chain_adopt(Inst& I, const ChainOut& o);

@@ -44,8 +44,7 @@ constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are coun
 constexpr uint64_t kSmallFresh = 4096;  // fresh lists up to this long are resolved in the scan's workgroup
 constexpr uint64_t kMinWinWords = 1024; // winner maps are allocated for at least 64K candidates
 constexpr uint64_t kMinSlots = 1ull << 17; // hash sets start at 4 MB
-constexpr int kFreshChunk = 256;      // fresh-list entries a wavefront reserves at a time (InstDesc::fresh_chunk)
-constexpr uint64_t kFreshChunkMin = 1ull << 22;  // levels of this many candidates reserve them so
+constexpr uint64_t kDirectMin = 1ull << 16;  // default vfb_tuning.direct_min (InstDesc::direct)
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 
@@ -65,15 +64,14 @@ struct ColInfo {
 };
 
 struct Ctr {
-    unsigned long long fresh_count;  // entries appended to the fresh list this epoch (with
-                                     // InstDesc::fresh_chunk: entries reserved, unused ones included)
+    unsigned long long fresh_count;  // entries appended to the fresh list this epoch (none with InstDesc::direct)
     unsigned long long claims;       // slots claimed this epoch (= distinct new keys)
     unsigned long long found;        // lowest matching winner, level-local ordinal
     unsigned long long winners;      // survivors of the epoch (set by the scan)
     unsigned long long found_rank;   // survivors ordered before `found`
     unsigned int overflow;           // 1 probe limit, 2 fresh list full, 4 load limit
     unsigned int pad;
-    unsigned long long fresh_real;   // with InstDesc::fresh_chunk: the entries appended
+    unsigned long long fresh_real;   // with InstDesc::direct: the fresh candidates (marked, not listed)
 };
 
 // One segment of a level's canonical candidate order: operator `op` applied to
@@ -99,8 +97,9 @@ struct InstDesc {
     int store_traces; // traces per k_store work item
     int defer_claims; // the launch holds this instance alone: k_expand adds up its claims per
                       // workgroup in LDS and checks the set's load limit at the workgroup's end
-    int fresh_chunk;  // > 0 (with defer_claims): each wavefront reserves fresh-list entries this
-                      // many at a time; entries it leaves unused hold kEmpty (kFreshHole)
+    int direct;       // winners marked during the expansion: a fresh candidate sets its byte in
+                      // winbyte (bit 0, bit 1 = match), one that lowers a slot's ordinal of this
+                      // epoch sets the displaced ordinal's byte in lostbyte; no fresh list, no resolve
     int pad_desc;
     uint64_t lenmask[kMaxT];
     uint64_t kmask[kMaxT];      // KEY_FIELD: low bits of the trace word that enter the key
@@ -122,7 +121,9 @@ struct InstDesc {
     unsigned long long* fresh_ord;
     uint32_t* fresh_slot;
     uint64_t fresh_cap;
-    unsigned char* winbyte;      // one byte per candidate, set by the resolve (plain stores)
+    unsigned char* winbyte;      // one byte per candidate, set by the resolve (plain stores), or
+                                 // by the expansion (direct)
+    unsigned char* lostbyte;     // direct: one byte per candidate, set when it was displaced
     unsigned long long* winbits; // the same, packed to bits by the count / scan kernels
     uint64_t nwords;
     uint32_t* blkcount;
@@ -237,6 +238,8 @@ struct ChainIn {
     uint64_t pass_min;          // levels the host probes first (vfb_tuning.pass_min) are left to it
     double win_ratio;           // survivors / candidates of the last completed level
     double fresh_ratio;         // fresh entries / candidates of the last completed level
+    double margin;              // the set holds the next level's survivors estimated as
+                                // candidates * win_ratio * margin (vfb_tuning.set_margin_pct)
     uint32_t pct;               // set load (percent) at which the host grows the set
     int32_t level;              // last completed level
     int32_t last_level;         // the chain builds levels up to this one

@@ -6,14 +6,20 @@
 //               (~,X,F,G per row; &,|,lUr,rUl per pair).  Computes the CS words
 //               trace by trace, folds them into the 128-bit key and the target
 //               mismatch bits, then probes the hash set.  Candidates whose key
-//               is new to the set are appended to the fresh list.
+//               is new to the set are appended to the fresh list; on a direct
+//               level they mark their own byte of the winner map instead, and
+//               one that lowers a slot's ordinal of this epoch marks the
+//               displaced ordinal's byte of the loser map.
 //               Reference: processOperator + applyOperator + makeRlxUnqChkCSs +
 //               processUniqueCS + insertInCache (LTL_BS.cu:30-110, 183-238,
 //               251-269, 299-329, 383-446).
 //   k_resolve : a fresh candidate survives iff it holds the lowest ordinal
 //               of its key (slot.ord == own ordinal); survivors set their bit in
 //               the level's winner bitmap; matching survivors atomicMin `found`.
-//   k_count / k_scan : popcount blocks of the bitmap, exclusive scan.
+//               (A direct level has no fresh list and no resolve.)
+//   k_count / k_scan : pack the bytes to the bitmap (direct: marked and not
+//               displaced, and the lowest matching one is `found`), popcount
+//               blocks of it, exclusive scan.
 //   k_store   : stable compaction: survivors in ordinal order, CS recomputed
 //               and written column-major with their ordinal as parent record
 //               (replaces the Thrust remove/copy passes, LTL_BS.cu:489-530).
@@ -450,7 +456,8 @@ __device__ void insert_row(const InstDesc& d, uint64_t x, uint32_t n, unsigned i
     key_fin(k);
     uint32_t slot;
     int claimed = 0;
-    if (probe_insert(glob(d.table), d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed) == PR_FAIL) or32(glob(fail), 1u);
+    uint64_t displaced;
+    if (probe_insert(glob(d.table), d.table_mask, k.hi, k.lo, x, 0, &slot, &claimed, &displaced) == PR_FAIL) or32(glob(fail), 1u);
 }
 
 // BS stage: bit 0 of every trace of each stored row, packed into one word
@@ -472,7 +479,8 @@ __global__ void k_insert_keys(Slot* tab, uint64_t mask, const unsigned long long
     if (i >= n) return;
     uint32_t slot;
     int claimed = 0;
-    if (probe_insert(glob(tab), mask, 0ull, keys[i], i, 0, &slot, &claimed) == PR_FAIL) or32(glob(fail), 1u);
+    uint64_t displaced;
+    if (probe_insert(glob(tab), mask, 0ull, keys[i], i, 0, &slot, &claimed, &displaced) == PR_FAIL) or32(glob(fail), 1u);
 }
 
 // the atoms of many instances in one launch (segment k = instance k's atoms)
@@ -641,7 +649,7 @@ __global__ __launch_bounds__(256) void k_count(const InstDesc* __restrict__ D, c
         const Seg& sg = segs[seg_find(segs, nseg, g)];
         const InstDesc& d = D[sg.inst];
         const uint64_t c = g - sg.begin;
-        uint32_t v = pack_word(d, c * kChunkWords + lane);
+        uint32_t v = pack_word(d, c * kChunkWords + lane, keep_marks(d));
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (lane == 0) glob(d.blkcount)[c] = v;
     }

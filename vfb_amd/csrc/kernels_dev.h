@@ -177,11 +177,9 @@ enum { PR_DUP = 0, PR_FRESH = 1, PR_FAIL = 2 };
 // flush once per workgroup (flush_claims).  One counter per instance takes
 // every wavefront's update otherwise, and atomics on one address run at
 // about 83 M/s on MI355X (measured), below the rate the expansion of a large
-// level issues them.
-__shared__ unsigned long long g_pending_claims;
-// with InstDesc::fresh_chunk: each wavefront's reserved range [cur, end) of
-// the fresh list, and the entries the workgroup appended (not yet counted)
-__shared__ unsigned long long g_fresh_cur[4], g_fresh_end[4], g_fresh_real;
+// level issues them.  The same holds for the fresh candidates a direct
+// level counts.
+__shared__ unsigned long long g_pending_claims, g_pending_fresh;
 
 __device__ __forceinline__ void count_claims(const InstDesc& d, uint32_t ctot) {
     if (d.defer_claims) {
@@ -192,30 +190,22 @@ __device__ __forceinline__ void count_claims(const InstDesc& d, uint32_t ctot) {
     }
 }
 
-// start and end of an expansion kernel: the workgroup's deferred claims into
-// the counter of the launch's one instance, with the load-limit check the
+__device__ __forceinline__ void count_fresh(const InstDesc& d, uint32_t ftot) {
+    if (d.defer_claims) atomicAdd(&g_pending_fresh, (unsigned long long)ftot);
+    else add64(glob(&d.ctr->fresh_real), ftot);
+}
+
+// start and end of an expansion kernel: the workgroup's deferred counts into
+// the counters of the launch's one instance, with the load-limit check the
 // undeferred update makes (the last workgroup to flush sees the total)
 __device__ __forceinline__ void begin_claims() {
     if (threadIdx.x == 0) {
         g_pending_claims = 0;
-        g_fresh_real = 0;
+        g_pending_fresh = 0;
     }
-    if (threadIdx.x < 4) g_fresh_cur[threadIdx.x] = g_fresh_end[threadIdx.x] = 0;
     __syncthreads();
 }
 __device__ __forceinline__ void flush_claims(const InstDesc* D, const Task* tasks, int ntasks) {
-    if (ntasks > 0) {
-        const InstDesc& d = D[tasks[0].inst];
-        if (d.fresh_chunk) {
-            // this wavefront's unused reserved entries: holes the resolve skips
-            const int w = (int)(threadIdx.x >> 6);
-            const uint64_t end = g_fresh_end[w] < d.fresh_cap ? g_fresh_end[w] : d.fresh_cap;
-            for (uint64_t e = g_fresh_cur[w] + (threadIdx.x & 63); e < end; e += 64) {
-                glob(d.fresh_ord)[e] = kEmpty;
-                glob(d.fresh_slot)[e] = 0u;
-            }
-        }
-    }
     __syncthreads();
     if (threadIdx.x == 0 && ntasks > 0) {
         const InstDesc& d = D[tasks[0].inst];
@@ -224,46 +214,8 @@ __device__ __forceinline__ void flush_claims(const InstDesc* D, const Task* task
             const unsigned long long before = add64(glob(&d.ctr->claims), c);
             if (before + c > d.claim_limit) or32(glob(&d.ctr->overflow), 4u);
         }
-        if (g_fresh_real) add64(glob(&d.ctr->fresh_real), g_fresh_real);
+        if (g_pending_fresh) add64(glob(&d.ctr->fresh_real), g_pending_fresh);
     }
-}
-
-// Fresh-list positions of a wavefront's ftot fresh entries (wave-uniform):
-// entry i (0 <= i < ftot, in lane order) goes to p0 + i for i < rem, else to
-// p1 + i - rem.  One atomic on the instance's counter per wavefront, or
-// (fresh_chunk) one per fresh_chunk entries the wavefront appends.
-struct FreshPos {
-    uint64_t p0, p1;
-    uint32_t rem;
-};
-__device__ __forceinline__ FreshPos fresh_positions(const InstDesc& d, uint32_t ftot, int lane) {
-    FreshPos fp{0, 0, ftot};
-    if (!ftot) return fp;
-    if (!d.fresh_chunk) {
-        unsigned long long b = 0;
-        if (lane == 0) b = add64(glob(&d.ctr->fresh_count), ftot);
-        fp.p0 = bcast64(b, 0);
-        return fp;
-    }
-    const int w = (int)(threadIdx.x >> 6);
-    const uint64_t cur = g_fresh_cur[w], end = g_fresh_end[w];
-    fp.p0 = cur;
-    fp.rem = (uint32_t)(end - cur < ftot ? end - cur : ftot);
-    unsigned long long b = 0;
-    if (fp.rem < ftot && lane == 0) b = add64(glob(&d.ctr->fresh_count), (unsigned long long)d.fresh_chunk);
-    fp.p1 = bcast64(b, 0);
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
-        if (fp.rem < ftot) {
-            g_fresh_cur[w] = fp.p1 + (ftot - fp.rem);
-            g_fresh_end[w] = fp.p1 + (uint64_t)d.fresh_chunk;
-        } else {
-            g_fresh_cur[w] = cur + ftot;
-        }
-        atomicAdd(&g_fresh_real, (unsigned long long)ftot);
-    }
-    __builtin_amdgcn_wave_barrier();
-    return fp;
 }
 
 // Insert-or-find of a 128-bit key with lowest-ordinal tracking, lock-free:
@@ -277,9 +229,11 @@ __device__ __forceinline__ FreshPos fresh_positions(const InstDesc& d, uint32_t 
 // slot's fields (th, tl, to) are passed in, loaded ahead by the caller.
 // PR_FRESH only for a candidate that may still hold the key's lowest
 // ordinal: one that sees a smaller ordinal of this epoch has certainly lost.
+// A fresh candidate that lowered the ordinal returns the one it displaced in
+// *displaced (kEmpty otherwise).
 __device__ __forceinline__ int probe_from(GLB Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo, uint64_t gord,
                                           uint64_t epoch_base, uint64_t i, uint64_t th, uint64_t tl,
-                                          uint64_t to, uint32_t* slot_out, int* claimed) {
+                                          uint64_t to, uint32_t* slot_out, int* claimed, uint64_t* displaced) {
     for (int n = 0; n < kProbeLimit; ++n) {
         GLB Slot* s = tab + i;
         if (to == kEmpty) {
@@ -301,7 +255,9 @@ __device__ __forceinline__ int probe_from(GLB Slot* tab, uint64_t mask, uint64_t
             *slot_out = (uint32_t)i;
             if (to < gord) return PR_DUP;      // older key, or a smaller ordinal of this epoch
             const uint64_t old = min64(&s->ord, gord);
-            return old < gord ? PR_DUP : PR_FRESH;
+            if (old < gord) return PR_DUP;
+            *displaced = old;
+            return PR_FRESH;
         }
         i = (i + 1) & mask;
         s = tab + i;
@@ -314,11 +270,11 @@ __device__ __forceinline__ int probe_from(GLB Slot* tab, uint64_t mask, uint64_t
 
 __device__ __forceinline__ int probe_insert(GLB Slot* tab, uint64_t mask, uint64_t hi, uint64_t lo,
                                             uint64_t gord, uint64_t epoch_base,
-                                            uint32_t* slot_out, int* claimed) {
+                                            uint32_t* slot_out, int* claimed, uint64_t* displaced) {
     const uint64_t i = slot_hash(hi, lo) & mask;
     GLB Slot* s = tab + i;
     return probe_from(tab, mask, hi, lo, gord, epoch_base, i, ld_any(&s->hi), ld_any(&s->lo), ld_any(&s->ord),
-                      slot_out, claimed);
+                      slot_out, claimed, displaced);
 }
 
 __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t* total) {
@@ -336,14 +292,15 @@ __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t* tot
 // machines -- per slot visited, one round of ord-CAS (a claimer stores its key
 // and is done) and one of atomicMin, each round issuing the atomics of all
 // four candidates together so that their latencies overlap -- then appends the
-// fresh ones with one atomic per wavefront.  Per-candidate semantics are those
+// fresh ones with one atomic per wavefront (a direct level marks them in the
+// winner and loser bytes instead).  Per-candidate semantics are those
 // of probe_from().
 __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* key, const uint64_t* ord,
                                                   uint32_t mismatch, uint32_t enable, int lane) {
     GLB Slot* tab = glob(d.table);
     const uint64_t mask = d.table_mask;
-    uint64_t idx[4], th[4], tl[4], to[4];
-    uint32_t act = enable, fresh = 0, claimed = 0;
+    uint64_t idx[4], th[4], tl[4], to[4], disp[4];
+    uint32_t act = enable, fresh = 0, claimed = 0, lowered = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         idx[q] = slot_hash(key[q].hi, key[q].lo) & mask;
@@ -388,7 +345,11 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
         for (int q = 0; q < 4; ++q)
             if ((need_min >> q) & 1u) {
                 act &= ~(1u << q);
-                if (r[q] >= d.epoch_base + ord[q]) fresh |= 1u << q;
+                if (r[q] >= d.epoch_base + ord[q]) {
+                    fresh |= 1u << q;
+                    lowered |= 1u << q;
+                    disp[q] = r[q];
+                }
             }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -404,6 +365,14 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
                 }
             }
     }
+    if (d.direct) {
+        // the candidate's own byte, and the byte of the ordinal it displaced
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if ((fresh >> q) & 1u) glob(d.winbyte)[ord[q]] = ((mismatch >> q) & 1u) ? 1 : 3;
+            if ((lowered >> q) & 1u) glob(d.lostbyte)[disp[q] - d.epoch_base] = 1;
+        }
+    }
     // one fresh-list reservation and one claims update per wavefront
     uint32_t ftot = 0, ctot = 0;
     const uint32_t fbase = wave_excl_scan_u32((uint32_t)__popc(fresh), &ftot);
@@ -413,14 +382,17 @@ __device__ __forceinline__ void finish_candidates(const InstDesc& d, const Key* 
         if (lane == 0) {
             if (ctot) count_claims(d, ctot);
             if (fail) or32(glob(&d.ctr->overflow), 1u);
+            if (d.direct && ftot) count_fresh(d, ftot);
         }
-        const FreshPos fp = fresh_positions(d, ftot, lane);
+        if (d.direct) return;
+        unsigned long long fb = 0;
+        if (ftot && lane == 0) fb = add64(glob(&d.ctr->fresh_count), ftot);
+        const uint64_t f0 = bcast64(fb, 0);
         uint32_t k = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             if ((fresh >> q) & 1u) {
-                const uint32_t i = fbase + k++;
-                const uint64_t e = i < fp.rem ? fp.p0 + i : fp.p1 + (i - fp.rem);
+                const uint64_t e = f0 + fbase + k++;
                 if (e < d.fresh_cap) {
                     const uint64_t flag = ((mismatch >> q) & 1u) ? 0ull : kMatchBit;
                     glob(d.fresh_ord)[e] = (d.epoch_base + ord[q]) | flag;
@@ -451,8 +423,15 @@ __device__ __forceinline__ void finish_one(const InstDesc& d, const Key& key, ui
     uint32_t slot = 0xffffffffu;
     int claimed = 0;
     int r = PR_DUP;
-    if (en) r = probe_insert(glob(d.table), d.table_mask, key.hi, key.lo, d.epoch_base + ord, d.epoch_base, &slot, &claimed);
+    uint64_t displaced = kEmpty;
+    if (en)
+        r = probe_insert(glob(d.table), d.table_mask, key.hi, key.lo, d.epoch_base + ord, d.epoch_base, &slot,
+                         &claimed, &displaced);
     const bool fresh = r == PR_FRESH;
+    if (d.direct && fresh) {
+        glob(d.winbyte)[ord] = mismatch ? 1 : 3;
+        if (displaced != kEmpty) glob(d.lostbyte)[displaced - d.epoch_base] = 1;
+    }
     uint32_t ftot = 0, ctot = 0;
     const uint32_t fpos = wave_excl_scan_u32(fresh ? 1u : 0u, &ftot);
     (void)wave_excl_scan_u32((uint32_t)claimed, &ctot);
@@ -461,9 +440,12 @@ __device__ __forceinline__ void finish_one(const InstDesc& d, const Key& key, ui
         if (lane == 0) {
             if (ctot) count_claims(d, ctot);
             if (fail) or32(glob(&d.ctr->overflow), 1u);
+            if (d.direct && ftot) count_fresh(d, ftot);
         }
-        const FreshPos fp = fresh_positions(d, ftot, lane);
-        const uint64_t base = fpos < fp.rem ? fp.p0 + fpos : fp.p1 + (fpos - fp.rem);
+        if (d.direct) return;
+        unsigned long long fb = 0;
+        if (ftot && lane == 0) fb = add64(glob(&d.ctr->fresh_count), ftot);
+        const uint64_t base = bcast64(fb, 0) + fpos;
         if (fresh) {
             if (base < d.fresh_cap) {
                 glob(d.fresh_ord)[base] = (d.epoch_base + ord) | (mismatch ? 0ull : kMatchBit);
@@ -788,7 +770,6 @@ __device__ __forceinline__ int seg_find(const Seg* segs, int nseg, uint64_t g) {
 // pass reports a matching winner below found_below.
 __device__ __forceinline__ void resolve_entry(const InstDesc& d, uint64_t e) {
     const uint64_t of = glob(d.fresh_ord)[e];
-    if (of == kEmpty) return;   // an entry a wavefront reserved and left unused (fresh_chunk)
     // a pass before the last only looks for a settled match
     if (!d.mark && !(of & kMatchBit)) return;
     const uint32_t slot = glob(d.fresh_slot)[e];
@@ -810,23 +791,82 @@ __device__ __forceinline__ uint64_t gather8(uint64_t x) {
     return (x | (x >> 28)) & 0xFFull;
 }
 
-// packs the 64 winner bytes of bitmap word w into the word and clears them;
-// returns the word's popcount
-__device__ __forceinline__ uint32_t pack_word(const InstDesc& d, uint64_t w) {
-    if (w >= d.nwords) return 0u;
+// A direct level's first pass (found_below = m0 + 1) builds the candidates up
+// to the probe's match m0, the only matching candidate among them.  When m0
+// lost, another pass builds the rest and packs the level's bytes again, so
+// this pass leaves them in place (keep).  Otherwise it packs and clears them
+// but m0's, which every packing thread reads for this test, and the scan
+// clears that byte last (scan_one).  Every other pass packs and clears.
+__device__ __forceinline__ bool keep_marks(const InstDesc& d) {
+    if (!d.direct || d.found_below == ~0ull) return false;
+    const uint64_t m0 = d.found_below - 1;
+    return !(glob(d.winbyte)[m0] == 3 && glob(d.lostbyte)[m0] == 0);
+}
+
+// packs the 64 winner bytes of bitmap word w into the word and clears them
+// (unless keep); returns the word's popcount.  A direct level's winners are
+// the candidates marked fresh and not displaced, and its lowest matching
+// winner below found_below is reported here, reduced over the wavefront
+// (every lane calls this for one word).
+__device__ __forceinline__ uint32_t pack_word(const InstDesc& d, uint64_t w, bool keep) {
     typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
-    GLB u64x2* src = (GLB u64x2*)(d.winbyte + w * 64);
-    uint64_t bits = 0;
+    uint64_t bits = 0, match = 0;
+    if (w < d.nwords) {
+        GLB u64x2* src = (GLB u64x2*)(d.winbyte + w * 64);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const u64x2 v = src[q];
-        bits |= gather8(v.x) << (16 * q);
-        bits |= gather8(v.y) << (16 * q + 8);
+        for (int q = 0; q < 4; ++q) {
+            const u64x2 v = src[q];
+            bits |= gather8(v.x) << (16 * q);
+            bits |= gather8(v.y) << (16 * q + 8);
+            if (d.direct) {
+                match |= gather8(v.x >> 1) << (16 * q);
+                match |= gather8(v.y >> 1) << (16 * q + 8);
+            }
+        }
+        if (d.direct) {
+            GLB u64x2* lost = (GLB u64x2*)(d.lostbyte + w * 64);
+            uint64_t lb = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const u64x2 v = lost[q];
+                lb |= gather8(v.x) << (16 * q);
+                lb |= gather8(v.y) << (16 * q + 8);
+            }
+            bits &= ~lb;
+            match &= bits;
+            if (!keep) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) lost[q] = u64x2{0ull, 0ull};
+            }
+        }
+        // the map starts every level cleared
+        if (!keep) {
+            u64x2 z[4] = {{0ull, 0ull}, {0ull, 0ull}, {0ull, 0ull}, {0ull, 0ull}};
+            if (d.direct && d.found_below != ~0ull && (d.found_below - 1) >> 6 == w) {
+                const uint32_t b = (uint32_t)((d.found_below - 1) & 63);   // m0 stays marked (3)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if ((int)(b >> 4) == q) {
+                        if ((b >> 3) & 1u) z[q].y = 3ull << (8 * (b & 7));
+                        else z[q].x = 3ull << (8 * (b & 7));
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) src[q] = z[q];
+        }
+        glob(d.winbits)[w] = bits;
     }
-    // the map starts every level cleared
-#pragma unroll
-    for (int q = 0; q < 4; ++q) src[q] = u64x2{0ull, 0ull};
-    glob(d.winbits)[w] = bits;
+    if (d.direct) {
+        uint64_t f = match ? w * 64 + (uint64_t)__builtin_ctzll(match) : kEmpty;
+        if (f >= d.found_below) f = kEmpty;
+        if (__ballot(f != kEmpty)) {
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint64_t o = __shfl_xor(f, off);
+                f = o < f ? o : f;
+            }
+            if ((threadIdx.x & 63) == 0) min64(glob(&d.ctr->found), f);
+        }
+    }
     return (uint32_t)__popcll(bits);
 }
 
@@ -849,10 +889,11 @@ __device__ void scan_one(const InstDesc& d) {
     __shared__ uint64_t carry;
     if (threadIdx.x == 0) carry = 0;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const bool keep = keep_marks(d);
     if (d.nblk <= (uint64_t)kSmallChunks) {
         // small bitmap: count the chunks here (one wavefront per chunk)
         for (uint64_t c = w; c < d.nblk; c += 4) {
-            uint32_t v = pack_word(d, c * kChunkWords + lane);
+            uint32_t v = pack_word(d, c * kChunkWords + lane, keep);
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
             if (lane == 0) d.blkcount[c] = v;
         }
@@ -884,6 +925,8 @@ __device__ void scan_one(const InstDesc& d) {
             rank = d.blkprefix[fc] + v;
         }
         if (threadIdx.x == 0) {
+            // the first pass's m0, kept marked while the bytes were packed
+            if (d.direct && d.found_below != ~0ull && !keep) glob(d.winbyte)[d.found_below - 1] = 0;
             d.ctr->winners = carry;
             d.ctr->found_rank = rank;
             if (d.ctr_host) {

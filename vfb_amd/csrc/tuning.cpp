@@ -41,6 +41,8 @@ const Field kFields[] = {
     {"chain_blocks", false, offsetof(vfb_tuning, chain_blocks)},
     {"chain_cands", false, offsetof(vfb_tuning, chain_cands)},
     {"lds_chain", false, offsetof(vfb_tuning, lds_chain)},
+    {"set_margin_pct", false, offsetof(vfb_tuning, set_margin_pct)},
+    {"direct_min", true, offsetof(vfb_tuning, direct_min)},
 };
 
 uint64_t get(const vfb_tuning& t, const Field& f) {
