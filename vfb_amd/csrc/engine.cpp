@@ -775,9 +775,49 @@ static void pass_work(const Inst& I, uint64_t* cand, uint64_t* bytes) {
 // the set again.  The engine with fixed scratch keeps the fresh list.
 bool Engine::direct_admits(const Inst& I) const { return !fixed_ && I.N >= tun_.direct_min; }
 
+// Whether a level is first probed (target checks only, k_check) before it
+// is built: a deduplicating level of a unit-cost search with at least
+// vfb_tuning.pass_min candidates and an operator block to split at.
+bool Engine::probed(const Inst& I) const {
+    return !I.check_only && !I.bs_mode && I.unit_cost && tun_.pass_min && I.N >= tun_.pass_min && nsplit_of(I) >= 1;
+}
+
+// keys the set makes room for ahead of a level that builds n candidates
+uint64_t Engine::set_keys_for(const Inst& I, uint64_t n) const {
+    return std::min<uint64_t>(n, std::max<uint64_t>(4096, (uint64_t)(n * std::min(1.0, I.win_ratio * set_margin()))));
+}
+
+// slots of a set for want_keys keys at load <= max_load_pct
+uint64_t Engine::set_slots_for(uint64_t want_keys) const {
+    // at least kMinSlots (or the tuned start size): the small levels of a
+    // solve then share one set, with no fill and rehash as it grows
+    const uint64_t start = std::max<uint64_t>(kMinSlots, tun_.table_min_slots);
+    const uint64_t need = next_pow2(std::max<uint64_t>(start, want_keys * 100 / tun_.max_load_pct + 1));
+    // no headroom: plain, batched and served solves size their sets alike, so
+    // their HBM-budget decisions (and hence their results) agree
+    return std::min<uint64_t>(need, 1ull << 32);
+}
+
 // fresh candidates of the level's epoch
 uint64_t Engine::fresh_exact(const Inst& I) const {
     return I.h.direct ? I.ctr.fresh_real : I.ctr.fresh_count;
+}
+
+// The set of a probed level, grown for the n candidates its next pass
+// builds.  False when the instance left the batch for want of memory; a
+// level whose set cannot grow is checked only (as prepare_or_check does),
+// and then I.check_only is set and its pass is P_FUSED.
+bool Engine::grow_for_pass(Inst& I, uint64_t n) {
+    try {
+        ensure_table(I, I.committed + set_keys_for(I, n), false);
+    } catch (CapacityError& e) {
+        if (on_capacity(I, e)) return false;
+        I.check_only = true;
+        build_units(I);
+        I.pass = P_FUSED;
+        I.usel = I.bsel = SEL_ALL;
+    }
+    return true;
 }
 
 // scratch and hash set for a deduplicating level, sized from the previous
@@ -846,22 +886,26 @@ void Engine::prepare_scratch(Inst& I) {
     h.blkcount = (uint32_t*)I.blkcount.p;
     h.blkprefix = (unsigned long long*)I.blkprefix.p;
     h.nblk = nblk;
-    const uint64_t est_new =
-        std::min<uint64_t>(N, std::max<uint64_t>(4096, (uint64_t)(N * std::min(1.0, I.win_ratio * set_margin()))));
-    ensure_table(I, I.committed + est_new, false);
+    const uint64_t want = I.committed + set_keys_for(I, N);
+    if (probed(I)) {
+        // The set grows once the probe has bounded the candidates the level
+        // builds (up to its match, or all of them), so a found level's set
+        // is sized for its first pass.  The HBM-budget decision is still
+        // that of the whole level's set, as if it were allocated here.
+        const uint64_t need = set_slots_for(want);
+        if (need > I.cap && I.used + align256(need * sizeof(Slot)) > I.budget)
+            throw CapacityError("per-solve HBM budget exhausted", false);
+        ensure_table(I, I.committed, false);
+    } else {
+        ensure_table(I, want, false);
+    }
 }
 
 // (re)builds the hash set for want_keys keys at load <= max_load_pct; keeps only keys
 // of completed epochs (ord < epoch_base)
 void Engine::ensure_table(Inst& I, uint64_t want_keys, bool force_rebuild) {
-    // at least kMinSlots (or the tuned start size): the small levels of a
-    // solve then share one set, with no fill and rehash as it grows
-    const uint64_t start = std::max<uint64_t>(kMinSlots, tun_.table_min_slots);
     const uint64_t pct = tun_.max_load_pct;
-    uint64_t need = next_pow2(std::max<uint64_t>(start, want_keys * 100 / pct + 1));
-    // no headroom: plain, batched and served solves size their sets alike, so
-    // their HBM-budget decisions (and hence their results) agree
-    if (need > (1ull << 32)) need = 1ull << 32;
+    uint64_t need = set_slots_for(want_keys);
     if (I.table.p && need <= I.cap && !force_rebuild) {
         I.h.claim_limit = I.cap * pct / 100 > I.committed ? I.cap * pct / 100 - I.committed : 0;
         return;
@@ -1345,9 +1389,8 @@ bool Engine::chain_admits(const Inst& I, uint64_t n_max) const {
         return false;
     if (I.N == 0 || I.N > n_max) return false;
     if (I.max_rows && I.N > (I.max_rows > I.stored ? I.max_rows - I.stored : 0)) return false;
-    if (tun_.pass_min && I.N >= tun_.pass_min && nsplit_of(I) >= 1) return false;   // probed first, on the host
-    const uint64_t est = std::min<uint64_t>(
-        I.N, std::max<uint64_t>(4096, (uint64_t)(I.N * std::min(1.0, I.win_ratio * set_margin()))));
+    if (probed(I)) return false;   // probed first, on the host
+    const uint64_t est = set_keys_for(I, I.N);
     const uint64_t start = std::max<uint64_t>(kMinSlots, tun_.table_min_slots);
     return start <= I.cap && (I.committed + est) * 100 / tun_.max_load_pct + 1 <= I.cap;
 }
@@ -1769,9 +1812,7 @@ void Engine::step(std::vector<Inst*>& act) {
         // list overflowed is grown and built again from its first pass ----
         for (auto* I : step) {
             // large deduplicating levels are probed first
-            const bool probe = !I->check_only && !I->bs_mode && I->unit_cost && tun_.pass_min &&
-                               I->N >= tun_.pass_min && nsplit_of(*I) >= 1;
-            I->pass = probe ? P_PROBE : P_FUSED;
+            I->pass = probed(*I) ? P_PROBE : P_FUSED;
             I->usel = I->bsel = SEL_ALL;
             I->evaluated = 0;
         }
@@ -1831,6 +1872,12 @@ void Engine::step(std::vector<Inst*>& act) {
                     I.res->probe_candidates += c;
                     // bytes counted with the kernel whose time they share
                     (last_epoch_probe_only_ ? g_kstats.bytes_probe : g_kstats.bytes_expand) += b;
+                    // the set, for the candidates the level builds next
+                    if (!grow_for_pass(I, I.ctr.found != kEmpty ? I.ctr.found + 1 : I.N)) continue;
+                    if (I.check_only) {
+                        again.push_back(&I);
+                        continue;
+                    }
                     if (I.ctr.found != kEmpty) {
                         uint8_t us = SEL_ALL, bs = SEL_ALL;
                         first_pass_for(I, I.ctr.found, &us, &bs);
@@ -1859,6 +1906,23 @@ void Engine::step(std::vector<Inst*>& act) {
                     I.pass = P_REST;
                     I.usel = SEL_ALL;
                     I.bsel = SEL_ALL;
+                    const uint64_t want = I.committed + set_keys_for(I, I.N);
+                    if (set_slots_for(want) > I.cap) {
+                        // the set was sized for the first pass: the whole
+                        // level's set, and the level built again from its
+                        // first pass (a rehash keeps completed levels' keys)
+                        if (I.h.direct) {
+                            fill(I, I.winbyte.p, I.h.nwords * 8, 0);
+                            fill(I, I.h.lostbyte, I.h.nwords * 8, 0);
+                        }
+                        if (!grow_for_pass(I, I.N)) continue;
+                        if (!I.check_only) {
+                            I.pass = P_FIRST;
+                            I.usel = I.first_usel;
+                            I.bsel = I.first_bsel;
+                        }
+                        I.evaluated = 0;
+                    }
                     again.push_back(&I);
                 }
             }

@@ -339,6 +339,10 @@ class Engine {
     bool chain_admits(const Inst& I, uint64_t n_max) const;
     bool lds_chain_admits(const Inst& I) const;
     bool direct_admits(const Inst& I) const;
+    bool probed(const Inst& I) const;
+    uint64_t set_keys_for(const Inst& I, uint64_t n) const;
+    uint64_t set_slots_for(uint64_t want_keys) const;
+    bool grow_for_pass(Inst& I, uint64_t n);
     double set_margin() const { return tun_.set_margin_pct / 100.0; }
     uint64_t fresh_exact(const Inst& I) const;
     void chain(std::vector<Inst*>& v, int G, uint64_t n_max, bool lds = false);
