@@ -134,3 +134,23 @@ def test_direct_every_level_relaxed_keys_equal_oracle(tuning, rt):
         got = V.solve(V.Instance.from_bits(names, lens, atoms, P), max_size=7, relaxed_type=rt)
         for f in ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "top_op"):
             assert got[f] == want[f], (rt, seed, f, got[f], want[f])
+
+
+def test_last_pass_grows_the_set_and_rebuilds(tuning):
+    """Config-4 instance 35 with relaxed keys 2 at max size 12, every level
+    probed: at a level of 18.6 M candidates the probe's match loses its key,
+    the set (sized for the first pass) grows for the whole level, and the
+    level is built again from its first pass (counted in retries).  Results
+    equal the OpenMP build's and the unprobed solve's."""
+    inst = V.Instance.generate(4, 35)
+    names, lens, atoms, P = inst.arrays()
+    want, _ = O.enumerate_omp(O.Instance(names, lens, atoms, P), 12, 2)
+    V.set_tuning(pass_min=0)
+    fused = V.solve(inst, max_size=12, relaxed_type=2)
+    V.set_tuning(pass_min=1)
+    print("[counters] config-4 #35 relaxed 2, every level probed", flush=True)
+    got = V.solve(inst, max_size=12, relaxed_type=2)
+    assert got["retries"] >= 1 and fused["retries"] == 0
+    for f in ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "top_op"):
+        assert got[f] == want[f], (f, got[f], want[f])
+        assert fused[f] == want[f], (f, fused[f], want[f])

@@ -1915,6 +1915,7 @@ void Engine::step(std::vector<Inst*>& act) {
                             fill(I, I.winbyte.p, I.h.nwords * 8, 0);
                             fill(I, I.h.lostbyte, I.h.nwords * 8, 0);
                         }
+                        I.res->retries++;
                         if (!grow_for_pass(I, I.N)) continue;
                         if (!I.check_only) {
                             I.pass = P_FIRST;
