@@ -899,15 +899,30 @@ __device__ void scan_one(const InstDesc& d) {
         }
     }
     __syncthreads();
-    for (uint64_t base = 0; base < d.nblk; base += 256) {
-        const uint64_t i = base + threadIdx.x;
-        const uint64_t v = i < d.nblk ? d.blkcount[i] : 0;
-        const uint64_t inc = wave_incl_scan(v);
+    // kScanPer consecutive chunks per thread: a level of 2^30 candidates has
+    // 2^18 chunks, which one chunk per thread scanned in ~1000 rounds of
+    // three barriers (2.2 ms)
+    constexpr int kScanPer = 8;
+    for (uint64_t base = 0; base < d.nblk; base += 256 * kScanPer) {
+        const uint64_t i0 = base + (uint64_t)threadIdx.x * kScanPer;
+        uint64_t v[kScanPer];
+        uint64_t tsum = 0;
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) {
+            v[j] = i0 + j < d.nblk ? d.blkcount[i0 + j] : 0;
+            tsum += v[j];
+        }
+        const uint64_t inc = wave_incl_scan(tsum);
         if (lane == 63) wsum[w] = inc;
         __syncthreads();
         uint64_t pre = carry;
         for (int k = 0; k < w; ++k) pre += wsum[k];
-        if (i < d.nblk) d.blkprefix[i] = pre + inc - v;
+        uint64_t run = pre + inc - tsum;
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) {
+            if (i0 + j < d.nblk) d.blkprefix[i0 + j] = run;
+            run += v[j];
+        }
         __syncthreads();
         if (threadIdx.x == 255) carry = pre + inc;
         __syncthreads();
