@@ -323,7 +323,7 @@ typedef struct {
                                       candidate, and the one a lower ordinal displaces, in
                                       byte maps that the count kernel packs (no resolve
                                       pass over the set); smaller levels list their new
-                                      keys and resolve them; >= 1 (default 65536) */
+                                      keys and resolve them; >= 1 (default 2^18) */
 } vfb_tuning;
 
 void vfb_tuning_default(vfb_tuning* t);

@@ -1,9 +1,9 @@
 """Large levels: winners marked during the expansion (InstDesc::direct) and the
 per-instance counters of launches over one instance.
 
-A level of 2^16 candidates and more keeps no fresh list: each fresh
-candidate marks its byte, a candidate that lowers a slot's ordinal of the
-level marks the displaced one lost, and the count kernel packs "marked and
+A level of vfb_tuning.direct_min candidates and more (default 2^18) keeps
+no fresh list: each fresh candidate marks its byte, a candidate that lowers
+a slot's ordinal of the level marks the displaced one lost, and the count kernel packs "marked and
 not lost" into the winner bitmap.  Launches over one instance add their
 claims and fresh candidates up per workgroup (InstDesc::defer_claims); a
 batch that does not park its large members (vfb_tuning.park_min = 0) builds
@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 
 FIELDS = ("status", "min_size", "formula", "distinct", "candidates", "level_winners", "level_candidates",
           "otf_level", "otf_stored", "found_ordinal", "levels_done", "top_op", "table_slots")
-DIRECT_MIN = 1 << 16
+DIRECT_MIN = V.tuning_default()["direct_min"]
 
 
 @pytest.fixture

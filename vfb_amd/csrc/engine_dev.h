@@ -44,7 +44,7 @@ constexpr int kSmallChunks = 64;      // bitmaps up to this many chunks are coun
 constexpr uint64_t kSmallFresh = 4096;  // fresh lists up to this long are resolved in the scan's workgroup
 constexpr uint64_t kMinWinWords = 1024; // winner maps are allocated for at least 64K candidates
 constexpr uint64_t kMinSlots = 1ull << 17; // hash sets start at 4 MB
-constexpr uint64_t kDirectMin = 1ull << 16;  // default vfb_tuning.direct_min (InstDesc::direct)
+constexpr uint64_t kDirectMin = 1ull << 18;  // default vfb_tuning.direct_min (InstDesc::direct)
 
 enum KeyKind : int { KEY_FIELD = 0, KEY_STRIDE = 1, KEY_MIX = 3 };
 
