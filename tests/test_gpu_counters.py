@@ -154,3 +154,37 @@ def test_last_pass_grows_the_set_and_rebuilds(tuning):
     for f in ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "top_op"):
         assert got[f] == want[f], (f, got[f], want[f])
         assert fused[f] == want[f], (f, fused[f], want[f])
+
+
+@pytest.mark.parametrize("setting", [dict(pass_min=1, direct_min=1), dict(pass_min=1 << 10, direct_min=1 << 10)])
+def test_random_sweep_direct_and_probed_equal_oracle(tuning, setting):
+    """Random instances (up to 30 traces, 5 atoms, lengths to 64, bit
+    densities 0.3-0.7) at max size 7-8 with relaxed keys 1-3, every level (or
+    every level of 1024 candidates and more) marked directly and probed
+    first: plain and batched solves against the sequential oracle."""
+    V.set_tuning(**setting)
+    F = ("status", "min_size", "formula", "distinct", "candidates", "found_ordinal", "top_op")
+    for rt in (1, 2, 3):
+        rng = random.Random(31337 + rt)
+        cases = []
+        for seed in range(40):
+            T = rng.randint(2, 30)
+            lens = [rng.randint(1, 64 if seed % 3 else 12) for _ in range(T)]
+            A = rng.randint(1, 5)
+            atoms = [[sum((rng.random() < rng.choice((0.3, 0.5, 0.7))) << j for j in range(lens[t])) for t in range(T)]
+                     for _ in range(A)]
+            P = rng.randint(1, T - 1)
+            names = [f"p{a}" for a in range(A)]
+            ms = 8 if A <= 3 else 7
+            want = O.enumerate_cs(O.Instance(names, lens, atoms, P), ms, rt)
+            inst = V.Instance.from_bits(names, lens, atoms, P)
+            got = V.solve(inst, max_size=ms, relaxed_type=rt)
+            for f in F:
+                assert got[f] == want[f], (setting, rt, seed, f, got[f], want[f])
+            cases.append((inst, ms, want))
+        print(f"[counters] sweep {setting} relaxed {rt}: plain ok, batches", flush=True)
+        for ms in (7, 8):
+            sub = [(i, w) for i, m, w in cases if m == ms]
+            for (i, w), g in zip(sub, V.solve_batch([i for i, _ in sub], max_size=ms, relaxed_type=rt)):
+                for f in F:
+                    assert g[f] == w[f], (setting, rt, ms, f, g[f], w[f])
